@@ -1,0 +1,207 @@
+/*
+ * nufi_hip.h — C ABI of the MI355X (gfx950) Numerical Flow Iteration rho path.
+ *
+ * Plain C types only: sizes are uint64_t, fields are double, every call
+ * returns an int status (NUFI_OK == 0) and the message of the last failure on
+ * a context is available from nufi_last_error(ctx).  All device memory is
+ * owned by the context; the caller owns every host pointer it passes in.
+ *
+ * Reference interface that each entry point replaces (paulwilhelmvlasov/
+ * NumericalFlowIteration, files relative to the repository root):
+ *   nufi_config              nufi/config.hpp:33-53 (dim1::config_t),
+ *                            nufi/config.hpp:167-194 (dim3::config_t)
+ *   nufi_create              nufi/cuda_scheduler.hpp:43-85 (ctors),
+ *                            nufi/cuda_kernel.cu:81-109, 468-483
+ *   nufi_compute_rho         nufi/cuda_scheduler.hpp:88-111,
+ *                            nufi/cuda_kernel.cu:112-132, 485-510
+ *   nufi_download_rho        nufi/cuda_scheduler.hpp:113-118,
+ *                            nufi/cuda_kernel.cu:134-145, 512-523
+ *   nufi_upload_phi          nufi/cuda_scheduler.hpp:120-125,
+ *                            nufi/cuda_kernel.cu:147-156, 525-536
+ *   nufi_compute_metrics     nufi/cuda_scheduler.hpp:128-151,
+ *                            nufi/cuda_kernel.cu:158-176
+ *   nufi_download_metrics    nufi/cuda_scheduler.hpp:153-157,
+ *                            nufi/cuda_kernel.cu:178-189
+ *   nufi_poisson_solve       nufi/poisson.cpp:66-89 (dim1), :328-362 (dim3)
+ *   nufi_interpolate         nufi/fields.hpp:63-142 (dim1), :352-490 (dim3)
+ *   nufi_step / nufi_run     the driver step bin/test_nufi_gpu_1d.cpp:241-253
+ *                            and bin/test_nufi_gpu_3d.cpp:150-162, made
+ *                            device-resident (no host round trip per step)
+ */
+#ifndef NUFI_HIP_H
+#define NUFI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum nufi_status {
+    NUFI_OK              = 0,
+    NUFI_ERR_INVALID     = 1, /* bad argument / config                     */
+    NUFI_ERR_RANGE       = 2, /* n > Nt, q range, too many blocks
+                                 (std::range_error in the reference)       */
+    NUFI_ERR_DEVICE      = 3, /* HIP runtime failure (cuda::exception)     */
+    NUFI_ERR_ALLOC       = 4, /* device allocation failure                 */
+    NUFI_ERR_UNSUPPORTED = 5  /* order / dim / precision not built         */
+};
+
+/* ---- precision of the characteristic trace ----------------------------- */
+enum nufi_precision {
+    NUFI_FP64  = 0, /* the plain FP64 method                                */
+    NUFI_FP32C = 1  /* FP32 trace with compensated (two-sum) position and
+                       velocity updates; Poisson, fit and rho stay FP64     */
+};
+
+/* ---- initial conditions (config_t::f0) --------------------------------- */
+enum nufi_f0_kind {
+    /* 1D families; M(u) = exp(-u^2/2)/sqrt(2 pi)                            */
+    NUFI_F0_LANDAU_1D     = 1, /* (1 + alpha cos(k x)) M(u)                 */
+    NUFI_F0_TWOSTREAM_1D  = 2, /* (1 + alpha cos(k x)) u^2 M(u)
+                                  (the dim1 default, config.hpp:80-82)      */
+    NUFI_F0_MULTIMODE_1D  = 3, /* (1 + sum_m a_m cos(m k x)) M(u), m=1..n_modes */
+    /* 3D families; M3(v) = exp(-|v|^2/2)/(2 pi)^(3/2)                        */
+    NUFI_F0_LANDAU_3D     = 11, /* (1 + alpha sum_d cos(k x_d)) M3(v)
+                                   (config.hpp:233-234)                     */
+    NUFI_F0_BUMPONTAIL_3D = 12, /* the dim3 default, config.hpp:244-246     */
+    NUFI_F0_TWOSTREAM_3D  = 13  /* config.hpp:237-240                        */
+};
+
+#define NUFI_MAX_MODES 8
+
+typedef struct nufi_f0 {
+    int32_t kind;                  /* enum nufi_f0_kind                     */
+    int32_t n_modes;               /* NUFI_F0_MULTIMODE_1D only             */
+    double  alpha;                 /* perturbation amplitude                */
+    double  k;                     /* perturbation wave number              */
+    double  a[NUFI_MAX_MODES];     /* multimode amplitudes a_1..a_n_modes   */
+} nufi_f0;
+
+/*
+ * Grid description: the fields of dim3::config_t (config.hpp:167-194); a 1D
+ * config has dim = 1 and ignores the y, z, v, w fields.  Derived fields
+ * (L*, d*, *_inv, du, dv, dw) are filled by nufi_config_derive in the order
+ * config.hpp:67-69 / 213-218 uses.
+ */
+typedef struct nufi_config {
+    int32_t  dim;                  /* 1 or 3                                */
+    int32_t  reserved;
+    uint64_t Nx, Ny, Nz;           /* grid points in physical space         */
+    uint64_t Nu, Nv, Nw;           /* velocity quadrature nodes per axis    */
+    uint64_t Nt;                   /* number of time steps                  */
+    double   dt;
+    double   x_min, x_max, y_min, y_max, z_min, z_max;
+    double   u_min, u_max, v_min, v_max, w_min, w_max;
+    double   dx, dx_inv, Lx, Lx_inv;
+    double   dy, dy_inv, Ly, Ly_inv;
+    double   dz, dz_inv, Lz, Lz_inv;
+    double   du, dv, dw;
+    nufi_f0  f0;
+} nufi_config;
+
+typedef struct nufi_ctx nufi_ctx;
+
+/* ---- library ----------------------------------------------------------- */
+const char *nufi_version(void);
+/* Fills the derived fields of *conf from the primary ones. */
+int nufi_config_derive(nufi_config *conf);
+int nufi_device_count(int *count);
+
+/* ---- context ----------------------------------------------------------- */
+/*
+ * Allocates, on `device`, the history of (Nt+1) coefficient slices of
+ * (Nx+order-1)^dim reals each, rho and metrics buffers (cuda_kernel.cu:81-109).
+ * conf_metrics may be NULL (then conf is used for metrics, as the dim3
+ * scheduler does).  Errors: NUFI_ERR_UNSUPPORTED for an order or dim that is
+ * not built, NUFI_ERR_ALLOC / NUFI_ERR_DEVICE.  On failure *out is NULL and
+ * nufi_last_error(NULL) gives the reason.
+ */
+int nufi_create(const nufi_config *conf, const nufi_config *conf_metrics,
+                int order, int precision, int device, nufi_ctx **out);
+int nufi_destroy(nufi_ctx *ctx);
+/* Message of the last failure on ctx (ctx == NULL: last failed create on
+ * this thread).  Never NULL. */
+const char *nufi_last_error(const nufi_ctx *ctx);
+/* The HIP stream (hipStream_t) every kernel of ctx runs on. */
+int nufi_get_stream(nufi_ctx *ctx, void **stream);
+int nufi_synchronize(nufi_ctx *ctx);
+
+/* ---- the reference scheduler contract ---------------------------------- */
+/*
+ * rho partials of nodes q in [q_begin, q_end) at time step n, from slices
+ * 0..n-1 of the device history.  Node q is (ix = q / Nvel, iv = q % Nvel)
+ * with Nvel = Nu (1D) or Nu*Nv*Nw with u innermost (3D).  Each node adds
+ * -dv * ftilda(n, x, v) to rho[ix] (no "+1": cuda_kernel.cu:50, 425).
+ * Asynchronous on the context stream.  NUFI_ERR_RANGE for n > Nt or
+ * q_end > total nodes; q_begin == q_end is a no-op.
+ */
+int nufi_compute_rho(nufi_ctx *ctx, uint64_t n, uint64_t q_begin, uint64_t q_end);
+/* Synchronises and ADDS the device rho (Nx*Ny*Nz reals) into rho_accum. */
+int nufi_download_rho(nufi_ctx *ctx, double *rho_accum);
+/* Copies slice n from history_base + n*stride_t (host) to the device. */
+int nufi_upload_phi(nufi_ctx *ctx, uint64_t n, const double *history_base);
+/*
+ * Metrics [int f, int f^2, kinetic energy, entropy] of nodes [q_begin,q_end)
+ * of the metrics grid at step n (with the initial half-kick, rho.hpp:63-96).
+ */
+int nufi_compute_metrics(nufi_ctx *ctx, uint64_t n, uint64_t q_begin, uint64_t q_end);
+/* Synchronises and ADDS the 4 metrics into metrics_accum. */
+int nufi_download_metrics(nufi_ctx *ctx, double *metrics_accum);
+
+/* ---- device-resident step loop ----------------------------------------- */
+/*
+ * One full time step on the device: rho(n) from slices 0..n-1, periodic
+ * Poisson solve, spline fit, slice n appended to the device history.
+ * *energy_out (may be NULL) receives the spectral electric energy
+ * (poisson.cpp:73-88) — this synchronises; with NULL the step stays async.
+ */
+int nufi_step(nufi_ctx *ctx, uint64_t n, double *energy_out);
+/* Steps n_begin..n_end-1; energies (may be NULL) gets n_end-n_begin values. */
+int nufi_run(nufi_ctx *ctx, uint64_t n_begin, uint64_t n_end, double *energies);
+/* Host copies of history slices [n_begin, n_end). */
+int nufi_download_history(nufi_ctx *ctx, uint64_t n_begin, uint64_t n_end, double *out);
+int nufi_upload_history(nufi_ctx *ctx, uint64_t n_begin, uint64_t n_end, const double *in);
+/* Energies of steps [n_begin, n_end) computed by nufi_step/nufi_run. */
+int nufi_download_energies(nufi_ctx *ctx, uint64_t n_begin, uint64_t n_end, double *out);
+
+/* ---- field solves on host buffers -------------------------------------- */
+/* In place: rho -> nodal phi; *energy (may be NULL) = spectral 1/2 int E^2. */
+int nufi_poisson_solve(nufi_ctx *ctx, double *data, double *energy);
+/* Nodal values (Nx^dim) -> padded coefficient slice ((Nx+order-1)^dim). */
+int nufi_interpolate(nufi_ctx *ctx, double *slice, const double *values);
+
+/* ---- trace kernel selection and launch shape -------------------------- */
+enum nufi_trace_variant {
+    NUFI_TRACE_FAST   = 0, /* FP64: cell-unit positions, FMA (order 4)       */
+    NUFI_TRACE_STRICT = 1, /* FP64 in the reference's operation order
+                              (rho.hpp:46-58, fields.hpp:42-60,
+                              splines.hpp:98-109), no FMA contraction        */
+    NUFI_TRACE_FP32C  = 2  /* set by precision NUFI_FP32C                     */
+};
+/* Switch an FP64 context between FAST and STRICT (no counterpart in the
+ * reference, which has one kernel; used by parity tests). */
+int nufi_set_trace_variant(nufi_ctx *ctx, int variant);
+/* Workgroup size, nodes per thread and history slices per LDS chunk of the
+ * trace kernel (what the autotuner sets); NUFI_ERR_UNSUPPORTED if the shape
+ * is not built or does not fit the LDS. */
+int nufi_set_launch_shape(nufi_ctx *ctx, int block, int nodes_per_thread, int chunk);
+int nufi_get_launch_shape(nufi_ctx *ctx, int *block, int *nodes_per_thread, int *chunk);
+
+/* ---- timing of the trace kernel ---------------------------------------- */
+/*
+ * When enabled, every trace-kernel launch is bracketed by HIP events on the
+ * context stream; nufi_trace_timing reports launches, summed kernel ms and
+ * summed characteristic steps since the last reset.
+ */
+int nufi_set_timing(nufi_ctx *ctx, int enabled);
+int nufi_trace_timing(nufi_ctx *ctx, uint64_t *launches, double *kernel_ms,
+                      double *char_steps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NUFI_HIP_H */

@@ -1,0 +1,749 @@
+// capi.hip — the C ABI of libnufi_hip.so (declared in include/nufi_hip.h).
+//
+// A context owns, on one device and one HIP stream (default priority):
+//   hist     (Nt+1) coefficient slices of (Nx+order-1)^dim reals   FP64
+//   dhist    1D: first differences of every slice (stride_d)        FP64
+//   dhist32  1D FP32C: the differences rounded to FP32
+//   hist32   3D FP32C: the coefficients rounded to FP32
+//   partials one compensated partial sum per wave of 64 velocity nodes
+//   rho, energies, the FFT twiddles and Fourier multipliers.
+// Kernels: trace1d.hip / trace3d.hip (the characteristic trace), field.hip
+// (Poisson + fit).  Reference semantics are cited at each entry point.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace nufi_impl;
+
+struct nufi_ctx
+{
+    nufi_config conf {}, conf_m {};
+    int order = 4, precision = NUFI_FP64, device = 0, dim = 1;
+    int variant = TRACE_FAST;
+    hipStream_t stream = nullptr;
+    uint32_t st = 0, sd = 0, sy = 0, sz = 0;
+    uint64_t n_space = 0, n_vel = 0, n_nodes = 0;
+    uint32_t wpx = 0;
+    int field_ok = 0;
+    int n_cu = 256;
+    size_t lds_cap = 65536;
+    launch_shape shape { 256, 1, 1 };
+
+    double *hist = nullptr, *dhist = nullptr;
+    float *dhist32 = nullptr, *hist32 = nullptr;
+    double *partials = nullptr, *rho = nullptr, *energies = nullptr, *values = nullptr;
+    double *scratch_slice = nullptr, *metrics = nullptr;
+    double2 *tw_x = nullptr, *tw_y = nullptr, *tw_z = nullptr;
+    double2 *mult_step = nullptr, *mult_pois = nullptr, *mult_fit = nullptr;
+    double *ewt = nullptr;
+
+    std::vector<double> host_rho;
+    std::string err;
+
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t,hipEvent_t>> ev;
+    size_t ev_used = 0;
+    uint64_t launches = 0;
+    double csteps = 0.0;
+};
+
+namespace
+{
+
+thread_local std::string g_create_error = "no error";
+
+int fail( nufi_ctx *c, int code, const std::string &msg )
+{
+    if ( c ) c->err = msg; else g_create_error = msg;
+    return code;
+}
+
+int hip_fail( nufi_ctx *c, hipError_t e, const char *where )
+{
+    return fail( c, e == hipErrorOutOfMemory ? NUFI_ERR_ALLOC : NUFI_ERR_DEVICE,
+                 std::string( where ) + ": " + hipGetErrorString( e ) );
+}
+
+#define HIP_TRY( ctx, expr ) do { hipError_t e_ = ( expr ); if ( e_ != hipSuccess ) return hip_fail( ( ctx ), e_, #expr ); } while ( 0 )
+
+uint32_t log2_or_none( uint64_t n )
+{
+    if ( n == 0 || ( n & ( n - 1 ) ) ) return 0xffffffffu;
+    uint32_t l = 0;
+    while ( ( uint64_t( 1 ) << l ) < n ) ++l;
+    return l;
+}
+
+// B-spline values at t = 0 of the `order` bases that are non-zero on [0,1)
+// (the collocation weights of fields.hpp:70-75), by the Cox-de Boor recurrence.
+std::vector<long double> basis_at_zero( int order )
+{
+    std::vector<long double> w( order, 0.0L );
+    w[order-1] = 1.0L;
+    const long double t = 0.0L;
+    for ( int m = 1; m < order; ++m )
+    {
+        w[order-m-1] = ( 1.0L - t ) * w[order-m];
+        for ( int s = 1 - m; s < 0; ++s )
+            w[order-1+s] = ( t - s ) * w[order-1+s] + ( ( m + 1 + s ) - t ) * w[order+s];
+        w[order-1] *= t;
+    }
+    long double f = 1.0L;
+    for ( int i = 2; i <= order - 1; ++i ) f *= i;
+    for ( auto &v : w ) v /= f;
+    return w;
+}
+
+std::vector<double2> twiddles( uint64_t L )
+{
+    std::vector<double2> tw( L ? L : 1 );
+    const long double two_pi = 6.283185307179586476925286766559L;
+    for ( uint64_t m = 0; m < L; ++m )
+    {
+        const long double a = -two_pi * (long double)m / (long double)L;
+        tw[m] = make_double2( (double)std::cos( a ), (double)std::sin( a ) );
+    }
+    return tw;
+}
+
+template <typename T>
+int dev_alloc( nufi_ctx *c, T **p, size_t count, const char *what )
+{
+    if ( count == 0 ) count = 1;
+    hipError_t e = hipMalloc( reinterpret_cast<void**>( p ), count * sizeof(T) );
+    if ( e != hipSuccess ) return hip_fail( c, e, what );
+    e = hipMemset( *p, 0, count * sizeof(T) );
+    if ( e != hipSuccess ) return hip_fail( c, e, what );
+    return NUFI_OK;
+}
+
+void free_all( nufi_ctx *c )
+{
+    void *ptrs[] = { c->hist, c->dhist, c->dhist32, c->hist32, c->partials, c->rho, c->energies,
+                     c->values, c->scratch_slice, c->metrics, c->tw_x, c->tw_y, c->tw_z,
+                     c->mult_step, c->mult_pois, c->mult_fit, c->ewt };
+    for ( void *p : ptrs ) if ( p ) (void)hipFree( p );
+    for ( auto &pr : c->ev ) { (void)hipEventDestroy( pr.first ); (void)hipEventDestroy( pr.second ); }
+    c->ev.clear();
+    if ( c->stream ) (void)hipStreamDestroy( c->stream );
+}
+
+int validate( const nufi_config *c, int order )
+{
+    if ( c == nullptr ) return NUFI_ERR_INVALID;
+    if ( c->dim != 1 && c->dim != 3 ) return NUFI_ERR_UNSUPPORTED;
+    if ( c->Nx == 0 || c->Nu == 0 || !( c->dt > 0 ) || !( c->x_max > c->x_min ) || !( c->u_max > c->u_min ) )
+        return NUFI_ERR_INVALID;
+    if ( c->dim == 3 && ( c->Ny == 0 || c->Nz == 0 || c->Nv == 0 || c->Nw == 0 ||
+                          !( c->y_max > c->y_min ) || !( c->z_max > c->z_min ) ||
+                          !( c->v_max > c->v_min ) || !( c->w_max > c->w_min ) ) )
+        return NUFI_ERR_INVALID;
+    if ( order < 3 || order > 8 ) return NUFI_ERR_UNSUPPORTED;
+    if ( c->Nx > ( 1u << 24 ) || c->Ny > ( 1u << 24 ) || c->Nz > ( 1u << 24 ) ) return NUFI_ERR_UNSUPPORTED;
+    return NUFI_OK;
+}
+
+// Host precomputation of the Fourier tables (long double).
+int build_tables( nufi_ctx *c )
+{
+    const nufi_config &g = c->conf;
+    const uint64_t Nx = g.Nx, Ny = c->dim == 3 ? g.Ny : 1, Nz = c->dim == 3 ? g.Nz : 1;
+    const uint64_t N = Nx * Ny * Nz;
+    const std::vector<long double> B = basis_at_zero( c->order );
+    const long double two_pi = 6.283185307179586476925286766559L;
+    using cld = std::complex<long double>;
+    auto symbol = [&]( uint64_t k, uint64_t L ) {
+        cld s = 0;
+        for ( int j = 0; j < c->order; ++j )
+        {
+            const long double a = two_pi * (long double)( ( uint64_t( j ) * k ) % L ) / (long double)L;
+            s += B[j] * cld( std::cos( a ), std::sin( a ) );
+        }
+        return s;
+    };
+    auto fold = []( uint64_t i, uint64_t L ) { return (long double)( ( 2*i < L ) ? i : L - i ); };
+    const long double Lx = g.Lx, Ly = c->dim == 3 ? g.Ly : 1.0L, Lz = c->dim == 3 ? g.Lz : 1.0L;
+    const long double fx = two_pi / Lx, fy = two_pi / Ly, fz = two_pi / Lz;
+    const long double vol = Lx * Ly * Lz;
+
+    std::vector<double2> ms( N ), mp( N ), mf( N );
+    std::vector<double> ew( N );
+    for ( uint64_t k = 0; k < Nz; ++k )
+    for ( uint64_t j = 0; j < Ny; ++j )
+    for ( uint64_t i = 0; i < Nx; ++i )
+    {
+        const uint64_t idx = ( k * Ny + j ) * Nx + i;
+        const cld s = symbol( i, Nx ) * symbol( j, Ny ) * symbol( k, Nz );
+        const long double ii = fold( i, Nx ) * fx, jj = fold( j, Ny ) * fy, kk = fold( k, Nz ) * fz;
+        const long double kap2 = ii*ii + jj*jj + kk*kk;
+        const bool singular = std::abs( s ) < 1e-12L;
+        const cld fit = singular ? cld( 0 ) : cld( 1 ) / ( (long double)N * s );
+        mf[idx] = make_double2( (double)fit.real(), (double)fit.imag() );
+        if ( idx == 0 )
+        {
+            ms[idx] = make_double2( 0.0, 0.0 );
+            mp[idx] = make_double2( 0.0, 0.0 );
+            ew[idx] = 0.0;
+            continue;
+        }
+        const long double pois = 1.0L / ( (long double)N * kap2 );
+        mp[idx] = make_double2( (double)pois, 0.0 );
+        const cld st = singular ? cld( 0 ) : cld( 1 ) / ( (long double)N * kap2 * s );
+        ms[idx] = make_double2( (double)st.real(), (double)st.imag() );
+        ew[idx] = (double)( ( vol / 2.0L ) / ( (long double)N * (long double)N * kap2 ) );
+    }
+    const auto tx = twiddles( Nx ), ty = twiddles( Ny ), tz = twiddles( Nz );
+    int rc;
+    if ( ( rc = dev_alloc( c, &c->tw_x, tx.size(), "twiddles" ) ) ) return rc;
+    if ( ( rc = dev_alloc( c, &c->tw_y, ty.size(), "twiddles" ) ) ) return rc;
+    if ( ( rc = dev_alloc( c, &c->tw_z, tz.size(), "twiddles" ) ) ) return rc;
+    if ( ( rc = dev_alloc( c, &c->mult_step, N, "multipliers" ) ) ) return rc;
+    if ( ( rc = dev_alloc( c, &c->mult_pois, N, "multipliers" ) ) ) return rc;
+    if ( ( rc = dev_alloc( c, &c->mult_fit, N, "multipliers" ) ) ) return rc;
+    if ( ( rc = dev_alloc( c, &c->ewt, N, "energy weights" ) ) ) return rc;
+    HIP_TRY( c, hipMemcpy( c->tw_x, tx.data(), tx.size() * sizeof(double2), hipMemcpyHostToDevice ) );
+    HIP_TRY( c, hipMemcpy( c->tw_y, ty.data(), ty.size() * sizeof(double2), hipMemcpyHostToDevice ) );
+    HIP_TRY( c, hipMemcpy( c->tw_z, tz.data(), tz.size() * sizeof(double2), hipMemcpyHostToDevice ) );
+    HIP_TRY( c, hipMemcpy( c->mult_step, ms.data(), N * sizeof(double2), hipMemcpyHostToDevice ) );
+    HIP_TRY( c, hipMemcpy( c->mult_pois, mp.data(), N * sizeof(double2), hipMemcpyHostToDevice ) );
+    HIP_TRY( c, hipMemcpy( c->mult_fit, mf.data(), N * sizeof(double2), hipMemcpyHostToDevice ) );
+    HIP_TRY( c, hipMemcpy( c->ewt, ew.data(), N * sizeof(double), hipMemcpyHostToDevice ) );
+    return NUFI_OK;
+}
+
+void default_shape( nufi_ctx *c )
+{
+    const uint64_t waves = c->n_space * c->wpx;
+    const size_t elem = ( c->variant == TRACE_FP32C ) ? sizeof(float) : sizeof(double);
+    if ( c->dim == 1 )
+    {
+        int block = 1024;
+        while ( block > 64 && waves / uint64_t( block / 64 ) < uint64_t( c->n_cu ) ) block /= 2;
+        if ( c->variant == TRACE_STRICT )
+            block = block >= 1024 ? 1024 : ( block >= 256 ? 256 : 64 );
+        const size_t budget = std::min<size_t>( c->lds_cap, 64 * 1024 );
+        int chunk = int( budget / ( size_t( c->sd ) * elem ) );
+        chunk = std::max( 1, std::min( chunk, 64 ) );
+        c->shape = launch_shape { block, 1, chunk };
+    }
+    else
+    {
+        int block = c->variant == TRACE_STRICT ? 256 : 256;
+        const size_t per = size_t( c->st ) * elem;
+        int chunk = int( std::min<size_t>( c->lds_cap, 64 * 1024 ) / per );
+        chunk = std::max( 1, std::min( chunk, 8 ) );
+        c->shape = launch_shape { block, 1, chunk };
+    }
+}
+
+int begin_timing( nufi_ctx *c, hipEvent_t *stop )
+{
+    *stop = nullptr;
+    if ( !c->timing ) return NUFI_OK;
+    if ( c->ev_used == c->ev.size() )
+    {
+        hipEvent_t a, b;
+        HIP_TRY( c, hipEventCreate( &a ) );
+        HIP_TRY( c, hipEventCreate( &b ) );
+        c->ev.emplace_back( a, b );
+    }
+    auto &pr = c->ev[ c->ev_used++ ];
+    HIP_TRY( c, hipEventRecord( pr.first, c->stream ) );
+    *stop = pr.second;
+    return NUFI_OK;
+}
+
+// rho partials of nodes [q_begin, q_end) at step n (no reduction).
+int launch_trace( nufi_ctx *c, uint64_t n, uint64_t q_begin, uint64_t q_end, uint64_t *first, uint64_t *count )
+{
+    const uint64_t l0 = q_begin / c->n_vel, l1 = ( q_end - 1 ) / c->n_vel;
+    *first = l0; *count = l1 - l0 + 1;
+    const uint64_t tasks = ( l1 - l0 + 1 ) * c->wpx;
+    if ( tasks >= ( uint64_t( 1 ) << 32 ) ) return fail( c, NUFI_ERR_RANGE, "compute_rho: too many wave tasks for one launch" );
+    hipEvent_t stop;
+    int rc = begin_timing( c, &stop );
+    if ( rc ) return rc;
+    hipError_t e;
+    if ( c->dim == 1 )
+    {
+        trace1_params p {};
+        p.dhist = c->dhist; p.dhist32 = c->dhist32; p.partials = c->partials;
+        p.n = n; p.q_begin = q_begin; p.q_end = q_end;
+        p.Nx = uint32_t( c->conf.Nx ); p.Nu = uint32_t( c->conf.Nu ); p.stride_d = c->sd; p.wpx = c->wpx;
+        p.ix_first = uint32_t( l0 ); p.n_tasks = uint32_t( tasks );
+        p.order = uint32_t( c->order );
+        p.x_min = c->conf.x_min; p.dx = c->conf.dx; p.dx_inv = c->conf.dx_inv;
+        p.Lx = c->conf.Lx; p.Lx_inv = c->conf.Lx_inv;
+        p.u_min = c->conf.u_min; p.du = c->conf.du; p.dt = c->conf.dt;
+        p.f0 = c->conf.f0;
+        e = launch_trace1d( p, c->variant, c->shape, c->stream );
+    }
+    else
+    {
+        trace3_params p {};
+        p.hist = c->hist; p.hist32 = c->hist32; p.partials = c->partials;
+        p.n = n; p.q_begin = q_begin; p.q_end = q_end;
+        p.Nx = uint32_t( c->conf.Nx ); p.Ny = uint32_t( c->conf.Ny ); p.Nz = uint32_t( c->conf.Nz );
+        p.Nu = uint32_t( c->conf.Nu ); p.Nv = uint32_t( c->conf.Nv ); p.Nw = uint32_t( c->conf.Nw );
+        p.sy = c->sy; p.sz = c->sz; p.st = c->st; p.wpx = c->wpx;
+        p.l_first = uint32_t( l0 ); p.n_tasks = uint32_t( tasks );
+        p.order = uint32_t( c->order );
+        const nufi_config &g = c->conf;
+        p.x_min = g.x_min; p.y_min = g.y_min; p.z_min = g.z_min;
+        p.dx = g.dx; p.dy = g.dy; p.dz = g.dz;
+        p.dx_inv = g.dx_inv; p.dy_inv = g.dy_inv; p.dz_inv = g.dz_inv;
+        p.Lx = g.Lx; p.Ly = g.Ly; p.Lz = g.Lz;
+        p.Lx_inv = g.Lx_inv; p.Ly_inv = g.Ly_inv; p.Lz_inv = g.Lz_inv;
+        p.u_min = g.u_min; p.v_min = g.v_min; p.w_min = g.w_min;
+        p.du = g.du; p.dv = g.dv; p.dw = g.dw; p.dt = g.dt;
+        p.f0 = g.f0;
+        e = launch_trace3d( p, c->variant, c->shape, c->stream );
+    }
+    if ( e != hipSuccess ) return hip_fail( c, e, "trace kernel launch" );
+    if ( stop ) HIP_TRY( c, hipEventRecord( stop, c->stream ) );
+    c->launches += 1;
+    c->csteps += double( q_end - q_begin ) * double( n );
+    return NUFI_OK;
+}
+
+double rho_weight( const nufi_ctx *c )
+{
+    // -du per node (cuda_kernel.cu:47-50); 3D: -du*dv*dw (cuda_kernel.cu:419-425)
+    if ( c->dim == 1 ) return -c->conf.du;
+    return -( c->conf.du * c->conf.dv * c->conf.dw );
+}
+
+field_params base_field( nufi_ctx *c )
+{
+    field_params f {};
+    f.dim = c->dim; f.order = c->order;
+    f.Nx = uint32_t( c->conf.Nx );
+    f.Ny = c->dim == 3 ? uint32_t( c->conf.Ny ) : 1u;
+    f.Nz = c->dim == 3 ? uint32_t( c->conf.Nz ) : 1u;
+    f.N = f.Nx * f.Ny * f.Nz;
+    f.log2x = log2_or_none( f.Nx ); f.log2y = log2_or_none( f.Ny ); f.log2z = log2_or_none( f.Nz );
+    f.wpx = c->wpx;
+    f.weight = rho_weight( c );
+    f.tw_x = c->tw_x; f.tw_y = c->tw_y; f.tw_z = c->tw_z;
+    f.ewt = c->ewt;
+    f.sy = c->sy; f.sz = c->sz; f.stride_d = c->sd;
+    return f;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *nufi_version( void ) { return "nufi-mi355x 0.1 (gfx950)"; }
+
+int nufi_config_derive( nufi_config *c )
+{
+    if ( c == nullptr || c->Nx == 0 || c->Nu == 0 ) return NUFI_ERR_INVALID;
+    // config.hpp:67-69 / 213-218
+    c->Lx = c->x_max - c->x_min; c->Lx_inv = 1 / c->Lx;
+    c->dx = c->Lx / c->Nx;       c->dx_inv = 1 / c->dx;
+    c->du = ( c->u_max - c->u_min ) / c->Nu;
+    if ( c->dim == 3 )
+    {
+        if ( c->Ny == 0 || c->Nz == 0 || c->Nv == 0 || c->Nw == 0 ) return NUFI_ERR_INVALID;
+        c->Ly = c->y_max - c->y_min; c->Ly_inv = 1 / c->Ly;
+        c->Lz = c->z_max - c->z_min; c->Lz_inv = 1 / c->Lz;
+        c->dy = c->Ly / c->Ny;       c->dy_inv = 1 / c->dy;
+        c->dz = c->Lz / c->Nz;       c->dz_inv = 1 / c->dz;
+        c->dv = ( c->v_max - c->v_min ) / c->Nv;
+        c->dw = ( c->w_max - c->w_min ) / c->Nw;
+    }
+    return NUFI_OK;
+}
+
+int nufi_device_count( int *count )
+{
+    if ( count == nullptr ) return NUFI_ERR_INVALID;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount( &n );
+    if ( e != hipSuccess ) { *count = 0; return fail( nullptr, NUFI_ERR_DEVICE, hipGetErrorString( e ) ); }
+    *count = n;
+    return NUFI_OK;
+}
+
+int nufi_create( const nufi_config *conf, const nufi_config *conf_metrics, int order, int precision,
+                 int device, nufi_ctx **out )
+{
+    if ( out == nullptr ) return fail( nullptr, NUFI_ERR_INVALID, "nufi_create: out is NULL" );
+    *out = nullptr;
+    int rc = validate( conf, order );
+    if ( rc ) return fail( nullptr, rc, "nufi_create: invalid or unsupported config/order" );
+    if ( conf_metrics && validate( conf_metrics, order ) )
+        return fail( nullptr, NUFI_ERR_INVALID, "nufi_create: invalid metrics config" );
+    if ( precision != NUFI_FP64 && precision != NUFI_FP32C )
+        return fail( nullptr, NUFI_ERR_UNSUPPORTED, "nufi_create: unknown precision" );
+    if ( precision == NUFI_FP32C && order != 4 )
+        return fail( nullptr, NUFI_ERR_UNSUPPORTED, "nufi_create: FP32C is built for order 4 only" );
+    if ( conf->dim == 3 && order != 4 )
+        return fail( nullptr, NUFI_ERR_UNSUPPORTED, "nufi_create: 3D is built for order 4 only" );
+
+    int ndev = 0;
+    hipError_t he = hipGetDeviceCount( &ndev );
+    if ( he != hipSuccess || ndev == 0 )
+        return fail( nullptr, NUFI_ERR_DEVICE, "nufi_create: no HIP device" );
+    if ( device < 0 || device >= ndev ) return fail( nullptr, NUFI_ERR_INVALID, "nufi_create: bad device index" );
+
+    nufi_ctx *c = new ( std::nothrow ) nufi_ctx;
+    if ( c == nullptr ) return fail( nullptr, NUFI_ERR_ALLOC, "nufi_create: host allocation" );
+    c->conf = *conf;
+    c->conf_m = conf_metrics ? *conf_metrics : *conf;
+    c->order = order; c->precision = precision; c->device = device; c->dim = conf->dim;
+    c->variant = precision == NUFI_FP32C ? TRACE_FP32C : ( ( c->dim == 1 && order == 4 ) || c->dim == 3 ? TRACE_FAST : TRACE_STRICT );
+
+    auto bail = [&]( int code ) { std::string m = c->err; free_all( c ); delete c; return fail( nullptr, code, m ); };
+
+    if ( ( he = hipSetDevice( device ) ) != hipSuccess ) { hip_fail( c, he, "hipSetDevice" ); return bail( NUFI_ERR_DEVICE ); }
+    if ( ( he = hipStreamCreateWithFlags( &c->stream, hipStreamNonBlocking ) ) != hipSuccess )
+    { hip_fail( c, he, "hipStreamCreate" ); return bail( NUFI_ERR_DEVICE ); }
+    hipDeviceProp_t prop;
+    if ( hipGetDeviceProperties( &prop, device ) == hipSuccess )
+    {
+        c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        int lds = 0;
+        if ( hipDeviceGetAttribute( &lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device ) == hipSuccess && lds > 0 )
+            c->lds_cap = size_t( lds );
+    }
+
+    const uint64_t o1 = uint64_t( order ) - 1;
+    if ( c->dim == 1 )
+    {
+        c->st = uint32_t( conf->Nx + o1 );
+        // extended differences D[-1] .. D[Nx+order-2]: Nx+order entries, padded
+        // to a multiple of 4 so every slice starts 16-byte aligned
+        c->sd = uint32_t( ( conf->Nx + order + 3 ) & ~uint64_t( 3 ) );
+        c->sy = c->st; c->sz = c->st;
+        c->n_space = conf->Nx; c->n_vel = conf->Nu;
+    }
+    else
+    {
+        c->sy = uint32_t( conf->Nx + o1 );
+        c->sz = uint32_t( c->sy * ( conf->Ny + o1 ) );
+        c->st = uint32_t( c->sz * ( conf->Nz + o1 ) );
+        c->sd = c->st;
+        c->n_space = conf->Nx * conf->Ny * conf->Nz;
+        c->n_vel = conf->Nu * conf->Nv * conf->Nw;
+    }
+    c->n_nodes = c->n_space * c->n_vel;
+    c->wpx = uint32_t( ( c->n_vel + 63 ) / 64 );
+    c->field_ok = field_supported( c->dim, uint32_t( conf->Nx ), c->dim == 3 ? uint32_t( conf->Ny ) : 1u,
+                                   c->dim == 3 ? uint32_t( conf->Nz ) : 1u );
+    const uint64_t slices = conf->Nt + 1;
+    if ( ( rc = dev_alloc( c, &c->hist, slices * c->st + 8, "history" ) ) ) return bail( rc );
+    if ( c->dim == 1 && ( rc = dev_alloc( c, &c->dhist, slices * c->sd + 8, "difference history" ) ) ) return bail( rc );
+    if ( precision == NUFI_FP32C )
+    {
+        if ( c->dim == 1 && ( rc = dev_alloc( c, &c->dhist32, slices * c->sd + 8, "FP32 history" ) ) ) return bail( rc );
+        if ( c->dim == 3 && ( rc = dev_alloc( c, &c->hist32, slices * c->st + 8, "FP32 history" ) ) ) return bail( rc );
+    }
+    if ( ( rc = dev_alloc( c, &c->partials, c->n_space * c->wpx, "partials" ) ) ) return bail( rc );
+    if ( ( rc = dev_alloc( c, &c->rho, c->n_space, "rho" ) ) ) return bail( rc );
+    if ( ( rc = dev_alloc( c, &c->values, c->n_space, "values" ) ) ) return bail( rc );
+    if ( ( rc = dev_alloc( c, &c->energies, slices, "energies" ) ) ) return bail( rc );
+    if ( ( rc = dev_alloc( c, &c->scratch_slice, c->st + 8, "scratch slice" ) ) ) return bail( rc );
+    if ( ( rc = dev_alloc( c, &c->metrics, 8, "metrics" ) ) ) return bail( rc );
+    if ( c->field_ok && ( rc = build_tables( c ) ) ) return bail( rc );
+    c->host_rho.assign( c->n_space, 0.0 );
+    default_shape( c );
+    *out = c;
+    return NUFI_OK;
+}
+
+int nufi_destroy( nufi_ctx *c )
+{
+    if ( c == nullptr ) return NUFI_OK;
+    (void)hipSetDevice( c->device );
+    if ( c->stream ) (void)hipStreamSynchronize( c->stream );
+    free_all( c );
+    delete c;
+    return NUFI_OK;
+}
+
+const char *nufi_last_error( const nufi_ctx *c )
+{
+    if ( c == nullptr ) return g_create_error.c_str();
+    return c->err.empty() ? "no error" : c->err.c_str();
+}
+
+int nufi_get_stream( nufi_ctx *c, void **stream )
+{
+    if ( c == nullptr || stream == nullptr ) return NUFI_ERR_INVALID;
+    *stream = reinterpret_cast<void*>( c->stream );
+    return NUFI_OK;
+}
+
+int nufi_synchronize( nufi_ctx *c )
+{
+    if ( c == nullptr ) return NUFI_ERR_INVALID;
+    HIP_TRY( c, hipSetDevice( c->device ) );
+    HIP_TRY( c, hipStreamSynchronize( c->stream ) );
+    return NUFI_OK;
+}
+
+// cuda_kernel.cu:112-132 / 485-510
+int nufi_compute_rho( nufi_ctx *c, uint64_t n, uint64_t q_begin, uint64_t q_end )
+{
+    if ( c == nullptr ) return NUFI_ERR_INVALID;
+    if ( n > c->conf.Nt ) return fail( c, NUFI_ERR_RANGE, "Time-step out of range." );
+    if ( q_begin > q_end || q_end > c->n_nodes ) return fail( c, NUFI_ERR_RANGE, "compute_rho: node range out of range" );
+    HIP_TRY( c, hipSetDevice( c->device ) );
+    if ( q_begin == q_end )
+    {
+        // empty range: the device rho becomes zero, so download_rho adds nothing
+        hipError_t e = launch_reduce_partials( c->partials, c->wpx, 0.0, 0, 0, uint32_t( c->n_space ), c->rho, c->stream );
+        if ( e != hipSuccess ) return hip_fail( c, e, "reduce launch" );
+        return NUFI_OK;
+    }
+    uint64_t first, count;
+    int rc = launch_trace( c, n, q_begin, q_end, &first, &count );
+    if ( rc ) return rc;
+    hipError_t e = launch_reduce_partials( c->partials, c->wpx, rho_weight( c ), uint32_t( first ), uint32_t( count ),
+                                           uint32_t( c->n_space ), c->rho, c->stream );
+    if ( e != hipSuccess ) return hip_fail( c, e, "reduce launch" );
+    return NUFI_OK;
+}
+
+// cuda_kernel.cu:134-145 — synchronising copy, then rho[i] += partial
+int nufi_download_rho( nufi_ctx *c, double *rho_accum )
+{
+    if ( c == nullptr || rho_accum == nullptr ) return NUFI_ERR_INVALID;
+    HIP_TRY( c, hipSetDevice( c->device ) );
+    HIP_TRY( c, hipMemcpyAsync( c->host_rho.data(), c->rho, c->n_space * sizeof(double), hipMemcpyDeviceToHost, c->stream ) );
+    HIP_TRY( c, hipStreamSynchronize( c->stream ) );
+    for ( uint64_t i = 0; i < c->n_space; ++i ) rho_accum[i] += c->host_rho[i];
+    return NUFI_OK;
+}
+
+// cuda_kernel.cu:147-156 — slice n from history_base + n*stride_t
+int nufi_upload_phi( nufi_ctx *c, uint64_t n, const double *history_base )
+{
+    if ( c == nullptr || history_base == nullptr ) return NUFI_ERR_INVALID;
+    if ( n > c->conf.Nt ) return fail( c, NUFI_ERR_RANGE, "upload_phi: time step out of range" );
+    HIP_TRY( c, hipSetDevice( c->device ) );
+    double *dst = c->hist + n * c->st;
+    HIP_TRY( c, hipMemcpyAsync( dst, history_base + n * c->st, c->st * sizeof(double), hipMemcpyHostToDevice, c->stream ) );
+    hipError_t e = launch_slice_tables( c->dim, dst, c->st, uint32_t( c->conf.Nx ),
+                                        c->dhist ? c->dhist + n * c->sd : nullptr,
+                                        c->dhist32 ? c->dhist32 + n * c->sd : nullptr,
+                                        c->hist32 ? c->hist32 + n * c->st : nullptr, c->stream );
+    if ( e != hipSuccess ) return hip_fail( c, e, "slice table launch" );
+    // the copy reads host memory the caller may reuse: complete it before returning
+    HIP_TRY( c, hipStreamSynchronize( c->stream ) );
+    return NUFI_OK;
+}
+
+int nufi_compute_metrics( nufi_ctx *c, uint64_t, uint64_t, uint64_t )
+{
+    if ( c == nullptr ) return NUFI_ERR_INVALID;
+    return fail( c, NUFI_ERR_UNSUPPORTED, "compute_metrics: not built yet" );
+}
+
+int nufi_download_metrics( nufi_ctx *c, double * )
+{
+    if ( c == nullptr ) return NUFI_ERR_INVALID;
+    return fail( c, NUFI_ERR_UNSUPPORTED, "download_metrics: not built yet" );
+}
+
+static int step_async( nufi_ctx *c, uint64_t n )
+{
+    if ( n > c->conf.Nt ) return fail( c, NUFI_ERR_RANGE, "step: time step out of range" );
+    if ( !c->field_ok ) return fail( c, NUFI_ERR_UNSUPPORTED, "step: grid too large for the single-workgroup field solve" );
+    uint64_t first, count;
+    int rc = launch_trace( c, n, 0, c->n_nodes, &first, &count );
+    if ( rc ) return rc;
+    field_params f = base_field( c );
+    f.mode = FIELD_STEP;
+    f.partials = c->partials;
+    f.mult = c->mult_step;
+    f.slice = c->hist + n * c->st;
+    f.dslice = c->dhist ? c->dhist + n * c->sd : nullptr;
+    f.dslice32 = c->dhist32 ? c->dhist32 + n * c->sd : nullptr;
+    f.slice32 = c->hist32 ? c->hist32 + n * c->st : nullptr;
+    f.energy = c->energies + n;
+    f.rho_out = c->rho;
+    hipError_t e = launch_field( f, c->stream );
+    if ( e != hipSuccess ) return hip_fail( c, e, "field kernel launch" );
+    return NUFI_OK;
+}
+
+int nufi_step( nufi_ctx *c, uint64_t n, double *energy_out )
+{
+    if ( c == nullptr ) return NUFI_ERR_INVALID;
+    HIP_TRY( c, hipSetDevice( c->device ) );
+    int rc = step_async( c, n );
+    if ( rc ) return rc;
+    if ( energy_out )
+    {
+        HIP_TRY( c, hipMemcpyAsync( energy_out, c->energies + n, sizeof(double), hipMemcpyDeviceToHost, c->stream ) );
+        HIP_TRY( c, hipStreamSynchronize( c->stream ) );
+    }
+    return NUFI_OK;
+}
+
+int nufi_run( nufi_ctx *c, uint64_t n_begin, uint64_t n_end, double *energies )
+{
+    if ( c == nullptr ) return NUFI_ERR_INVALID;
+    if ( n_end < n_begin || ( n_end > 0 && n_end - 1 > c->conf.Nt ) ) return fail( c, NUFI_ERR_RANGE, "run: step range out of range" );
+    HIP_TRY( c, hipSetDevice( c->device ) );
+    for ( uint64_t n = n_begin; n < n_end; ++n )
+    {
+        int rc = step_async( c, n );
+        if ( rc ) return rc;
+    }
+    if ( energies && n_end > n_begin )
+    {
+        HIP_TRY( c, hipMemcpyAsync( energies, c->energies + n_begin, ( n_end - n_begin ) * sizeof(double),
+                                    hipMemcpyDeviceToHost, c->stream ) );
+        HIP_TRY( c, hipStreamSynchronize( c->stream ) );
+    }
+    return NUFI_OK;
+}
+
+int nufi_download_history( nufi_ctx *c, uint64_t n_begin, uint64_t n_end, double *out )
+{
+    if ( c == nullptr || out == nullptr ) return NUFI_ERR_INVALID;
+    if ( n_end < n_begin || n_end > c->conf.Nt + 1 ) return fail( c, NUFI_ERR_RANGE, "download_history: range" );
+    HIP_TRY( c, hipSetDevice( c->device ) );
+    HIP_TRY( c, hipMemcpyAsync( out, c->hist + n_begin * c->st, ( n_end - n_begin ) * c->st * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream ) );
+    HIP_TRY( c, hipStreamSynchronize( c->stream ) );
+    return NUFI_OK;
+}
+
+int nufi_upload_history( nufi_ctx *c, uint64_t n_begin, uint64_t n_end, const double *in )
+{
+    if ( c == nullptr || in == nullptr ) return NUFI_ERR_INVALID;
+    if ( n_end < n_begin || n_end > c->conf.Nt + 1 ) return fail( c, NUFI_ERR_RANGE, "upload_history: range" );
+    for ( uint64_t n = n_begin; n < n_end; ++n )
+    {
+        int rc = nufi_upload_phi( c, n, in - n_begin * c->st );
+        if ( rc ) return rc;
+    }
+    return NUFI_OK;
+}
+
+int nufi_download_energies( nufi_ctx *c, uint64_t n_begin, uint64_t n_end, double *out )
+{
+    if ( c == nullptr || out == nullptr ) return NUFI_ERR_INVALID;
+    if ( n_end < n_begin || n_end > c->conf.Nt + 1 ) return fail( c, NUFI_ERR_RANGE, "download_energies: range" );
+    HIP_TRY( c, hipSetDevice( c->device ) );
+    HIP_TRY( c, hipMemcpyAsync( out, c->energies + n_begin, ( n_end - n_begin ) * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream ) );
+    HIP_TRY( c, hipStreamSynchronize( c->stream ) );
+    return NUFI_OK;
+}
+
+// poisson.cpp:66-89 / 328-362 on a host buffer
+int nufi_poisson_solve( nufi_ctx *c, double *data, double *energy )
+{
+    if ( c == nullptr || data == nullptr ) return NUFI_ERR_INVALID;
+    if ( !c->field_ok ) return fail( c, NUFI_ERR_UNSUPPORTED, "poisson_solve: grid too large" );
+    HIP_TRY( c, hipSetDevice( c->device ) );
+    HIP_TRY( c, hipMemcpyAsync( c->values, data, c->n_space * sizeof(double), hipMemcpyHostToDevice, c->stream ) );
+    field_params f = base_field( c );
+    f.mode = FIELD_POISSON; f.values = c->values; f.mult = c->mult_pois; f.energy = c->metrics;
+    hipError_t e = launch_field( f, c->stream );
+    if ( e != hipSuccess ) return hip_fail( c, e, "field kernel launch" );
+    HIP_TRY( c, hipMemcpyAsync( data, c->values, c->n_space * sizeof(double), hipMemcpyDeviceToHost, c->stream ) );
+    double en = 0;
+    HIP_TRY( c, hipMemcpyAsync( &en, c->metrics, sizeof(double), hipMemcpyDeviceToHost, c->stream ) );
+    HIP_TRY( c, hipStreamSynchronize( c->stream ) );
+    if ( energy ) *energy = en;
+    return NUFI_OK;
+}
+
+// fields.hpp:63-142 / 352-490 on host buffers
+int nufi_interpolate( nufi_ctx *c, double *slice, const double *values )
+{
+    if ( c == nullptr || slice == nullptr || values == nullptr ) return NUFI_ERR_INVALID;
+    if ( !c->field_ok ) return fail( c, NUFI_ERR_UNSUPPORTED, "interpolate: grid too large" );
+    HIP_TRY( c, hipSetDevice( c->device ) );
+    HIP_TRY( c, hipMemcpyAsync( c->values, values, c->n_space * sizeof(double), hipMemcpyHostToDevice, c->stream ) );
+    field_params f = base_field( c );
+    f.mode = FIELD_FIT; f.values = c->values; f.mult = c->mult_fit; f.slice = c->scratch_slice;
+    hipError_t e = launch_field( f, c->stream );
+    if ( e != hipSuccess ) return hip_fail( c, e, "field kernel launch" );
+    HIP_TRY( c, hipMemcpyAsync( slice, c->scratch_slice, c->st * sizeof(double), hipMemcpyDeviceToHost, c->stream ) );
+    HIP_TRY( c, hipStreamSynchronize( c->stream ) );
+    return NUFI_OK;
+}
+
+int nufi_set_trace_variant( nufi_ctx *c, int variant )
+{
+    if ( c == nullptr ) return NUFI_ERR_INVALID;
+    if ( c->precision == NUFI_FP32C )
+        return variant == TRACE_FP32C ? NUFI_OK : fail( c, NUFI_ERR_UNSUPPORTED, "FP32C context: variant is fixed" );
+    if ( variant != TRACE_FAST && variant != TRACE_STRICT )
+        return fail( c, NUFI_ERR_UNSUPPORTED, "set_trace_variant: FP64 contexts take FAST or STRICT" );
+    if ( variant == TRACE_FAST && c->order != 4 )
+        return fail( c, NUFI_ERR_UNSUPPORTED, "set_trace_variant: FAST is built for order 4 only" );
+    c->variant = variant;
+    default_shape( c );
+    return NUFI_OK;
+}
+
+int nufi_set_launch_shape( nufi_ctx *c, int block, int npt, int chunk )
+{
+    if ( c == nullptr ) return NUFI_ERR_INVALID;
+    const launch_shape s { block, npt, chunk };
+    const bool ok = c->dim == 1 ? trace1d_shape_ok( c->variant, c->order, s )
+                                : trace3d_shape_ok( c->variant, c->order, s );
+    const size_t elem = ( c->variant == TRACE_FP32C ) ? sizeof(float) : sizeof(double);
+    const size_t lds = size_t( chunk ) * ( c->dim == 1 ? c->sd : c->st ) * elem;
+    if ( !ok || lds > c->lds_cap || lds > 160u * 1024u )
+        return fail( c, NUFI_ERR_UNSUPPORTED, "set_launch_shape: shape not built or LDS too large" );
+    c->shape = s;
+    return NUFI_OK;
+}
+
+int nufi_get_launch_shape( nufi_ctx *c, int *block, int *npt, int *chunk )
+{
+    if ( c == nullptr ) return NUFI_ERR_INVALID;
+    if ( block ) *block = c->shape.block;
+    if ( npt ) *npt = c->shape.npt;
+    if ( chunk ) *chunk = c->shape.chunk;
+    return NUFI_OK;
+}
+
+int nufi_set_timing( nufi_ctx *c, int enabled )
+{
+    if ( c == nullptr ) return NUFI_ERR_INVALID;
+    HIP_TRY( c, hipSetDevice( c->device ) );
+    HIP_TRY( c, hipStreamSynchronize( c->stream ) );
+    c->timing = enabled != 0;
+    c->ev_used = 0; c->launches = 0; c->csteps = 0.0;
+    return NUFI_OK;
+}
+
+int nufi_trace_timing( nufi_ctx *c, uint64_t *launches, double *kernel_ms, double *char_steps )
+{
+    if ( c == nullptr ) return NUFI_ERR_INVALID;
+    HIP_TRY( c, hipSetDevice( c->device ) );
+    HIP_TRY( c, hipStreamSynchronize( c->stream ) );
+    double ms = 0.0;
+    for ( size_t i = 0; i < c->ev_used; ++i )
+    {
+        float t = 0.0f;
+        HIP_TRY( c, hipEventElapsedTime( &t, c->ev[i].first, c->ev[i].second ) );
+        ms += t;
+    }
+    if ( launches ) *launches = c->launches;
+    if ( kernel_ms ) *kernel_ms = ms;
+    if ( char_steps ) *char_steps = c->csteps;
+    return NUFI_OK;
+}
+
+} // extern "C"

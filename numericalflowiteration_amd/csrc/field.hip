@@ -1,0 +1,293 @@
+// field.hip — the per-step field solve on the device: rho -> periodic Poisson
+// solve -> spline fit -> coefficient slice n, in one LDS-resident workgroup.
+//
+// Reference: poisson<double>::solve (nufi/poisson.cpp:66-89 dim1, :328-362
+// dim3) runs a DHT, scales mode k by 1/(N |kappa_k|^2) with the folded index
+// (2i < N ? i : N-i), accumulates the spectral energy, zeroes mode 0 and
+// transforms back; dim{1,3}::interpolate (nufi/fields.hpp:63-142, 352-490)
+// then solves the circulant collocation system sum_j B_j(0) c[i+j] = phi_i
+// by LSMR and pads the slice periodically.
+//
+// Both operators are diagonal in Fourier space, so here they are one
+// multiplier: c_hat_k = rho_hat_k / (N kappa_k^2 s_k) with the collocation
+// symbol s_k = prod_axes sum_j B_j(0) e^{+2 pi i j k / N_axis} (exact solve;
+// the pseudo-inverse where s_k = 0, which is what LSMR from the zero-filled
+// slot converges to).  A real, folded multiplier commutes with the Hartley
+// kernel, so DHT-scale-DHT equals FFT-scale-IFFT and the spectral energy
+// sum_k H_k^2 w_k equals sum_k |F_k|^2 w_k.  The transform is a batched
+// radix-2 Stockham FFT along each axis (a direct DFT for an axis whose length
+// is not a power of two), ping-ponged between two LDS buffers.
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace nufi_impl
+{
+
+using namespace nufi_dev;
+
+namespace
+{
+
+constexpr int FIELD_BLOCK = 512;
+
+__device__ __forceinline__ double2 cmul( double2 a, double2 b )
+{
+    return make_double2( a.x*b.x - a.y*b.y, a.x*b.y + a.y*b.x );
+}
+
+struct axis_desc
+{
+    uint32_t L, log2L, S;       // length, log2 (or ~0u), stride
+    uint32_t nlines;
+    uint32_t mode;              // 0: x lines, 1: y lines, 2: z lines
+    uint32_t Nx, Ny;
+};
+
+__device__ __forceinline__ uint32_t line_base( const axis_desc &a, uint32_t line )
+{
+    if ( a.mode == 0 ) return line * a.Nx;
+    if ( a.mode == 1 ) return ( line / a.Nx ) * a.Nx * a.Ny + ( line % a.Nx );
+    return line;
+}
+
+// Transform all lines along one axis; returns the buffer holding the result.
+__device__ double2 *fft_axis( double2 *in, double2 *out, const axis_desc &a, const double2 *tw, bool inverse )
+{
+    const uint32_t L = a.L;
+    if ( L <= 1 ) return in;
+    if ( a.log2L == 0xffffffffu )
+    {
+        // direct DFT, O(L^2) per line, exact angle reduction j*k mod L
+        const uint32_t total = a.nlines * L;
+        for ( uint32_t e = threadIdx.x; e < total; e += FIELD_BLOCK )
+        {
+            const uint32_t line = e / L, k = e % L;
+            const uint32_t base = line_base( a, line );
+            double2 acc = make_double2( 0.0, 0.0 );
+            uint32_t m = 0;
+            for ( uint32_t j = 0; j < L; ++j )
+            {
+                double2 w = tw[m];
+                if ( inverse ) w.y = -w.y;
+                const double2 v = cmul( in[ base + j*a.S ], w );
+                acc.x += v.x; acc.y += v.y;
+                m += k; if ( m >= L ) m -= L;
+            }
+            out[ base + k*a.S ] = acc;
+        }
+        __syncthreads();
+        return out;
+    }
+    const uint32_t half = L >> 1, lh = a.log2L - 1;
+    const uint32_t total = a.nlines * half;
+    uint32_t log2Ns = 0;
+    for ( uint32_t Ns = 1; Ns < L; Ns <<= 1, ++log2Ns )
+    {
+        const uint32_t twstep = L >> ( log2Ns + 1 );       // L / (2 Ns)
+        for ( uint32_t e = threadIdx.x; e < total; e += FIELD_BLOCK )
+        {
+            const uint32_t line = e >> lh, j = e & ( half - 1 );
+            const uint32_t base = line_base( a, line );
+            const double2 x0 = in[ base + j*a.S ];
+            double2 x1 = in[ base + ( j + half )*a.S ];
+            const uint32_t k = j & ( Ns - 1 );
+            double2 w = tw[ k * twstep ];
+            if ( inverse ) w.y = -w.y;
+            x1 = cmul( x1, w );
+            const uint32_t o = ( ( j >> log2Ns ) << ( log2Ns + 1 ) ) + k;
+            out[ base + o*a.S ]        = make_double2( x0.x + x1.x, x0.y + x1.y );
+            out[ base + ( o + Ns )*a.S ] = make_double2( x0.x - x1.x, x0.y - x1.y );
+        }
+        __syncthreads();
+        double2 *t = in; in = out; out = t;
+    }
+    return in;
+}
+
+__device__ double2 *fft3( double2 *A, double2 *B, const field_params &p, bool inverse )
+{
+    axis_desc ax { p.Nx, p.log2x, 1u,          p.N / p.Nx, 0u, p.Nx, p.Ny };
+    axis_desc ay { p.Ny, p.log2y, p.Nx,        p.N / p.Ny, 1u, p.Nx, p.Ny };
+    axis_desc az { p.Nz, p.log2z, p.Nx * p.Ny, p.N / p.Nz, 2u, p.Nx, p.Ny };
+    double2 *r = fft_axis( A, B, ax, p.tw_x, inverse );
+    double2 *o = ( r == A ) ? B : A;
+    r = fft_axis( r, o, ay, p.tw_y, inverse );
+    o = ( r == A ) ? B : A;
+    r = fft_axis( r, o, az, p.tw_z, inverse );
+    return r;
+}
+
+__global__ __launch_bounds__( FIELD_BLOCK )
+void field_kernel( field_params p )
+{
+    extern __shared__ double2 lds2[];
+    double2 *A = lds2, *B = lds2 + p.N;
+    double *red = reinterpret_cast<double*>( lds2 + 2*p.N );
+    const uint32_t N = p.N;
+
+    // 1. rho (or the nodal values) into A
+    for ( uint32_t i = threadIdx.x; i < N; i += FIELD_BLOCK )
+    {
+        double v;
+        if ( p.mode == FIELD_STEP && p.partials != nullptr )
+        {
+            csum acc { 0.0, 0.0 };
+            const double *pp = p.partials + size_t( i ) * p.wpx;
+            for ( uint32_t j = 0; j < p.wpx; ++j ) acc = csum_add( acc, csum { pp[j], 0.0 } );
+            v = p.weight * ( acc.s + acc.c );
+        }
+        else
+        {
+            v = p.values[i];
+        }
+        if ( p.rho_out ) p.rho_out[i] = v;
+        A[i] = make_double2( v, 0.0 );
+    }
+    __syncthreads();
+
+    // 2. forward transform
+    double2 *F = fft3( A, B, p, false );
+    double2 *G = ( F == A ) ? B : A;
+
+    // 3. spectral energy and the multiplier
+    double e = 0.0;
+    if ( p.mode != FIELD_FIT )
+        for ( uint32_t k = threadIdx.x; k < N; k += FIELD_BLOCK )
+        {
+            const double2 f = F[k];
+            e += p.ewt[k] * ( f.x*f.x + f.y*f.y );
+        }
+    e = wave_sum( e );
+    if ( threadIdx.x % WAVE == 0 ) red[ threadIdx.x / WAVE ] = e;
+    for ( uint32_t k = threadIdx.x; k < N; k += FIELD_BLOCK )
+        F[k] = cmul( F[k], p.mult[k] );
+    __syncthreads();
+    if ( threadIdx.x == 0 && p.mode != FIELD_FIT && p.energy != nullptr )
+    {
+        double s = 0.0;
+        for ( int w = 0; w < FIELD_BLOCK / WAVE; ++w ) s += red[w];
+        *p.energy = s;
+    }
+
+    // 4. inverse transform
+    double2 *R = fft3( F, G, p, true );
+
+    // 5. outputs
+    if ( p.mode == FIELD_POISSON )
+    {
+        for ( uint32_t i = threadIdx.x; i < N; i += FIELD_BLOCK ) p.values[i] = R[i].x;
+        return;
+    }
+    const uint32_t o1 = uint32_t( p.order ) - 1;
+    if ( p.dim == 1 )
+    {
+        const uint32_t len = p.Nx + o1;
+        for ( uint32_t i = threadIdx.x; i < len; i += FIELD_BLOCK )
+        {
+            const double c = R[ i % p.Nx ].x;
+            p.slice[i] = c;
+            if ( p.slice32 ) p.slice32[i] = float( c );
+        }
+        // extended differences: entry j holds D[j-1] = c[j] - c[j-1] (periodic)
+        for ( uint32_t j = threadIdx.x; j < p.stride_d; j += FIELD_BLOCK )
+        {
+            double d = 0.0;
+            if ( j < len + 1 ) d = R[ j % p.Nx ].x - R[ ( j + p.Nx - 1 ) % p.Nx ].x;
+            if ( p.dslice )   p.dslice[j] = d;
+            if ( p.dslice32 ) p.dslice32[j] = float( d );
+        }
+    }
+    else
+    {
+        const uint32_t ex = p.Nx + o1, ey = p.Ny + o1, ez = p.Nz + o1;
+        const uint32_t total = ex * ey * ez;
+        for ( uint32_t e2 = threadIdx.x; e2 < total; e2 += FIELD_BLOCK )
+        {
+            const uint32_t i = e2 % ex, j = ( e2 / ex ) % ey, k = e2 / ( ex * ey );
+            const double c = R[ ( ( k % p.Nz ) * p.Ny + ( j % p.Ny ) ) * p.Nx + ( i % p.Nx ) ].x;
+            p.slice[ k * p.sz + j * p.sy + i ] = c;
+            if ( p.slice32 ) p.slice32[ k * p.sz + j * p.sy + i ] = float( c );
+        }
+    }
+}
+
+__global__ void reduce_partials_kernel( const double *partials, uint32_t wpx, double weight,
+                                        uint32_t first, uint32_t count, uint32_t n_space, double *rho )
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( i >= n_space ) return;
+    double v = 0.0;
+    if ( i >= first && i < first + count )
+    {
+        csum acc { 0.0, 0.0 };
+        const double *pp = partials + size_t( i ) * wpx;
+        for ( uint32_t j = 0; j < wpx; ++j ) acc = csum_add( acc, csum { pp[j], 0.0 } );
+        v = weight * ( acc.s + acc.c );
+    }
+    rho[i] = v;
+}
+
+// FP32 copy of an uploaded slice of `len` reals and, in 1D (period Nx), the
+// extended differences dslice[j] = c[j mod Nx] - c[(j-1) mod Nx], j <= len.
+__global__ void slice_tables_kernel( int dim, const double *slice, uint32_t len, uint32_t Nx, double *dslice,
+                                     float *dslice32, float *slice32 )
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( i > len ) return;
+    if ( slice32 && i < len ) slice32[i] = float( slice[i] );
+    if ( dim == 1 )
+    {
+        const double d = slice[ i % Nx ] - slice[ ( i + Nx - 1 ) % Nx ];
+        if ( dslice )   dslice[i] = d;
+        if ( dslice32 ) dslice32[i] = float( d );
+    }
+}
+
+} // namespace
+
+int field_supported( int, uint32_t Nx, uint32_t Ny, uint32_t Nz )
+{
+    const uint64_t N = uint64_t( Nx ) * Ny * Nz;
+    if ( N == 0 ) return 0;
+    // two complex double buffers of N entries must fit the 160 KiB LDS
+    return ( N * 2 * sizeof(double2) + 64 * sizeof(double) ) <= 160u * 1024u ? 1 : 0;
+}
+
+size_t field_lds_bytes( const field_params &p )
+{
+    return size_t( p.N ) * 2 * sizeof(double2) + ( FIELD_BLOCK / WAVE ) * sizeof(double);
+}
+
+hipError_t launch_field( const field_params &p, hipStream_t st )
+{
+    const size_t lds = field_lds_bytes( p );
+    if ( lds > 65536 )
+    {
+        hipError_t e = hipFuncSetAttribute( reinterpret_cast<const void*>( field_kernel ),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, int( lds ) );
+        if ( e != hipSuccess ) return e;
+    }
+    hipLaunchKernelGGL( field_kernel, dim3( 1 ), dim3( FIELD_BLOCK ), lds, st, p );
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_partials( const double *partials, uint32_t wpx, double weight,
+                                   uint32_t first, uint32_t count, uint32_t n_space,
+                                   double *rho, hipStream_t st )
+{
+    const uint32_t block = 256, grid = ( n_space + block - 1 ) / block;
+    hipLaunchKernelGGL( reduce_partials_kernel, dim3( grid ), dim3( block ), 0, st,
+                        partials, wpx, weight, first, count, n_space, rho );
+    return hipGetLastError();
+}
+
+hipError_t launch_slice_tables( int dim, const double *slice, uint32_t len, uint32_t Nx, double *dslice,
+                                float *dslice32, float *slice32, hipStream_t st )
+{
+    const uint32_t block = 256, grid = ( len + 1 + block - 1 ) / block;
+    hipLaunchKernelGGL( slice_tables_kernel, dim3( grid ), dim3( block ), 0, st,
+                        dim, slice, len, Nx, dslice, dslice32, slice32 );
+    return hipGetLastError();
+}
+
+} // namespace nufi_impl

@@ -1,0 +1,129 @@
+// internal.hpp — host-side declarations shared by the translation units of
+// libnufi_hip.so (context layout, kernel parameter blocks, launchers).
+#ifndef NUFI_INTERNAL_HPP
+#define NUFI_INTERNAL_HPP
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/nufi_hip.h"
+
+namespace nufi_impl
+{
+
+// Trace kernel variants.
+enum trace_variant : int
+{
+    TRACE_FAST   = 0, // FP64, cell-unit positions, FMA, closed-form spline
+    TRACE_STRICT = 1, // FP64 in the reference's operation order, no contraction
+    TRACE_FP32C  = 2  // FP32 with compensated (two-sum) position/velocity
+};
+
+// ---- 1D trace ----------------------------------------------------------------
+// Node q = ix*Nu + iu (cuda_kernel.cu:39-44).  The launch is cut into
+// wave-tasks: task g covers ix = ix_first + g / wpx and iu = (g % wpx)*64 +
+// lane; lanes with iu >= Nu or q outside [q_begin, q_end) contribute zero.
+// Each task writes one partial sum of f to partials[ix*wpx + g % wpx].
+struct trace1_params
+{
+    const double *dhist;     // extended differences of slice m at dhist + m*stride_d (FP64)
+    const float  *dhist32;   // FP32 copy (TRACE_FP32C)
+    double       *partials;
+    uint64_t n;
+    uint64_t q_begin, q_end;
+    uint32_t Nx, Nu, stride_d, wpx;
+    uint32_t ix_first, n_tasks;
+    uint32_t chunk;          // slices per LDS chunk
+    uint32_t order;
+    double x_min, dx, dx_inv, Lx, Lx_inv, u_min, du, dt;
+    nufi_f0 f0;
+};
+
+// ---- 3D trace ----------------------------------------------------------------
+// Node q = l*Nvel + iv with l = (iz*Ny + iy)*Nx + ix and iv = (iw*Nv + iv')*Nu
+// + iu (cuda_kernel.cu:399-412).  Same wave-task cut with Nu -> Nvel.
+struct trace3_params
+{
+    const double *hist;      // coefficient slices (FP64), stride st
+    const float  *hist32;    // FP32 copy (TRACE_FP32C)
+    double       *partials;
+    uint64_t n;
+    uint64_t q_begin, q_end;
+    uint32_t Nx, Ny, Nz, Nu, Nv, Nw;
+    uint32_t sy, sz, st;     // strides of a slice: sy = Nx+order-1, sz = sy*(Ny+order-1)
+    uint32_t wpx;            // waves per spatial point: ceil(Nvel/64)
+    uint32_t l_first, n_tasks;
+    uint32_t chunk;
+    uint32_t order;
+    double x_min, y_min, z_min, dx, dy, dz, dx_inv, dy_inv, dz_inv;
+    double Lx, Ly, Lz, Lx_inv, Ly_inv, Lz_inv;
+    double u_min, v_min, w_min, du, dv, dw, dt;
+    nufi_f0 f0;
+};
+
+struct launch_shape
+{
+    int block;   // threads per workgroup
+    int npt;     // nodes (wave-task rows) per thread
+    int chunk;   // history slices per LDS chunk
+};
+
+// Launchers (trace1d.hip / trace3d.hip).  Return hipSuccess or the error.
+hipError_t launch_trace1d( const trace1_params &p, int variant, const launch_shape &s, hipStream_t st );
+hipError_t launch_trace3d( const trace3_params &p, int variant, const launch_shape &s, hipStream_t st );
+size_t trace1d_lds_bytes( const trace1_params &p, int variant, const launch_shape &s );
+size_t trace3d_lds_bytes( const trace3_params &p, int variant, const launch_shape &s );
+bool trace1d_shape_ok( int variant, int order, const launch_shape &s );
+bool trace3d_shape_ok( int variant, int order, const launch_shape &s );
+
+// ---- field solve (field.hip) -------------------------------------------------
+// Input rho either from per-wave partials (sum_j partials[i*wpx + j], times
+// -dv), or from a plain array.  Output: the padded coefficient slice, its
+// differences (1D) and FP32 copies, the energy, and optionally the nodal phi.
+enum field_mode : int
+{
+    FIELD_STEP    = 0, // rho -> Poisson -> fit -> slice n
+    FIELD_POISSON = 1, // rho -> nodal phi (in/out buffer) + energy
+    FIELD_FIT     = 2  // nodal values -> slice
+};
+
+struct field_params
+{
+    int      dim, order, mode;
+    uint32_t Nx, Ny, Nz, N;      // N = Nx*Ny*Nz
+    uint32_t log2x, log2y, log2z; // 0xffffffff if not a power of two
+    const double *partials;      // FIELD_STEP: partials (nullable -> use values)
+    uint32_t wpx;
+    double   weight;             // -dv multiplier applied to partial sums
+    double  *values;             // rho / phi (FIELD_POISSON in/out, FIELD_FIT in)
+    double  *rho_out;            // optional copy of rho
+    const double2 *tw_x, *tw_y, *tw_z; // forward twiddles exp(-2 pi i k/N_axis)
+    const double2 *mult;         // per-mode multiplier (complex), N entries
+    const double  *ewt;          // per-mode energy weight, N entries
+    double  *slice;              // history slice n (coefficients)
+    double  *dslice;             // 1D differences of slice n (stride_d)
+    float   *slice32;            // FP32 coefficients (3D FP32C) or null
+    float   *dslice32;           // FP32 differences (1D FP32C) or null
+    double  *energy;             // energies[n]
+    uint32_t sy, sz, stride_d;
+};
+
+hipError_t launch_field( const field_params &p, hipStream_t st );
+size_t field_lds_bytes( const field_params &p );
+int field_supported( int dim, uint32_t Nx, uint32_t Ny, uint32_t Nz );
+
+// Sum of per-wave partials into rho[ix] = weight * sum_j partials[ix*wpx+j]
+// for ix in [first, first+count), zero elsewhere (n_space entries).
+hipError_t launch_reduce_partials( const double *partials, uint32_t wpx, double weight,
+                                   uint32_t first, uint32_t count, uint32_t n_space,
+                                   double *rho, hipStream_t st );
+// Differences (and FP32 copies) of an uploaded 1D slice.
+hipError_t launch_slice_tables( int dim, const double *slice, uint32_t len, uint32_t Nx, double *dslice,
+                                float *dslice32, float *slice32, hipStream_t st );
+
+} // namespace nufi_impl
+
+#endif

@@ -1,0 +1,399 @@
+// trace1d.hip — the 1D characteristic trace and rho partial sums on gfx950.
+//
+// Reference: dim1::eval_ftilda (nufi/rho.hpp:31-61) inside dim1::cuda_eval_rho
+// (nufi/cuda_kernel.cu:31-51).  For every node (x_ix, u_iu) the characteristic
+// is traced backward through slices n-1 .. 1 (drift, E from slice m, kick),
+// then a final drift and half-kick with slice 0, and f0 is evaluated at the
+// foot point.  The reference adds -du*f into rho[ix] with an atomic per node;
+// here each wave of 64 velocity nodes of one x reduces its f values with a
+// compensated butterfly and writes one partial, which the field kernel sums in
+// a fixed order — deterministic, no atomics.
+//
+// Every workgroup streams the history through LDS in chunks of `chunk`
+// slices; all nodes of the workgroup consume the same slice at the same
+// time (the trace loop is uniform in m), so each slice is read from L2 once
+// per workgroup and then gathered from LDS by every node.  A slice holds the
+// first differences D[i] = c[i+1] - c[i] of the coefficient slice, which are
+// exactly the differences splines.hpp:98-100 forms on the fly, extended
+// periodically by one entry on each side (stored at offset 1: D[-1] ..
+// D[Nx+order-2]) so that a cell index of -1 or Nx from a wrap that rounds
+// onto the boundary needs no correction.
+//
+// Variants:
+//   strict : the reference's operation order (unwrapped x, the wrap of
+//            fields.hpp:42-46, de Boor of splines.hpp:102-106), no FMA
+//            contraction — bit-level faithful up to device cos/exp.
+//   fast   : cell-unit position s = (x - x_min)/dx wrapped exactly into
+//            [0, Nx) every step, the derivative of the cubic B-spline in the
+//            closed form  dphi/dt = D1 + (1-t)^2/2 (D0-D1) + t^2/2 (D2-D1)
+//            and FMAs (order 4).
+//   fp32c  : as fast, in FP32, with the position and velocity carried as
+//            unevaluated sums hi + lo updated by TwoSum / TwoProd; f0 and the
+//            rho sums stay FP64 (order 4).
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace nufi_impl
+{
+
+using namespace nufi_dev;
+
+namespace
+{
+
+template <int K>
+__host__ __device__ constexpr double inv_factorial()
+{
+    double f = 1.0;
+    for ( int i = 2; i <= K; ++i ) f *= double(i);
+    return 1.0 / f;
+}
+
+// Cooperative copy of `count` elements into LDS (16-byte vectors where the
+// source allows it; chunk bases are 16-byte aligned by construction).
+template <int BLOCK, typename T>
+__device__ __forceinline__ void stage( T *__restrict__ dst, const T *__restrict__ src, uint32_t count )
+{
+    constexpr uint32_t V = 16 / sizeof(T);
+    const uint32_t nv = count / V;
+    const uint4 *s4 = reinterpret_cast<const uint4*>( src );
+    uint4 *d4 = reinterpret_cast<uint4*>( dst );
+    for ( uint32_t i = threadIdx.x; i < nv; i += BLOCK ) d4[i] = s4[i];
+    for ( uint32_t i = nv*V + threadIdx.x; i < count; i += BLOCK ) dst[i] = src[i];
+}
+
+struct node_id
+{
+    uint32_t g, ix, j, iu;
+    bool task, active;
+};
+
+template <int WPB, int NPT>
+__device__ __forceinline__ node_id make_node( const trace1_params &p, int r )
+{
+    node_id id;
+    const uint32_t lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    id.g = ( blockIdx.x * NPT + r ) * WPB + wave;
+    id.task = id.g < p.n_tasks;
+    const uint32_t ixl = id.task ? id.g / p.wpx : 0;
+    id.j  = id.task ? id.g % p.wpx : 0;
+    id.ix = p.ix_first + ixl;
+    id.iu = id.j * WAVE + lane;
+    const uint64_t q = uint64_t(id.ix) * p.Nu + id.iu;
+    id.active = id.task && id.iu < p.Nu && q >= p.q_begin && q < p.q_end;
+    return id;
+}
+
+__device__ __forceinline__ void write_partial( const trace1_params &p, const node_id &id, double f )
+{
+    const csum v = wave_csum( csum { f, 0.0 } );
+    if ( id.task && ( threadIdx.x % WAVE ) == 0 )
+        p.partials[ size_t(id.ix) * p.wpx + id.j ] = v.s + v.c;
+}
+
+// ---- fast FP64 (order 4) ----------------------------------------------------
+template <int BLOCK, int NPT>
+__global__ __launch_bounds__( BLOCK )
+void trace1d_fast_kernel( trace1_params p )
+{
+    extern __shared__ double lds[];
+    constexpr int WPB = BLOCK / WAVE;
+
+    node_id id[NPT];
+    double s[NPT], u[NPT];
+    #pragma unroll
+    for ( int r = 0; r < NPT; ++r )
+    {
+        id[r] = make_node<WPB,NPT>( p, r );
+        s[r] = double( id[r].ix );
+        u[r] = p.u_min + id[r].iu*p.du + p.du/2;   // cuda_kernel.cu:44
+    }
+
+    const double Nxd   = double( p.Nx );
+    const double invNx = 1.0 / Nxd;
+    const double cd    = p.dt * p.dx_inv;          // drift in cells per unit u
+    const double ck    = -0.5 * p.dt * p.dx_inv;   // kick per unit 2 dphi/dt
+    const uint32_t sd  = p.stride_d;
+
+    for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
+    {
+        const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
+        const int cnt = int( mhi - mlo + 1 );
+        __syncthreads();
+        stage<BLOCK>( lds, p.dhist + size_t( mlo ) * sd, uint32_t( cnt ) * sd );
+        __syncthreads();
+        for ( int jj = cnt - 1; jj >= 0; --jj )
+        {
+            const double *D = lds + jj * sd + 1;      // D[-1] .. D[Nx+order-2]
+            const double kf = ( mlo + jj == 0 ) ? 0.5 * ck : ck;
+            #pragma unroll
+            for ( int r = 0; r < NPT; ++r )
+            {
+                double x = fma( -cd, u[r], s[r] );
+                const double w = floor( x * invNx );
+                x = fma( -Nxd, w, x );
+                const double k = floor( x );
+                const double t = x - k;
+                const int i = int( k );                   // in [-1, Nx]: D is extended
+                const double D0 = D[i], D1 = D[i+1], D2 = D[i+2];
+                const double a = 1.0 - t;
+                const double r2 = fma( a*a, D0 - D1, fma( t*t, D2 - D1, D1 + D1 ) );
+                u[r] = fma( kf, r2, u[r] );
+                s[r] = x;
+            }
+        }
+    }
+
+    #pragma unroll
+    for ( int r = 0; r < NPT; ++r )
+    {
+        const double f = id[r].active ? f0_1d( p.f0, p.x_min + s[r]*p.dx, u[r] ) : 0.0;
+        write_partial( p, id[r], f );
+    }
+}
+
+// ---- strict FP64 (reference operation order, any order 3..8) ---------------
+template <int ORDER>
+__device__ __forceinline__ double efield_strict( double x, const double *D, const trace1_params &p )
+{
+#pragma clang fp contract(off)
+    x = x - p.x_min;
+    x = x - p.Lx * floor( x * p.Lx_inv );
+    const double knot = floor( x * p.dx_inv );
+    // ii in [-1, Nx]: x rounded onto Lx gives Nx (fields.hpp:49), a wrap that
+    // rounds below 0 gives -1; the extended difference table covers both with
+    // the periodic values, so no read leaves the slice.
+    const int ii = int( knot );
+    x = x * p.dx_inv - knot;
+
+    constexpr int n = ORDER, d = 1;
+    double c[ORDER];
+    #pragma unroll
+    for ( int j = 1; j < n; ++j ) c[j] = D[ ii + j - 1 ];
+    #pragma unroll
+    for ( int j = 1; j < n - d; ++j )
+    {
+        #pragma unroll
+        for ( int i = n - d - 1; i >= j; --i )
+        {
+            const double lo = ( ( ( x + double(n) ) - double(d) ) - 1.0 ) - double(i);
+            const double hi = double( i - j + 1 ) - x;
+            c[d+i] = lo * c[d+i] + hi * c[d+i-1];
+        }
+    }
+    const double val = inv_factorial<n - d - 1>() * c[n-1];
+    return -( p.dx_inv * val );
+}
+
+template <int ORDER, int BLOCK>
+__global__ __launch_bounds__( BLOCK )
+void trace1d_strict_kernel( trace1_params p )
+{
+#pragma clang fp contract(off)
+    extern __shared__ double lds[];
+    constexpr int WPB = BLOCK / WAVE;
+    const node_id id = make_node<WPB,1>( p, 0 );
+
+    double x = p.x_min + id.ix * p.dx;                      // cuda_kernel.cu:43
+    double u = p.u_min + id.iu * p.du + p.du / 2;           // cuda_kernel.cu:44
+    const uint32_t sd = p.stride_d;
+    const double half_dt = 0.5 * p.dt;
+
+    for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
+    {
+        const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
+        const int cnt = int( mhi - mlo + 1 );
+        __syncthreads();
+        stage<BLOCK>( lds, p.dhist + size_t( mlo ) * sd, uint32_t( cnt ) * sd );
+        __syncthreads();
+        for ( int jj = cnt - 1; jj >= 0; --jj )
+        {
+            x = x - p.dt * u;
+            const double E = efield_strict<ORDER>( x, lds + jj * sd + 1, p );
+            u = ( mlo + jj == 0 ) ? u + half_dt * E : u + p.dt * E;
+        }
+    }
+    const double f = id.active ? f0_1d( p.f0, x, u ) : 0.0;
+    write_partial( p, id, f );
+}
+
+// ---- FP32 compensated (order 4) --------------------------------------------
+template <int BLOCK, int NPT>
+__global__ __launch_bounds__( BLOCK )
+void trace1d_fp32c_kernel( trace1_params p )
+{
+    extern __shared__ float ldsf[];
+    constexpr int WPB = BLOCK / WAVE;
+
+    node_id id[NPT];
+    float sh[NPT], sl[NPT], uh[NPT], ul[NPT];
+    #pragma unroll
+    for ( int r = 0; r < NPT; ++r )
+    {
+        id[r] = make_node<WPB,NPT>( p, r );
+        const double u0 = p.u_min + id[r].iu*p.du + p.du/2;
+        sh[r] = float( id[r].ix ); sl[r] = 0.0f;
+        uh[r] = float( u0 );       ul[r] = float( u0 - double( uh[r] ) );
+    }
+
+    const float Nxf   = float( p.Nx );
+    const float invNx = 1.0f / Nxf;
+    const double cd   = p.dt * p.dx_inv;
+    const float cd_hi = float( cd ), cd_lo = float( cd - double( cd_hi ) );
+    const float ck    = float( -0.5 * p.dt * p.dx_inv );
+    const uint32_t sd = p.stride_d;
+
+    for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
+    {
+        const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
+        const int cnt = int( mhi - mlo + 1 );
+        __syncthreads();
+        stage<BLOCK>( ldsf, p.dhist32 + size_t( mlo ) * sd, uint32_t( cnt ) * sd );
+        __syncthreads();
+        for ( int jj = cnt - 1; jj >= 0; --jj )
+        {
+            const float *D = ldsf + jj * sd + 1;
+            const float kf = ( mlo + jj == 0 ) ? 0.5f * ck : ck;
+            #pragma unroll
+            for ( int r = 0; r < NPT; ++r )
+            {
+                // drift: s -= cd*u with TwoProd on the leading product
+                const float pr = cd_hi * uh[r];
+                float pe = fmaf( cd_hi, uh[r], -pr );
+                pe = fmaf( cd_hi, ul[r], fmaf( cd_lo, uh[r], pe ) );
+                float y, e;
+                two_sum( sh[r], -pr, y, e );
+                e = e + ( sl[r] - pe );
+                fast_two_sum( y, e, sh[r], sl[r] );
+                // exact periodic wrap into [0, Nx)
+                const float w = floorf( sh[r] * invNx );
+                float y2, e2;
+                fast_two_sum( -Nxf * w, sh[r], y2, e2 );
+                sh[r] = y2; sl[r] = sl[r] + e2;
+                // locate
+                const float k = floorf( sh[r] );
+                const float t = ( sh[r] - k ) + sl[r];
+                const int i = int( k );                   // in [-1, Nx]
+                const float D0 = D[i], D1 = D[i+1], D2 = D[i+2];
+                const float a = 1.0f - t;
+                const float r2 = fmaf( a*a, D0 - D1, fmaf( t*t, D2 - D1, D1 + D1 ) );
+                // kick: u += kf*r2 with TwoSum
+                float y3, e3;
+                two_sum( uh[r], kf * r2, y3, e3 );
+                uh[r] = y3; ul[r] = ul[r] + e3;
+            }
+        }
+    }
+
+    #pragma unroll
+    for ( int r = 0; r < NPT; ++r )
+    {
+        const double s = double( sh[r] ) + double( sl[r] );
+        const double u = double( uh[r] ) + double( ul[r] );
+        const double f = id[r].active ? f0_1d( p.f0, p.x_min + s*p.dx, u ) : 0.0;
+        write_partial( p, id[r], f );
+    }
+}
+
+template <typename K>
+hipError_t launch_kernel( K kernel, uint32_t grid, int block, size_t lds, const trace1_params &p,
+                          hipStream_t st )
+{
+    if ( lds > 65536 )
+    {
+        hipError_t e = hipFuncSetAttribute( reinterpret_cast<const void*>( kernel ),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, int( lds ) );
+        if ( e != hipSuccess ) return e;
+    }
+    hipLaunchKernelGGL( kernel, dim3( grid ), dim3( block ), lds, st, p );
+    return hipGetLastError();
+}
+
+template <int BLOCK, int NPT>
+hipError_t launch_fast_or_32( const trace1_params &p, int variant, uint32_t grid, size_t lds, hipStream_t st )
+{
+    if ( variant == TRACE_FP32C )
+        return launch_kernel( trace1d_fp32c_kernel<BLOCK,NPT>, grid, BLOCK, lds, p, st );
+    return launch_kernel( trace1d_fast_kernel<BLOCK,NPT>, grid, BLOCK, lds, p, st );
+}
+
+template <int BLOCK>
+hipError_t launch_npt( const trace1_params &p, int variant, int npt, uint32_t grid, size_t lds, hipStream_t st )
+{
+    switch ( npt )
+    {
+    case 1: return launch_fast_or_32<BLOCK,1>( p, variant, grid, lds, st );
+    case 2: return launch_fast_or_32<BLOCK,2>( p, variant, grid, lds, st );
+    case 4: return launch_fast_or_32<BLOCK,4>( p, variant, grid, lds, st );
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int ORDER>
+hipError_t launch_strict( const trace1_params &p, int block, uint32_t grid, size_t lds, hipStream_t st )
+{
+    switch ( block )
+    {
+    case 64:   return launch_kernel( trace1d_strict_kernel<ORDER,64>,   grid, 64,   lds, p, st );
+    case 256:  return launch_kernel( trace1d_strict_kernel<ORDER,256>,  grid, 256,  lds, p, st );
+    case 1024: return launch_kernel( trace1d_strict_kernel<ORDER,1024>, grid, 1024, lds, p, st );
+    default:   return hipErrorInvalidValue;
+    }
+}
+
+} // namespace
+
+bool trace1d_shape_ok( int variant, int order, const launch_shape &s )
+{
+    if ( variant == TRACE_STRICT )
+        return order >= 3 && order <= 8 && s.npt == 1 &&
+               ( s.block == 64 || s.block == 256 || s.block == 1024 ) && s.chunk >= 1;
+    if ( order != 4 ) return false;
+    const bool bok = s.block == 64 || s.block == 128 || s.block == 256 || s.block == 512 || s.block == 1024;
+    const bool nok = s.npt == 1 || s.npt == 2 || s.npt == 4;
+    return bok && nok && s.chunk >= 1;
+}
+
+size_t trace1d_lds_bytes( const trace1_params &p, int variant, const launch_shape &s )
+{
+    const size_t elem = ( variant == TRACE_FP32C ) ? sizeof(float) : sizeof(double);
+    const uint64_t c = p.n < uint64_t( s.chunk ) ? ( p.n > 0 ? p.n : 1 ) : uint64_t( s.chunk );
+    return size_t( c ) * p.stride_d * elem;
+}
+
+hipError_t launch_trace1d( const trace1_params &p_in, int variant, const launch_shape &s, hipStream_t st )
+{
+    if ( !trace1d_shape_ok( variant, int( p_in.order ), s ) ) return hipErrorInvalidValue;
+    if ( p_in.n_tasks == 0 ) return hipSuccess;
+    trace1_params p = p_in;
+    const uint32_t tasks_per_block = uint32_t( s.block / WAVE ) * uint32_t( s.npt );
+    const uint32_t grid = ( p.n_tasks + tasks_per_block - 1 ) / tasks_per_block;
+    p.chunk = uint32_t( s.chunk );
+    if ( p.n < p.chunk ) p.chunk = p.n > 0 ? uint32_t( p.n ) : 1u;
+    const size_t lds = trace1d_lds_bytes( p, variant, s );
+
+    if ( variant == TRACE_STRICT )
+    {
+        switch ( p.order )
+        {
+        case 3: return launch_strict<3>( p, s.block, grid, lds, st );
+        case 4: return launch_strict<4>( p, s.block, grid, lds, st );
+        case 5: return launch_strict<5>( p, s.block, grid, lds, st );
+        case 6: return launch_strict<6>( p, s.block, grid, lds, st );
+        case 7: return launch_strict<7>( p, s.block, grid, lds, st );
+        case 8: return launch_strict<8>( p, s.block, grid, lds, st );
+        default: return hipErrorInvalidValue;
+        }
+    }
+    switch ( s.block )
+    {
+    case 64:   return launch_npt<64>  ( p, variant, s.npt, grid, lds, st );
+    case 128:  return launch_npt<128> ( p, variant, s.npt, grid, lds, st );
+    case 256:  return launch_npt<256> ( p, variant, s.npt, grid, lds, st );
+    case 512:  return launch_npt<512> ( p, variant, s.npt, grid, lds, st );
+    case 1024: return launch_npt<1024>( p, variant, s.npt, grid, lds, st );
+    default:   return hipErrorInvalidValue;
+    }
+}
+
+} // namespace nufi_impl

@@ -1,0 +1,542 @@
+// trace3d.hip — the 3D3V characteristic trace and rho partial sums on gfx950.
+//
+// Reference: dim3::eval_ftilda (nufi/rho.hpp:318-367) inside dim3::cuda_eval_rho
+// (nufi/cuda_kernel.cu:392-426).  Per characteristic step: drift x,y,z; E =
+// -grad phi_m from three tensor-spline evaluations (fields.hpp:308-350 ->
+// splines.hpp:144-172); kick u,v,w.  After the loop a final drift and
+// half-kick with slice 0, and f0 at the foot.
+//
+// The workgroup streams coefficient slices through LDS (one slice is
+// (Nx+3)(Ny+3)(Nz+3) reals: 6859 = 54.9 KB FP64 at 16^3) and every node
+// gathers its 4x4x4 window from LDS.  Rho partial sums: one compensated
+// butterfly per wave of 64 velocity nodes of one spatial point, summed in a
+// fixed order by the field kernel.
+//
+// Variants:
+//   strict : the reference's arithmetic (three separate tensor evaluations,
+//            Cox-de Boor bases, de Boor in z, unwrapped positions), no FMA
+//            contraction.
+//   fast   : cell-unit positions wrapped exactly each step; the value and
+//            derivative bases of each axis in closed form; the three gradient
+//            components share one pass over the 64 coefficients:
+//              A_jk = sum_i c_ijk Bx_i,  X_jk = sum_i c_ijk Dx_i    (128 FMA)
+//              P_k = sum_j A_jk By_j, Q_k = sum_j A_jk Dy_j,
+//              R_k = sum_j X_jk By_j                                  (48 FMA)
+//              Ex ~ sum_k R_k Bz_k, Ey ~ sum_k Q_k Bz_k, Ez ~ sum_k P_k Dz_k (12)
+//   fp32c  : as fast in FP32 from an FP32 copy of the slice, positions and
+//            velocities as hi + lo pairs with TwoSum/TwoProd updates; f0 and
+//            the rho sums in FP64.
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace nufi_impl
+{
+
+using namespace nufi_dev;
+
+namespace
+{
+
+template <int BLOCK, typename T>
+__device__ __forceinline__ void stage3( T *__restrict__ dst, const T *__restrict__ src, uint32_t count )
+{
+    // slice bases are not 16-byte aligned in general (odd strides): scalar copy
+    for ( uint32_t i = threadIdx.x; i < count; i += BLOCK ) dst[i] = src[i];
+}
+
+struct node3
+{
+    uint32_t g, l, j, iv;
+    bool task, active;
+};
+
+template <int WPB, int NPT>
+__device__ __forceinline__ node3 make_node3( const trace3_params &p, int r )
+{
+    node3 id;
+    const uint32_t lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    id.g = ( blockIdx.x * NPT + r ) * WPB + wave;
+    id.task = id.g < p.n_tasks;
+    const uint32_t ll = id.task ? id.g / p.wpx : 0;
+    id.j  = id.task ? id.g % p.wpx : 0;
+    id.l  = p.l_first + ll;
+    id.iv = id.j * WAVE + lane;
+    const uint64_t nvel = uint64_t( p.Nu ) * p.Nv * p.Nw;
+    const uint64_t q = uint64_t( id.l ) * nvel + id.iv;
+    id.active = id.task && id.iv < nvel && q >= p.q_begin && q < p.q_end;
+    return id;
+}
+
+__device__ __forceinline__ void write_partial3( const trace3_params &p, const node3 &id, double f )
+{
+    const csum v = wave_csum( csum { f, 0.0 } );
+    if ( id.task && ( threadIdx.x % WAVE ) == 0 )
+        p.partials[ size_t( id.l ) * p.wpx + id.j ] = v.s + v.c;
+}
+
+// node coordinates, cuda_kernel.cu:402-418
+__device__ __forceinline__ void node_coords( const trace3_params &p, const node3 &id,
+                                             uint32_t &ix, uint32_t &iy, uint32_t &iz,
+                                             double &u, double &v, double &w )
+{
+    ix = id.l % p.Nx; iy = ( id.l / p.Nx ) % p.Ny; iz = id.l / ( p.Nx * p.Ny );
+    const uint32_t iu = id.iv % p.Nu, ivv = ( id.iv / p.Nu ) % p.Nv, iw = id.iv / ( p.Nu * p.Nv );
+    u = p.u_min + iu  * p.du + p.du / 2;
+    v = p.v_min + ivv * p.dv + p.dv / 2;
+    w = p.w_min + iw  * p.dw + p.dw / 2;
+}
+
+// ---- closed-form cubic B-spline bases on [0,1) ------------------------------
+// values  B = ((1-t)^3, 3t^3-6t^2+4, -3t^3+3t^2+3t+1, t^3)/6
+// slopes  D = (-(1-t)^2, 3t^2-4t, -3t^2+2t+1, t^2)/2
+template <typename T>
+__device__ __forceinline__ void cubic_bases( T t, T B[4], T D[4] )
+{
+    const T a = T(1) - t, t2 = t*t, a2 = a*a;
+    const T sixth = T(1) / T(6);
+    B[0] = a2 * a * sixth;
+    B[3] = t2 * t * sixth;
+    B[1] = fma( t2, fma( T(0.5), t, T(-1) ), T(2) / T(3) );   // 2/3 - t^2 + t^3/2
+    B[2] = ( ( T(1) - B[0] ) - B[1] ) - B[3];
+    D[0] = T(-0.5) * a2;
+    D[3] = T(0.5) * t2;
+    D[1] = t * fma( T(1.5), t, T(-2) );                         // 1.5 t^2 - 2 t
+    D[2] = ( -D[0] - D[1] ) - D[3];
+}
+
+// Shared-basis gradient in reference-cell units: g = (dphi/dtx, dphi/dty, dphi/dtz).
+template <typename T>
+__device__ __forceinline__ void grad_window( const T *__restrict__ c, uint32_t sy, uint32_t sz,
+                                             const T Bx[4], const T Dx[4], const T By[4], const T Dy[4],
+                                             const T Bz[4], const T Dz[4], T &gx, T &gy, T &gz )
+{
+    gx = T(0); gy = T(0); gz = T(0);
+    #pragma unroll
+    for ( int k = 0; k < 4; ++k )
+    {
+        T P = T(0), Q = T(0), R = T(0);
+        #pragma unroll
+        for ( int j = 0; j < 4; ++j )
+        {
+            const T *row = c + k*sz + j*sy;
+            const T c0 = row[0], c1 = row[1], c2 = row[2], c3 = row[3];
+            const T A = fma( c3, Bx[3], fma( c2, Bx[2], fma( c1, Bx[1], c0*Bx[0] ) ) );
+            const T X = fma( c3, Dx[3], fma( c2, Dx[2], fma( c1, Dx[1], c0*Dx[0] ) ) );
+            P = fma( A, By[j], P );
+            Q = fma( A, Dy[j], Q );
+            R = fma( X, By[j], R );
+        }
+        gx = fma( R, Bz[k], gx );
+        gy = fma( Q, Bz[k], gy );
+        gz = fma( P, Dz[k], gz );
+    }
+}
+
+// one axis of the fast FP64 step: drift, exact wrap, locate
+__device__ __forceinline__ int advance_axis( double &s, double vel, double cd, double N, double invN, int Ni,
+                                             double &t )
+{
+    double x = fma( -cd, vel, s );
+    const double w = floor( x * invN );
+    x = fma( -N, w, x );
+    const double k = floor( x );
+    t = x - k;
+    int i = int( k );
+    i = ( i < 0 ) ? i + Ni : i;
+    i = ( i >= Ni ) ? i - Ni : i;
+    s = x;
+    return i;
+}
+
+template <int BLOCK, int NPT>
+__global__ __launch_bounds__( BLOCK )
+void trace3d_fast_kernel( trace3_params p )
+{
+    extern __shared__ double lds[];
+    constexpr int WPB = BLOCK / WAVE;
+
+    node3 id[NPT];
+    double sx[NPT], sy_[NPT], sz_[NPT], vu[NPT], vv[NPT], vw[NPT];
+    #pragma unroll
+    for ( int r = 0; r < NPT; ++r )
+    {
+        id[r] = make_node3<WPB,NPT>( p, r );
+        uint32_t ix, iy, iz;
+        node_coords( p, id[r], ix, iy, iz, vu[r], vv[r], vw[r] );
+        sx[r] = double( ix ); sy_[r] = double( iy ); sz_[r] = double( iz );
+    }
+    const double Nxd = p.Nx, Nyd = p.Ny, Nzd = p.Nz;
+    const double iNx = 1.0 / Nxd, iNy = 1.0 / Nyd, iNz = 1.0 / Nzd;
+    const int Nxi = int( p.Nx ), Nyi = int( p.Ny ), Nzi = int( p.Nz );
+    const double cdx = p.dt * p.dx_inv, cdy = p.dt * p.dy_inv, cdz = p.dt * p.dz_inv;
+    const double kx = -p.dt * p.dx_inv, ky = -p.dt * p.dy_inv, kz = -p.dt * p.dz_inv;
+    const uint32_t st = p.st, sy = p.sy, sz = p.sz;
+
+    for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
+    {
+        const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
+        const int cnt = int( mhi - mlo + 1 );
+        __syncthreads();
+        stage3<BLOCK>( lds, p.hist + size_t( mlo ) * st, uint32_t( cnt ) * st );
+        __syncthreads();
+        for ( int jj = cnt - 1; jj >= 0; --jj )
+        {
+            const double *C = lds + size_t( jj ) * st;
+            const double hf = ( mlo + jj == 0 ) ? 0.5 : 1.0;
+            #pragma unroll
+            for ( int r = 0; r < NPT; ++r )
+            {
+                double tx, ty, tz;
+                const int i = advance_axis( sx[r],  vu[r], cdx, Nxd, iNx, Nxi, tx );
+                const int j = advance_axis( sy_[r], vv[r], cdy, Nyd, iNy, Nyi, ty );
+                const int k = advance_axis( sz_[r], vw[r], cdz, Nzd, iNz, Nzi, tz );
+                double Bx[4], Dx[4], By[4], Dy[4], Bz[4], Dz[4];
+                cubic_bases( tx, Bx, Dx );
+                cubic_bases( ty, By, Dy );
+                cubic_bases( tz, Bz, Dz );
+                double gx, gy, gz;
+                grad_window( C + k*sz + j*sy + i, sy, sz, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
+                vu[r] = fma( hf * kx, gx, vu[r] );
+                vv[r] = fma( hf * ky, gy, vv[r] );
+                vw[r] = fma( hf * kz, gz, vw[r] );
+            }
+        }
+    }
+
+    #pragma unroll
+    for ( int r = 0; r < NPT; ++r )
+    {
+        const double f = id[r].active
+            ? f0_3d( p.f0, p.x_min + sx[r]*p.dx, p.y_min + sy_[r]*p.dy, p.z_min + sz_[r]*p.dz,
+                     vu[r], vv[r], vw[r] )
+            : 0.0;
+        write_partial3( p, id[r], f );
+    }
+}
+
+// ---- FP32 compensated --------------------------------------------------------
+struct f2 { float h, l; };
+
+__device__ __forceinline__ int advance_axis32( float &sh, float &sl, float uh, float ul,
+                                               float cdh, float cdl, float N, float invN, int Ni, float &t )
+{
+    const float pr = cdh * uh;
+    float pe = fmaf( cdh, uh, -pr );
+    pe = fmaf( cdh, ul, fmaf( cdl, uh, pe ) );
+    float y, e;
+    two_sum( sh, -pr, y, e );
+    e = e + ( sl - pe );
+    fast_two_sum( y, e, sh, sl );
+    const float w = floorf( sh * invN );
+    float y2, e2;
+    fast_two_sum( -N * w, sh, y2, e2 );
+    sh = y2; sl = sl + e2;
+    const float k = floorf( sh );
+    t = ( sh - k ) + sl;
+    int i = int( k );
+    i = ( i < 0 ) ? i + Ni : i;
+    i = ( i >= Ni ) ? i - Ni : i;
+    return i;
+}
+
+__device__ __forceinline__ void kick32( float &uh, float &ul, float q )
+{
+    float y, e;
+    two_sum( uh, q, y, e );
+    uh = y; ul = ul + e;
+}
+
+template <int BLOCK, int NPT>
+__global__ __launch_bounds__( BLOCK )
+void trace3d_fp32c_kernel( trace3_params p )
+{
+    extern __shared__ float ldsf[];
+    constexpr int WPB = BLOCK / WAVE;
+
+    node3 id[NPT];
+    float xh[NPT], xl[NPT], yh[NPT], yl[NPT], zh[NPT], zl[NPT];
+    float uh[NPT], ul[NPT], vh[NPT], vl[NPT], wh[NPT], wl[NPT];
+    #pragma unroll
+    for ( int r = 0; r < NPT; ++r )
+    {
+        id[r] = make_node3<WPB,NPT>( p, r );
+        uint32_t ix, iy, iz; double u, v, w;
+        node_coords( p, id[r], ix, iy, iz, u, v, w );
+        xh[r] = float( ix ); xl[r] = 0.0f;
+        yh[r] = float( iy ); yl[r] = 0.0f;
+        zh[r] = float( iz ); zl[r] = 0.0f;
+        uh[r] = float( u ); ul[r] = float( u - double( uh[r] ) );
+        vh[r] = float( v ); vl[r] = float( v - double( vh[r] ) );
+        wh[r] = float( w ); wl[r] = float( w - double( wh[r] ) );
+    }
+    const float Nxf = p.Nx, Nyf = p.Ny, Nzf = p.Nz;
+    const float iNx = 1.0f / Nxf, iNy = 1.0f / Nyf, iNz = 1.0f / Nzf;
+    const int Nxi = int( p.Nx ), Nyi = int( p.Ny ), Nzi = int( p.Nz );
+    const double cdx = p.dt * p.dx_inv, cdy = p.dt * p.dy_inv, cdz = p.dt * p.dz_inv;
+    const float cxh = float( cdx ), cxl = float( cdx - double( cxh ) );
+    const float cyh = float( cdy ), cyl = float( cdy - double( cyh ) );
+    const float czh = float( cdz ), czl = float( cdz - double( czh ) );
+    const float kx = float( -p.dt * p.dx_inv ), ky = float( -p.dt * p.dy_inv ), kz = float( -p.dt * p.dz_inv );
+    const uint32_t st = p.st, sy = p.sy, sz = p.sz;
+
+    for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
+    {
+        const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
+        const int cnt = int( mhi - mlo + 1 );
+        __syncthreads();
+        stage3<BLOCK>( ldsf, p.hist32 + size_t( mlo ) * st, uint32_t( cnt ) * st );
+        __syncthreads();
+        for ( int jj = cnt - 1; jj >= 0; --jj )
+        {
+            const float *C = ldsf + size_t( jj ) * st;
+            const float hf = ( mlo + jj == 0 ) ? 0.5f : 1.0f;
+            #pragma unroll
+            for ( int r = 0; r < NPT; ++r )
+            {
+                float tx, ty, tz;
+                const int i = advance_axis32( xh[r], xl[r], uh[r], ul[r], cxh, cxl, Nxf, iNx, Nxi, tx );
+                const int j = advance_axis32( yh[r], yl[r], vh[r], vl[r], cyh, cyl, Nyf, iNy, Nyi, ty );
+                const int k = advance_axis32( zh[r], zl[r], wh[r], wl[r], czh, czl, Nzf, iNz, Nzi, tz );
+                float Bx[4], Dx[4], By[4], Dy[4], Bz[4], Dz[4];
+                cubic_bases( tx, Bx, Dx );
+                cubic_bases( ty, By, Dy );
+                cubic_bases( tz, Bz, Dz );
+                float gx, gy, gz;
+                grad_window( C + k*sz + j*sy + i, sy, sz, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
+                kick32( uh[r], ul[r], ( hf * kx ) * gx );
+                kick32( vh[r], vl[r], ( hf * ky ) * gy );
+                kick32( wh[r], wl[r], ( hf * kz ) * gz );
+            }
+        }
+    }
+
+    #pragma unroll
+    for ( int r = 0; r < NPT; ++r )
+    {
+        double f = 0.0;
+        if ( id[r].active )
+        {
+            const double x = p.x_min + ( double( xh[r] ) + double( xl[r] ) ) * p.dx;
+            const double y = p.y_min + ( double( yh[r] ) + double( yl[r] ) ) * p.dy;
+            const double z = p.z_min + ( double( zh[r] ) + double( zl[r] ) ) * p.dz;
+            f = f0_3d( p.f0, x, y, z, double( uh[r] ) + double( ul[r] ), double( vh[r] ) + double( vl[r] ),
+                       double( wh[r] ) + double( wl[r] ) );
+        }
+        write_partial3( p, id[r], f );
+    }
+}
+
+// ---- strict FP64 (the reference's arithmetic, order 4) ----------------------
+template <int ORDER, int DERIV>
+__device__ __forceinline__ void basis_strict( double t, double out[ORDER] )
+{
+#pragma clang fp contract(off)
+    double w[ORDER];
+    w[ORDER-1] = 1.0;
+    #pragma unroll
+    for ( int m = 1; m < ORDER - DERIV; ++m )
+    {
+        w[ORDER-m-1] = ( 1.0 - t ) * w[ORDER-m];
+        #pragma unroll
+        for ( int s = 1 - m; s < 0; ++s )
+            w[ORDER-1+s] = ( t - double(s) ) * w[ORDER-1+s] + ( double(m+1+s) - t ) * w[ORDER+s];
+        w[ORDER-1] = w[ORDER-1] * t;
+    }
+    #pragma unroll
+    for ( int j = DERIV - 1; j >= 0; --j )
+    {
+        w[j] = -w[j+1];
+        #pragma unroll
+        for ( int i = j + 1; i < ORDER - 1; ++i ) w[i] = w[i] - w[i+1];
+    }
+    double f = 1.0;
+    #pragma unroll
+    for ( int i = 2; i <= ORDER - DERIV - 1; ++i ) f *= double(i);
+    const double inv = 1.0 / f;
+    #pragma unroll
+    for ( int i = 0; i < ORDER; ++i ) out[i] = w[i] * inv;
+}
+
+template <int ORDER, int DERIV>
+__device__ __forceinline__ double deboor_strict( double t, double c[ORDER] )
+{
+#pragma clang fp contract(off)
+    constexpr int n = ORDER, d = DERIV;
+    #pragma unroll
+    for ( int j = 1; j <= d; ++j )
+        #pragma unroll
+        for ( int i = n - 1; i >= j; --i ) c[i] = c[i] - c[i-1];
+    #pragma unroll
+    for ( int j = 1; j < n - d; ++j )
+        #pragma unroll
+        for ( int i = n - d - 1; i >= j; --i )
+        {
+            const double lo = ( ( ( t + double(n) ) - double(d) ) - 1.0 ) - double(i);
+            const double hi = double( i - j + 1 ) - t;
+            c[d+i] = lo * c[d+i] + hi * c[d+i-1];
+        }
+    double f = 1.0;
+    #pragma unroll
+    for ( int i = 2; i <= n - d - 1; ++i ) f *= double(i);
+    return ( 1.0 / f ) * c[n-1];
+}
+
+template <int ORDER, int DX, int DY, int DZ>
+__device__ __forceinline__ double field3_strict( double x, double y, double z, const double *C,
+                                                 const trace3_params &p )
+{
+#pragma clang fp contract(off)
+    x = x - p.x_min; y = y - p.y_min; z = z - p.z_min;
+    x = x - p.Lx * floor( x * p.Lx_inv );
+    y = y - p.Ly * floor( y * p.Ly_inv );
+    z = z - p.Lz * floor( z * p.Lz_inv );
+    const double xk = floor( x * p.dx_inv ), yk = floor( y * p.dy_inv ), zk = floor( z * p.dz_inv );
+    int ii = int( xk ), jj = int( yk ), kk = int( zk );
+    x = x * p.dx_inv - xk; y = y * p.dy_inv - yk; z = z * p.dz_inv - zk;
+    ii = ( ii >= int( p.Nx ) ) ? ii - int( p.Nx ) : ii; ii = ( ii < 0 ) ? ii + int( p.Nx ) : ii;
+    jj = ( jj >= int( p.Ny ) ) ? jj - int( p.Ny ) : jj; jj = ( jj < 0 ) ? jj + int( p.Ny ) : jj;
+    kk = ( kk >= int( p.Nz ) ) ? kk - int( p.Nz ) : kk; kk = ( kk < 0 ) ? kk + int( p.Nz ) : kk;
+    double factor = 1.0;
+    if ( DX ) factor = factor * p.dx_inv;
+    if ( DY ) factor = factor * p.dy_inv;
+    if ( DZ ) factor = factor * p.dz_inv;
+
+    const double *c = C + kk * p.sz + jj * p.sy + ii;
+    double czy[ORDER*ORDER], cz[ORDER], B[ORDER];
+    #pragma unroll
+    for ( int i = 0; i < ORDER*ORDER; ++i ) czy[i] = 0.0;
+    #pragma unroll
+    for ( int i = 0; i < ORDER; ++i ) cz[i] = 0.0;
+    basis_strict<ORDER,DX>( x, B );
+    #pragma unroll
+    for ( int k = 0; k < ORDER; ++k )
+    #pragma unroll
+    for ( int j = 0; j < ORDER; ++j )
+    #pragma unroll
+    for ( int i = 0; i < ORDER; ++i )
+        czy[k*ORDER+j] = czy[k*ORDER+j] + c[k*p.sz + j*p.sy + i] * B[i];
+    basis_strict<ORDER,DY>( y, B );
+    #pragma unroll
+    for ( int k = 0; k < ORDER; ++k )
+    #pragma unroll
+    for ( int j = 0; j < ORDER; ++j )
+        cz[k] = cz[k] + czy[k*ORDER+j] * B[j];
+    return factor * deboor_strict<ORDER,DZ>( z, cz );
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__( BLOCK )
+void trace3d_strict_kernel( trace3_params p )
+{
+#pragma clang fp contract(off)
+    extern __shared__ double lds[];
+    constexpr int WPB = BLOCK / WAVE;
+    const node3 id = make_node3<WPB,1>( p, 0 );
+    uint32_t ix, iy, iz; double u, v, w;
+    node_coords( p, id, ix, iy, iz, u, v, w );
+    double x = p.x_min + ix * p.dx, y = p.y_min + iy * p.dy, z = p.z_min + iz * p.dz;
+    const uint32_t st = p.st;
+    const double half_dt = 0.5 * p.dt;
+
+    for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
+    {
+        const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
+        const int cnt = int( mhi - mlo + 1 );
+        __syncthreads();
+        stage3<BLOCK>( lds, p.hist + size_t( mlo ) * st, uint32_t( cnt ) * st );
+        __syncthreads();
+        for ( int jj = cnt - 1; jj >= 0; --jj )
+        {
+            const double *C = lds + size_t( jj ) * st;
+            x = x - p.dt * u; y = y - p.dt * v; z = z - p.dt * w;
+            const double Ex = -field3_strict<4,1,0,0>( x, y, z, C, p );
+            const double Ey = -field3_strict<4,0,1,0>( x, y, z, C, p );
+            const double Ez = -field3_strict<4,0,0,1>( x, y, z, C, p );
+            const double h = ( mlo + jj == 0 ) ? half_dt : p.dt;
+            u = u + h * Ex; v = v + h * Ey; w = w + h * Ez;
+        }
+    }
+    const double f = id.active ? f0_3d( p.f0, x, y, z, u, v, w ) : 0.0;
+    write_partial3( p, id, f );
+}
+
+template <typename K>
+hipError_t launch3( K kernel, uint32_t grid, int block, size_t lds, const trace3_params &p, hipStream_t st )
+{
+    if ( lds > 65536 )
+    {
+        hipError_t e = hipFuncSetAttribute( reinterpret_cast<const void*>( kernel ),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, int( lds ) );
+        if ( e != hipSuccess ) return e;
+    }
+    hipLaunchKernelGGL( kernel, dim3( grid ), dim3( block ), lds, st, p );
+    return hipGetLastError();
+}
+
+template <int BLOCK>
+hipError_t launch3_npt( const trace3_params &p, int variant, int npt, uint32_t grid, size_t lds, hipStream_t st )
+{
+    if ( variant == TRACE_FP32C )
+    {
+        switch ( npt )
+        {
+        case 1: return launch3( trace3d_fp32c_kernel<BLOCK,1>, grid, BLOCK, lds, p, st );
+        case 2: return launch3( trace3d_fp32c_kernel<BLOCK,2>, grid, BLOCK, lds, p, st );
+        case 4: return launch3( trace3d_fp32c_kernel<BLOCK,4>, grid, BLOCK, lds, p, st );
+        default: return hipErrorInvalidValue;
+        }
+    }
+    switch ( npt )
+    {
+    case 1: return launch3( trace3d_fast_kernel<BLOCK,1>, grid, BLOCK, lds, p, st );
+    case 2: return launch3( trace3d_fast_kernel<BLOCK,2>, grid, BLOCK, lds, p, st );
+    case 4: return launch3( trace3d_fast_kernel<BLOCK,4>, grid, BLOCK, lds, p, st );
+    default: return hipErrorInvalidValue;
+    }
+}
+
+} // namespace
+
+bool trace3d_shape_ok( int variant, int order, const launch_shape &s )
+{
+    if ( order != 4 || s.chunk < 1 ) return false;
+    if ( variant == TRACE_STRICT )
+        return s.npt == 1 && ( s.block == 64 || s.block == 256 );
+    return ( s.block == 256 || s.block == 512 || s.block == 1024 ) &&
+           ( s.npt == 1 || s.npt == 2 || s.npt == 4 );
+}
+
+size_t trace3d_lds_bytes( const trace3_params &p, int variant, const launch_shape &s )
+{
+    const size_t elem = ( variant == TRACE_FP32C ) ? sizeof(float) : sizeof(double);
+    const uint64_t c = p.n < uint64_t( s.chunk ) ? ( p.n > 0 ? p.n : 1 ) : uint64_t( s.chunk );
+    return size_t( c ) * p.st * elem;
+}
+
+hipError_t launch_trace3d( const trace3_params &p_in, int variant, const launch_shape &s, hipStream_t st )
+{
+    if ( !trace3d_shape_ok( variant, int( p_in.order ), s ) ) return hipErrorInvalidValue;
+    if ( p_in.n_tasks == 0 ) return hipSuccess;
+    trace3_params p = p_in;
+    const uint32_t tasks_per_block = uint32_t( s.block / WAVE ) * uint32_t( s.npt );
+    const uint32_t grid = ( p.n_tasks + tasks_per_block - 1 ) / tasks_per_block;
+    p.chunk = uint32_t( s.chunk );
+    if ( p.n < p.chunk ) p.chunk = p.n > 0 ? uint32_t( p.n ) : 1u;
+    const size_t lds = trace3d_lds_bytes( p, variant, s );
+    if ( lds > 160u * 1024u ) return hipErrorInvalidValue;
+
+    if ( variant == TRACE_STRICT )
+    {
+        if ( s.block == 64 )  return launch3( trace3d_strict_kernel<64>,  grid, 64,  lds, p, st );
+        return launch3( trace3d_strict_kernel<256>, grid, 256, lds, p, st );
+    }
+    switch ( s.block )
+    {
+    case 256:  return launch3_npt<256> ( p, variant, s.npt, grid, lds, st );
+    case 512:  return launch3_npt<512> ( p, variant, s.npt, grid, lds, st );
+    case 1024: return launch3_npt<1024>( p, variant, s.npt, grid, lds, st );
+    default:   return hipErrorInvalidValue;
+    }
+}
+
+} // namespace nufi_impl

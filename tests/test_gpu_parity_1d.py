@@ -1,0 +1,175 @@
+"""1D rho path on the GPU (through the C ABI) against the CPU oracle.
+
+Tolerances: FP64 (strict and fast) per-step parity on identical inputs
+<= 1e-12 relative (normwise, mean-free rho / slice / energy); FP32C <= 1e-6.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import numericalflowiteration_amd as nufi
+from numericalflowiteration_amd import _capi
+from tests import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32C_TOL = 1e-6
+
+
+def mean_free(a):
+    return a - a.mean()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+_hist_cache = {}
+
+
+def oracle_history(conf, n_slices):
+    key = (repr(conf), conf.f0.kind, conf.f0.alpha, conf.f0.k, n_slices)
+    if key not in _hist_cache:
+        h = O.new_history(conf, n_slices + 1)
+        e = O.run(conf, 0, n_slices, h)
+        _hist_cache[key] = (h, e)
+    return _hist_cache[key]
+
+
+SMALL_LANDAU = dict(Nx=32, Nu=128, T=2.0)       # Nt = 32
+SMALL_TWOSTREAM = dict(Nx=64, Nu=256, T=1.0)    # Nt = 32
+
+
+def make(kind):
+    if kind == "landau":
+        return nufi.landau_1d(**SMALL_LANDAU)
+    return nufi.twostream_1d(**SMALL_TWOSTREAM)
+
+
+VARIANTS = [("strict", nufi.FP64, _capi.TRACE_STRICT, FP64_TOL),
+            ("fast", nufi.FP64, _capi.TRACE_FAST, FP64_TOL),
+            ("fp32c", nufi.FP32C, _capi.TRACE_FP32C, FP32C_TOL)]
+
+
+@pytest.mark.parametrize("kind", ["landau", "twostream"])
+@pytest.mark.parametrize("name,prec,variant,tol", VARIANTS)
+def test_rho_parity_identical_history(kind, name, prec, variant, tol):
+    conf = make(kind)
+    hist, _ = oracle_history(conf, 30)
+    st = conf.stride_t()
+    print(f"[rho parity] {kind} {name} Nx={conf.Nx} Nu={conf.Nu}", flush=True)
+    with nufi.GpuScheduler(conf, precision=prec) as s:
+        s.set_trace_variant(variant)
+        s.upload_history(hist[: 30 * st], 0, 30)
+        for n in [0, 1, 2, 7, 29, 30]:
+            rho = np.zeros(conf.Nx)
+            s.compute_rho(n)
+            s.download_rho(rho)
+            ref = O.rho(conf, n, hist)
+            err = rel(mean_free(rho), mean_free(ref))
+            print(f"  n={n} rel err {err:.3e}", flush=True)
+            assert err <= tol, (n, err)
+            # GPU rho has no "+1" (cuda_kernel.cu:50): rho_gpu = ref - 1 up to rounding
+            assert abs(rho.mean() - (ref.mean() - 1.0)) < 1e-9
+
+
+@pytest.mark.parametrize("name,prec,variant,tol", VARIANTS)
+def test_step_parity_slice_and_energy(name, prec, variant, tol):
+    conf = make("twostream")
+    hist, energies = oracle_history(conf, 21)
+    st = conf.stride_t()
+    print(f"[step parity] {name}", flush=True)
+    with nufi.GpuScheduler(conf, precision=prec) as s:
+        s.set_trace_variant(variant)
+        s.upload_history(hist[: 20 * st], 0, 20)
+        e = s.step(20)
+        got = s.history(20, 21)
+    ref = hist[20 * st: 21 * st]
+    print(f"  slice rel err {rel(got, ref):.3e} energy rel err {abs(e - energies[20]) / energies[20]:.3e}", flush=True)
+    assert rel(got, ref) <= tol * 10
+    assert abs(e - energies[20]) <= tol * 10 * energies[20]
+
+
+def test_q_subranges_accumulate_to_full_rho():
+    conf = make("landau")
+    hist, _ = oracle_history(conf, 12)
+    st = conf.stride_t()
+    with nufi.GpuScheduler(conf) as s:
+        s.upload_history(hist[: 12 * st], 0, 12)
+        full = np.zeros(conf.Nx)
+        s.compute_rho(11)
+        s.download_rho(full)
+        acc = np.zeros(conf.Nx)
+        N = conf.n_nodes()
+        cuts = [0, 37, 37, 1000, 2049, N - 5, N]      # ragged, empty and unaligned ranges
+        for a, b in zip(cuts[:-1], cuts[1:]):
+            s.compute_rho(11, a, b)
+            s.download_rho(acc)                        # download_rho ADDS (cuda_kernel.cu:143-144)
+    assert rel(acc, full) < 1e-14
+
+
+def test_range_errors():
+    conf = make("landau")
+    with nufi.GpuScheduler(conf) as s:
+        with pytest.raises(nufi.NufiRangeError):
+            s.compute_rho(conf.Nt + 1)
+        with pytest.raises(nufi.NufiRangeError):
+            s.compute_rho(0, 0, conf.n_nodes() + 1)
+        with pytest.raises(nufi.NufiRangeError):
+            s.step(conf.Nt + 1)
+
+
+def test_poisson_and_fit_against_oracle():
+    conf = nufi.twostream_1d()          # Nx = 256
+    rng = np.random.default_rng(11)
+    rho = rng.standard_normal(conf.Nx)
+    with nufi.GpuScheduler(conf) as s:
+        phi = rho.copy()
+        e = s.poisson_solve(phi)
+        sl = s.interpolate(phi)
+    phi_ref, e_ref = O.poisson(conf, rho)
+    sl_ref = O.fit(conf, phi_ref)
+    assert rel(phi, phi_ref) < 1e-14
+    assert abs(e - e_ref) < 1e-13 * e_ref
+    assert rel(sl, sl_ref) < 1e-13
+
+
+@pytest.mark.parametrize("Nx", [24, 100])
+def test_non_power_of_two_grid(Nx):
+    conf = nufi.config_1d(Nx, 96, 1 / 16, 1.0, 0.0, 4 * math.pi, -10.0, 10.0, nufi.config.F0_LANDAU_1D, 0.05, 0.5)
+    hist = O.new_history(conf, conf.Nt + 1)
+    e_ref = O.run(conf, 0, conf.Nt + 1, hist)
+    with nufi.GpuScheduler(conf) as s:
+        e = s.run()
+        h = s.history()
+    st = conf.stride_t()
+    assert rel(h, hist[: (conf.Nt + 1) * st]) < 1e-11
+    assert np.max(np.abs(e - e_ref) / e_ref) < 1e-11
+
+
+@pytest.mark.parametrize("name,prec,variant,tol", VARIANTS)
+def test_whole_run_energy(name, prec, variant, tol):
+    conf = nufi.landau_1d(Nx=64, Nu=256, T=4.0)        # 64 steps
+    hist = O.new_history(conf, conf.Nt + 1)
+    e_ref = O.run(conf, 0, conf.Nt + 1, hist)
+    with nufi.GpuScheduler(conf, precision=prec) as s:
+        s.set_trace_variant(variant)
+        e = s.run()
+    err = np.max(np.abs(e - e_ref) / e_ref)
+    print(f"[whole run] {name}: max energy rel err {err:.3e}", flush=True)
+    assert err <= (1e-10 if prec == nufi.FP64 else 1e-5)
+
+
+@pytest.mark.parametrize("order", [3, 5, 6])
+def test_strict_other_orders(order):
+    conf = nufi.landau_1d(Nx=32, Nu=128, T=1.0)
+    hist = O.new_history(conf, conf.Nt + 1, order=order)
+    e_ref = O.run(conf, 0, conf.Nt + 1, hist, order=order)
+    with nufi.GpuScheduler(conf, order=order) as s:
+        e = s.run()
+        h = s.history()
+    st = conf.stride_t(order)
+    assert rel(h, hist[: (conf.Nt + 1) * st]) < 1e-11
+    assert np.max(np.abs(e - e_ref) / e_ref) < 1e-11

@@ -1,0 +1,111 @@
+"""3D3V rho path on the GPU (through the C ABI) against the CPU oracle.
+
+Tolerances: FP64 (strict, fast) <= 1e-12 relative per step on identical
+history (normwise, mean-free rho / slice / energy); FP32C <= 1e-6.
+"""
+import numpy as np
+import pytest
+
+import numericalflowiteration_amd as nufi
+from numericalflowiteration_amd import _capi
+from tests import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def mean_free(a):
+    return a - a.mean()
+
+
+_cache = {}
+
+
+def small_3d():
+    return nufi.landau_3d(N=8, V=8, T=1.0)     # 8^3 x 8^3 nodes, Nt = 8
+
+
+def oracle_history(conf, n_slices):
+    key = n_slices
+    if key not in _cache:
+        h = O.new_history(conf, n_slices + 1)
+        e = O.run(conf, 0, n_slices, h)
+        _cache[key] = (h, e)
+    return _cache[key]
+
+
+VARIANTS = [("strict", nufi.FP64, _capi.TRACE_STRICT, 1e-12),
+            ("fast", nufi.FP64, _capi.TRACE_FAST, 1e-12),
+            ("fp32c", nufi.FP32C, _capi.TRACE_FP32C, 1e-6)]
+
+
+@pytest.mark.parametrize("name,prec,variant,tol", VARIANTS)
+def test_rho_parity_3d(name, prec, variant, tol):
+    conf = small_3d()
+    hist, _ = oracle_history(conf, 6)
+    st = conf.stride_t()
+    print(f"[3d rho parity] {name}", flush=True)
+    with nufi.GpuScheduler(conf, precision=prec) as s:
+        s.set_trace_variant(variant)
+        s.upload_history(hist[: 6 * st], 0, 6)
+        for n in [0, 1, 3, 6]:
+            rho = np.zeros(conf.n_space())
+            s.compute_rho(n)
+            s.download_rho(rho)
+            ref = O.rho(conf, n, hist)
+            err = rel(mean_free(rho), mean_free(ref))
+            print(f"  n={n} rel err {err:.3e}", flush=True)
+            assert err <= tol, (n, err)
+
+
+@pytest.mark.parametrize("name,prec,variant,tol", VARIANTS)
+def test_step_parity_3d(name, prec, variant, tol):
+    conf = small_3d()
+    hist, energies = oracle_history(conf, 6)
+    st = conf.stride_t()
+    with nufi.GpuScheduler(conf, precision=prec) as s:
+        s.set_trace_variant(variant)
+        s.upload_history(hist[: 5 * st], 0, 5)
+        e = s.step(5)
+        got = s.history(5, 6)
+    ref = hist[5 * st: 6 * st]
+    print(f"[3d step] {name}: slice {rel(got, ref):.3e} energy {abs(e - energies[5]) / energies[5]:.3e}",
+          flush=True)
+    assert rel(got, ref) <= tol * 10
+    assert abs(e - energies[5]) <= tol * 10 * energies[5]
+
+
+def test_poisson_fit_3d_against_oracle():
+    conf = nufi.landau_3d()           # 16^3 grid
+    rng = np.random.default_rng(5)
+    rho = rng.standard_normal(conf.n_space())
+    with nufi.GpuScheduler(conf) as s:
+        phi = rho.copy()
+        e = s.poisson_solve(phi)
+        sl = s.interpolate(phi)
+    phi_ref, e_ref = O.poisson(conf, rho)
+    sl_ref = O.fit(conf, phi_ref)
+    assert rel(phi, phi_ref) < 1e-14
+    assert abs(e - e_ref) < 1e-13 * e_ref
+    assert rel(sl, sl_ref) < 1e-13
+
+
+def test_q_subranges_3d():
+    conf = small_3d()
+    hist, _ = oracle_history(conf, 6)
+    st = conf.stride_t()
+    with nufi.GpuScheduler(conf) as s:
+        s.upload_history(hist[: 4 * st], 0, 4)
+        full = np.zeros(conf.n_space())
+        s.compute_rho(4)
+        s.download_rho(full)
+        acc = np.zeros(conf.n_space())
+        N = conf.n_nodes()
+        cuts = [0, 1, 700, 700, 70001, N]
+        for a, b in zip(cuts[:-1], cuts[1:]):
+            s.compute_rho(4, a, b)
+            s.download_rho(acc)
+    assert rel(acc, full) < 1e-13
