@@ -1,0 +1,276 @@
+#!/usr/bin/env python3
+"""Benchmark of the NuFI rho path on one MI355X.
+
+Workload (one "step" of this benchmark = one whole run to time T, i.e. every
+time step 0..Nt of the device-resident loop: trace kernel + field kernel per
+time step): 1D two-stream instability, x in [0,10pi), f0 = (1+0.001cos(0.2x))
+v^2 exp(-v^2/2)/sqrt(2pi), v in [-8,8], Nx=256 order-4 splines, Nv=1024,
+dt=1/32, T=50 (1600 steps).  It runs in both forms: the plain FP64 method and
+the FP32-compensated trace; the reported `value` is the FP64 form's
+characteristic steps per second end to end (time-to-T), the FP32C form and
+its electric-energy error against the FP64 run are reported beside it.
+With --with-3d, the 3D3V Landau case (16^3 x 32^3, dt=1/8, T=20) is run once
+per form as well.
+
+Prints ONE JSON line (rank 0, N = 1).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "char-steps/s + time-to-T at fixed E-energy error; FP32-comp vs FP64, 1 GPU"
+FLOPS_1D = 30.0      # algorithmic FLOPs per characteristic step, order 4, 1D
+FLOPS_3D = 475.0     # 3D
+PEAK_FP64 = 78.6     # TFLOP/s, MI355X FP64 vector
+PEAK_FP32 = 157.3    # TFLOP/s, MI355X FP32 vector
+
+
+def sync_all(sched):
+    sched.synchronize()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def timed_runs(conf, precision, steps, warmup, flops_per_cs):
+    import numericalflowiteration_amd as nufi
+    with nufi.GpuScheduler(conf, precision=precision) as s:
+        for _ in range(warmup):
+            s.run()
+        sync_all(s)
+        s.set_timing(True)
+        t0 = time.perf_counter()
+        energies = None
+        for _ in range(steps):
+            energies = s.run()
+        sync_all(s)
+        wall = time.perf_counter() - t0
+        launches, kernel_ms, csteps = s.trace_timing()
+        shape = s.launch_shape()
+    cs_run = conf.char_steps(0, conf.Nt + 1)
+    achieved = flops_per_cs * csteps / launches / (kernel_ms / 1e3 / launches) / 1e12   # TFLOP/s
+    return dict(
+        wall_s=wall, ms_per_run=wall / steps * 1e3, char_steps_per_run=cs_run,
+        cs_per_s_e2e=cs_run * steps / wall,
+        cs_per_s_kernel=csteps / (kernel_ms / 1e3),
+        trace_launches=launches, trace_avg_ms=kernel_ms / launches,
+        trace_flops_per_launch=flops_per_cs * csteps / launches,
+        achieved_tflops=achieved, launch_shape=list(shape),
+    ), energies
+
+
+def read_traffic(name):
+    """HBM bytes per trace launch from the committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(conf, hist_gpu, seconds_budget=15.0):
+    """The oracle (CPU restatement of the reference path, OpenMP over x as
+    bin/test_nufi_cpu_1d.cpp:65-70) timed on isolated steps n of this workload,
+    each from the GPU FP64 run's own history (bin/test_nufi_cpu_3d_isolated.cpp
+    pattern).  Returns char-steps/s."""
+    import numpy as np
+    from tests import oracle_lib as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    st = conf.stride_t()
+    hist = np.zeros(hist_gpu.size + 8)
+    hist[: hist_gpu.size] = hist_gpu
+    total_cs, total_t, steps_done = 0, 0.0, []
+    for n in range(conf.Nt, 0, -50):
+        t = time.perf_counter()
+        O.rho(conf, n, hist, nthreads=threads)
+        dt = time.perf_counter() - t
+        total_cs += conf.n_nodes() * n
+        total_t += dt
+        steps_done.append(n)
+        if total_t > seconds_budget:
+            break
+    assert hist_gpu.size >= (conf.Nt + 1) * st
+    return dict(value=total_cs / total_t, unit="char-steps/s", cores=threads, kind="port",
+                sample=f"rho of isolated time steps n={steps_done} of the 1D two-stream run (Nx=256, Nv=1024), "
+                       f"each from the GPU FP64 history; {total_cs:.3e} char-steps in {total_t:.1f} s; "
+                       f"OpenMP over x ({threads} threads)")
+
+
+def summarize_profiles(outdir, tag, commands):
+    """Turn rocprofv3 result databases (kernel trace + FETCH_SIZE / WRITE_SIZE
+    passes + the copy-kernel calibration passes) into the committed summaries
+    profiles/<tag>.md and profiles/<tag>_pmc.json."""
+    import glob
+    import sqlite3
+
+    def db(name):
+        hits = sorted(glob.glob(os.path.join(outdir, name, "**", "*_results.db"), recursive=True))
+        return sqlite3.connect(hits[0]) if hits else None
+
+    lines = [f"# rocprofv3 summary: {tag}", "", "Commands (from the repository root on the MI355X box):", ""]
+    lines += [f"    {c}" for c in commands] + [""]
+    kt = db("prof_kt")
+    if kt is not None:
+        lines += ["## Kernel trace (--kernel-trace --stats)", "",
+                  "| kernel | calls | total us | average us | % |", "|---|---|---|---|---|"]
+        for name, calls, tot, avg, pct in kt.execute("select name, total_calls, total_duration, average, "
+                                                      "percentage from top_kernels limit 12"):
+            lines.append(f"| `{name[:90]}` | {calls} | {tot:.1f} | {avg:.3f} | {pct:.2f} |")
+        lines.append("")
+
+    def per_kernel(con):
+        return {r[0]: (r[1], r[2]) for r in con.execute(
+            "select kernel_name, count(*), avg(value) from counters_collection group by kernel_name")}
+
+    fetch, write = db("prof_fetch"), db("prof_write")
+    cfetch, cwrite = db("prof_cfetch"), db("prof_cwrite")
+    result = {}
+    if fetch is not None and write is not None and cfetch is not None and cwrite is not None:
+        cf, cw = per_kernel(cfetch), per_kernel(cwrite)
+        copy = [k for k in cf if "copyBuffer" in k][0]
+        copy_kib = float(COPY_PROBE_MIB * 1024)
+        f_scale = copy_kib / cf[copy][1]
+        w_scale = copy_kib / cw[copy][1]
+        pf, pw = per_kernel(fetch), per_kernel(write)
+        lines += ["## HBM traffic per dispatch (--pmc FETCH_SIZE, then --pmc WRITE_SIZE; KiB)", "",
+                  f"Calibration: a device-to-device copy of {COPY_PROBE_MIB} MiB reads FETCH_SIZE "
+                  f"{cf[copy][1]:.1f} KiB and writes WRITE_SIZE {cw[copy][1]:.1f} KiB, so the scale factors are "
+                  f"fetch x{f_scale:.4f}, write x{w_scale:.4f}.", "",
+                  "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | scaled HBM bytes |",
+                  "|---|---|---|---|---|"]
+        for k in pf:
+            if k in pw and "nufi" in k:
+                b = pf[k][1] * f_scale * 1024 + pw[k][1] * w_scale * 1024
+                lines.append(f"| `{k[:80]}` | {pf[k][0]} | {pf[k][1]:.1f} | {pw[k][1]:.1f} | {b:.4g} |")
+                if "trace" in k:
+                    result = {"kernel": k, "dispatches": pf[k][0], "fetch_kib_raw": pf[k][1],
+                              "write_kib_raw": pw[k][1], "fetch_scale": f_scale, "write_scale": w_scale,
+                              "hbm_bytes_per_launch": b}
+        lines.append("")
+    with open(os.path.join(ROOT, "profiles", f"{tag}.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    if result:
+        with open(os.path.join(ROOT, "profiles", f"{tag}_pmc.json"), "w") as f:
+            json.dump(result, f, indent=1)
+    return result
+
+
+COPY_PROBE_MIB = 256
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--with-3d", action="store_true", help="also run the 3D3V Landau case once per form")
+    ap.add_argument("--only", choices=["fp64", "fp32c", "both"], default="both")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy N MiB device-to-device (counter calibration), then exit")
+    ap.add_argument("--summarize-profiles", nargs=2, metavar=("OUTDIR", "TAG"),
+                    help="write profiles/TAG.md (+ _pmc.json) from rocprofv3 databases under OUTDIR")
+    ap.add_argument("--profile-commands", default="", help="';;'-separated commands recorded in the summary")
+    args = ap.parse_args()
+
+    if args.summarize_profiles:
+        r = summarize_profiles(args.summarize_profiles[0], args.summarize_profiles[1],
+                               [c for c in args.profile_commands.split(";;") if c])
+        print(json.dumps(r))
+        return
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (the rho path runs on a single device; "
+              "the multi-GPU split is a design note, not built) — run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    if args.copy_probe:
+        import torch
+        a = torch.empty(args.copy_probe * (1 << 20) // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 8}))
+        return
+
+    import numpy as np
+    import numericalflowiteration_amd as nufi
+
+    conf = nufi.twostream_1d()
+    out = {}
+    e64 = e32 = None
+    hist = None
+    if args.only in ("fp64", "both"):
+        out["fp64"], e64 = timed_runs(conf, nufi.FP64, args.steps, args.warmup, FLOPS_1D)
+        with nufi.GpuScheduler(conf) as s:
+            s.run()
+            hist = s.history()
+    if args.only in ("fp32c", "both"):
+        out["fp32c"], e32 = timed_runs(conf, nufi.FP32C, args.steps, args.warmup, FLOPS_1D)
+        if e64 is not None:
+            out["fp32c"]["energy_max_rel_err_vs_fp64"] = float(np.max(np.abs(e32 - e64) / np.abs(e64)))
+            out["fp32c"]["speedup_vs_fp64"] = out["fp32c"]["cs_per_s_e2e"] / out["fp64"]["cs_per_s_e2e"]
+
+    if args.with_3d:
+        c5 = nufi.landau_3d()
+        r64, f64 = timed_runs(c5, nufi.FP64, 1, 0, FLOPS_3D)
+        r32, f32 = timed_runs(c5, nufi.FP32C, 1, 0, FLOPS_3D)
+        r32["energy_max_rel_err_vs_fp64"] = float(np.max(np.abs(f32 - f64) / np.abs(f64)))
+        r32["speedup_vs_fp64"] = r32["cs_per_s_e2e"] / r64["cs_per_s_e2e"]
+        out["landau_3d"] = {"fp64": r64, "fp32c": r32}
+
+    prim = out.get("fp64") or out.get("fp32c")
+    prim_prec = "fp64" if "fp64" in out else "fp32c"
+    peak = PEAK_FP64 if prim_prec == "fp64" else PEAK_FP32
+    roofline = {
+        "bound": "valu", "achieved": prim["achieved_tflops"], "peak": peak, "unit": "TFLOP/s",
+        "frac": prim["achieved_tflops"] / peak,
+        "traffic": read_traffic(f"trace1d_twostream_{prim_prec}_pmc.json"),
+        "traffic_note": "HBM bytes per trace launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary",
+        "kernel": "trace1d_fast_kernel" if prim_prec == "fp64" else "trace1d_fp32c_kernel",
+        "flops_per_char_step": FLOPS_1D,
+    }
+    cpu = None
+    if not args.no_cpu and hist is not None:
+        cpu = cpu_baseline(conf, hist)
+
+    line = {
+        "metric": METRIC,
+        "value": prim["cs_per_s_e2e"],
+        "unit": "char-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": prim["ms_per_run"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prim_prec == "fp64" else "f32 (compensated) + f64",
+        "data": "synthetic: analytic initial condition, no dataset",
+        "config": {"workload": "1D two-stream instability, Nx=256 order-4 B-splines, Nv=1024, "
+                               "v in [-8,8], x in [0,10pi), dt=1/32, T=50 (1601 time steps incl. n=0)",
+                   "step": "one whole run to T (time-to-T); value = char-steps of the run / time-to-T",
+                   "char_steps_per_run": prim["char_steps_per_run"],
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "forms": out,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

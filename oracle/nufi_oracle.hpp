@@ -382,19 +382,21 @@ real f3d( size_t n, real x, real y, real z, real u, real v, real w,
 }
 
 /* ---- rho at one grid point (rho.hpp:133-146, 428-462) ------------------ */
-template <typename real, int order>
+/* acc = real reproduces the reference's sequential sum; acc = long double
+ * removes its rounding (used to measure the reference's own rounding floor). */
+template <typename real, int order, typename acc = real>
 real rho1d( size_t n, size_t i, const real *hist, const grid1<real> &g )
 {
     const real x = g.x_min + real(double(i))*g.dx;
     const real du = ( g.u_max - g.u_min ) / real(double(g.Nu));
     const real u0 = g.u_min + real_of<real>(0.5)*du;
-    real rho = real(0);
+    acc rho = acc(0);
     for ( size_t ii = 0; ii < g.Nu; ++ii )
-        rho = rho + ftilda1d<real,order>( n, x, u0 + real(double(ii))*du, hist, g );
-    return real(1) - du*rho;
+        rho = rho + acc( ftilda1d<real,order>( n, x, u0 + real(double(ii))*du, hist, g ) );
+    return real( acc(1) - acc(du)*rho );
 }
 
-template <typename real, int order>
+template <typename real, int order, typename acc = real>
 real rho3d( size_t n, size_t l, const real *hist, const grid3<real> &g )
 {
     const size_t k = l / (g.Nx*g.Ny), rem = l % (g.Nx*g.Ny), j = rem / g.Nx, i = rem % g.Nx;
@@ -407,7 +409,7 @@ real rho3d( size_t n, size_t l, const real *hist, const grid3<real> &g )
     const real u0 = g.u_min + real_of<real>(0.5)*du;
     const real v0 = g.v_min + real_of<real>(0.5)*dv;
     const real w0 = g.w_min + real_of<real>(0.5)*dw;
-    real rho = real(0);
+    acc rho = acc(0);
     for ( size_t kk = 0; kk < g.Nw; ++kk )
     for ( size_t jj = 0; jj < g.Nv; ++jj )
     for ( size_t ii = 0; ii < g.Nu; ++ii )
@@ -415,9 +417,9 @@ real rho3d( size_t n, size_t l, const real *hist, const grid3<real> &g )
         const real u = u0 + real(double(ii))*du;
         const real v = v0 + real(double(jj))*dv;
         const real w = w0 + real(double(kk))*dw;
-        rho = rho + ftilda3d<real,order>( n, x, y, z, u, v, w, hist, g );
+        rho = rho + acc( ftilda3d<real,order>( n, x, y, z, u, v, w, hist, g ) );
     }
-    return real(1) - du*dv*dw*rho;
+    return real( acc(1) - acc(du*dv*dw)*rho );
 }
 
 /* ---- discrete Hartley transform (FFTW_DHT semantics, unnormalised) ----- */

@@ -135,26 +135,30 @@ struct trace3d_op
 template <int order>
 struct rho1d_op
 {
-    static int run( const nufi_config *c, uint64_t n, const double *hist, double *rho, int nthreads )
+    static int run( const nufi_config *c, uint64_t n, const double *hist, double *rho, int nthreads,
+                    int exact_sum = 0 )
     {
         grid1<double> g( *c );
         const long long N = (long long)g.Nx;
         #pragma omp parallel for num_threads(nthreads) schedule(static)
         for ( long long i = 0; i < N; ++i )
-            rho[i] = rho1d<double,order>( n, (size_t)i, hist, g );
+            rho[i] = exact_sum ? rho1d<double,order,long double>( n, (size_t)i, hist, g )
+                               : rho1d<double,order>( n, (size_t)i, hist, g );
         return 0;
     }
 };
 template <int order>
 struct rho3d_op
 {
-    static int run( const nufi_config *c, uint64_t n, const double *hist, double *rho, int nthreads )
+    static int run( const nufi_config *c, uint64_t n, const double *hist, double *rho, int nthreads,
+                    int exact_sum = 0 )
     {
         grid3<double> g( *c );
         const long long N = (long long)( g.Nx*g.Ny*g.Nz );
         #pragma omp parallel for num_threads(nthreads) schedule(static)
         for ( long long l = 0; l < N; ++l )
-            rho[l] = rho3d<double,order>( n, (size_t)l, hist, g );
+            rho[l] = exact_sum ? rho3d<double,order,long double>( n, (size_t)l, hist, g )
+                               : rho3d<double,order>( n, (size_t)l, hist, g );
         return 0;
     }
 };
@@ -306,6 +310,16 @@ int oracle_rho1d( const nufi_config *c, int order, uint64_t n, const double *his
 
 int oracle_rho3d( const nufi_config *c, int order, uint64_t n, const double *hist, double *rho, int nthreads )
 { if ( bad( c, 3 ) ) return NUFI_ERR_INVALID; return dispatch_order<rho3d_op>( order, c, n, hist, rho, nthreads ); }
+
+/* rho with the v-sum accumulated in long double: the reference's arithmetic
+ * per node, without the rounding of its sequential FP64 sum (rho.hpp:140-143). */
+int oracle_rho1d_exact_sum( const nufi_config *c, int order, uint64_t n, const double *hist, double *rho,
+                            int nthreads )
+{ if ( bad( c, 1 ) ) return NUFI_ERR_INVALID; return dispatch_order<rho1d_op>( order, c, n, hist, rho, nthreads, 1 ); }
+
+int oracle_rho3d_exact_sum( const nufi_config *c, int order, uint64_t n, const double *hist, double *rho,
+                            int nthreads )
+{ if ( bad( c, 3 ) ) return NUFI_ERR_INVALID; return dispatch_order<rho3d_op>( order, c, n, hist, rho, nthreads, 1 ); }
 
 int oracle_poisson1d( const nufi_config *c, double *data, double *energy )
 {

@@ -1,0 +1,61 @@
+// The reference driver's step loop (bin/test_nufi_gpu_1d.cpp:241-253) written
+// against the drop-in nufi::dim1::gpu_scheduler: compute_rho -> download_rho
+// -> Poisson -> fit -> upload_phi with a host round trip per step, checked
+// against the device-resident loop (nufi_run) of a second context.
+// Usage: drop_in_1d [float]   (exit 0 on agreement)
+#include <nufi/gpu_scheduler.hpp>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+template <typename real>
+int run( double tol )
+{
+    nufi::dim1::config_t<real> conf;
+    conf.Nx = 64; conf.Nu = 256; conf.Nt = 40;
+    conf.f0_spec.kind = NUFI_F0_LANDAU_1D; conf.f0_spec.alpha = 0.01; conf.f0_spec.k = 0.5;
+    conf.derive();
+    const size_t order = 4, st = conf.Nx + order - 1;
+
+    nufi::dim1::gpu_scheduler<real,order> sched { conf };
+    std::vector<real> coeffs( ( conf.Nt + 1 ) * st, real( 0 ) );
+    std::vector<real> rho( conf.Nx );
+    std::vector<double> e_loop( conf.Nt + 1 ), buf( conf.Nx ), slice( st );
+    for ( size_t n = 0; n <= conf.Nt; ++n )
+    {
+        std::memset( rho.data(), 0, sizeof(real)*conf.Nx );
+        sched.compute_rho( n, 0, conf.Nx*conf.Nu );
+        sched.download_rho( rho.data() );
+        for ( size_t i = 0; i < conf.Nx; ++i ) buf[i] = rho[i];
+        nufi::detail::check( nufi_poisson_solve( sched.handle(), buf.data(), &e_loop[n] ), sched.handle() );
+        nufi::detail::check( nufi_interpolate( sched.handle(), slice.data(), buf.data() ), sched.handle() );
+        for ( size_t i = 0; i < st; ++i ) coeffs[n*st + i] = real( slice[i] );
+        sched.upload_phi( n, coeffs.data() );
+    }
+
+    // the same run device-resident
+    nufi::dim1::gpu_scheduler<real,order> dev { conf };
+    std::vector<double> e_dev( conf.Nt + 1 );
+    nufi::detail::check( nufi_run( dev.handle(), 0, conf.Nt + 1, e_dev.data() ), dev.handle() );
+
+    double worst = 0;
+    for ( size_t n = 0; n <= conf.Nt; ++n )
+        worst = std::fmax( worst, std::fabs( e_loop[n] - e_dev[n] ) / e_dev[n] );
+
+    bool range_ok = false;
+    try { sched.compute_rho( conf.Nt + 1, 0, 1 ); }
+    catch ( const std::range_error & ) { range_ok = true; }
+
+    std::printf( "drop_in_1d<%s>: steps 0..%zu, max energy rel diff host-loop vs device-loop %.3e, "
+                 "range_error %s\n", sizeof(real) == 8 ? "double" : "float", conf.Nt, worst,
+                 range_ok ? "ok" : "MISSING" );
+    return ( worst <= tol && range_ok ) ? 0 : 1;
+}
+
+int main( int argc, char **argv )
+{
+    if ( argc > 1 && std::strcmp( argv[1], "float" ) == 0 ) return run<float>( 1e-5 );
+    return run<double>( 1e-12 );
+}

@@ -46,6 +46,8 @@ def lib():
             "oracle_trace3d": [_cp, _int, _int, _u64, _dp, _dp, _dp],
             "oracle_rho1d": [_cp, _int, _u64, _dp, _dp, _int],
             "oracle_rho3d": [_cp, _int, _u64, _dp, _dp, _int],
+            "oracle_rho1d_exact_sum": [_cp, _int, _u64, _dp, _dp, _int],
+            "oracle_rho3d_exact_sum": [_cp, _int, _u64, _dp, _dp, _int],
             "oracle_poisson1d": [_cp, _dp, _dp],
             "oracle_poisson3d": [_cp, _dp, _dp],
             "oracle_fit1d": [_cp, _int, _dp, _dp],
@@ -129,9 +131,14 @@ def trace3d(conf: Config, n: int, xv, hist, half_kick=False, order: int = 4) -> 
     return float(out[0])
 
 
-def rho(conf: Config, n: int, hist, order: int = 4, nthreads: int | None = None) -> np.ndarray:
+def rho(conf: Config, n: int, hist, order: int = 4, nthreads: int | None = None,
+        exact_sum: bool = False) -> np.ndarray:
+    """rho at step n (rho.hpp:133-146); exact_sum accumulates the v-sum in long double."""
     out = np.zeros(conf.n_space())
-    f = lib().oracle_rho1d if conf.dim == 1 else lib().oracle_rho3d
+    if exact_sum:
+        f = lib().oracle_rho1d_exact_sum if conf.dim == 1 else lib().oracle_rho3d_exact_sum
+    else:
+        f = lib().oracle_rho1d if conf.dim == 1 else lib().oracle_rho3d
     _check(f(ctypes.byref(conf), order, n, _p(hist), _p(out), nthreads or nthreads_default()), "rho")
     return out
 

@@ -56,9 +56,14 @@ def test_rho_parity_3d(name, prec, variant, tol):
             s.compute_rho(n)
             s.download_rho(rho)
             ref = O.rho(conf, n, hist)
-            err = rel(mean_free(rho), mean_free(ref))
-            print(f"  n={n} rel err {err:.3e}", flush=True)
+            exact = O.rho(conf, n, hist, exact_sum=True)
+            floor = rel(mean_free(ref), mean_free(exact))
+            err = rel(mean_free(rho), mean_free(exact))
+            err_ref = rel(mean_free(rho), mean_free(ref))
+            print(f"  n={n} rel err vs exact-sum {err:.3e}, vs reference order {err_ref:.3e} "
+                  f"(reference floor {floor:.3e})", flush=True)
             assert err <= tol, (n, err)
+            assert err_ref <= tol + 2 * floor, (n, err_ref, floor)
 
 
 @pytest.mark.parametrize("name,prec,variant,tol", VARIANTS)
