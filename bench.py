@@ -42,11 +42,14 @@ def sync_all(sched):
         pass
 
 
-def timed_runs(conf, precision, steps, warmup, flops_per_cs):
+def timed_runs(conf, precision, steps, warmup, flops_per_cs, tune_n=None):
     import numericalflowiteration_amd as nufi
     with nufi.GpuScheduler(conf, precision=precision) as s:
         for _ in range(warmup):
             s.run()
+        tuned = None
+        if tune_n is not None:          # untimed: pick the launch shape on this history
+            tuned = s.autotune(tune_n, repeats=2)
         sync_all(s)
         s.set_timing(True)
         t0 = time.perf_counter()
@@ -66,7 +69,20 @@ def timed_runs(conf, precision, steps, warmup, flops_per_cs):
         trace_launches=launches, trace_avg_ms=kernel_ms / launches,
         trace_flops_per_launch=flops_per_cs * csteps / launches,
         achieved_tflops=achieved, launch_shape=list(shape),
+        autotune_probe=None if tuned is None else {"n": tune_n, "best_ms": tuned[1]},
     ), energies
+
+
+def energy_errors(e, ref):
+    """Electric-energy error of a run against the FP64 run: the pointwise maximum
+    of |e-ref|/ref, and the time-normwise ||e-ref||_2/||ref||_2.  The pointwise
+    figure is ill-conditioned where the energy of an oscillating mode passes near
+    zero (Landau damping); both are reported."""
+    import numpy as np
+    d = np.abs(e - ref)
+    return {"energy_max_rel_err_vs_fp64": float(np.max(d / np.abs(ref))),
+            "energy_normwise_rel_err_vs_fp64": float(np.linalg.norm(e - ref) / np.linalg.norm(ref)),
+            "energy_err_at_max_pointwise": int(np.argmax(d / np.abs(ref)))}
 
 
 def read_traffic(name):
@@ -178,6 +194,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--with-3d", action="store_true", help="also run the 3D3V Landau case once per form")
     ap.add_argument("--only", choices=["fp64", "fp32c", "both"], default="both")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--dump-energies", default="", help="directory for the per-step energy arrays (.npy)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy N MiB device-to-device (counter calibration), then exit")
     ap.add_argument("--summarize-profiles", nargs=2, metavar=("OUTDIR", "TAG"),
@@ -213,24 +231,35 @@ def main():
     out = {}
     e64 = e32 = None
     hist = None
+    tune = None if args.no_autotune else conf.Nt
     if args.only in ("fp64", "both"):
-        out["fp64"], e64 = timed_runs(conf, nufi.FP64, args.steps, args.warmup, FLOPS_1D)
+        out["fp64"], e64 = timed_runs(conf, nufi.FP64, args.steps, args.warmup, FLOPS_1D, tune)
         with nufi.GpuScheduler(conf) as s:
             s.run()
             hist = s.history()
     if args.only in ("fp32c", "both"):
-        out["fp32c"], e32 = timed_runs(conf, nufi.FP32C, args.steps, args.warmup, FLOPS_1D)
+        out["fp32c"], e32 = timed_runs(conf, nufi.FP32C, args.steps, args.warmup, FLOPS_1D, tune)
         if e64 is not None:
-            out["fp32c"]["energy_max_rel_err_vs_fp64"] = float(np.max(np.abs(e32 - e64) / np.abs(e64)))
+            out["fp32c"].update(energy_errors(e32, e64))
             out["fp32c"]["speedup_vs_fp64"] = out["fp32c"]["cs_per_s_e2e"] / out["fp64"]["cs_per_s_e2e"]
 
     if args.with_3d:
         c5 = nufi.landau_3d()
-        r64, f64 = timed_runs(c5, nufi.FP64, 1, 0, FLOPS_3D)
-        r32, f32 = timed_runs(c5, nufi.FP32C, 1, 0, FLOPS_3D)
-        r32["energy_max_rel_err_vs_fp64"] = float(np.max(np.abs(f32 - f64) / np.abs(f64)))
+        t3 = None if args.no_autotune else 16
+        r64, f64 = timed_runs(c5, nufi.FP64, 1, 0, FLOPS_3D, t3)
+        r32, f32 = timed_runs(c5, nufi.FP32C, 1, 0, FLOPS_3D, t3)
+        r32.update(energy_errors(f32, f64))
         r32["speedup_vs_fp64"] = r32["cs_per_s_e2e"] / r64["cs_per_s_e2e"]
-        out["landau_3d"] = {"fp64": r64, "fp32c": r32}
+        out["landau_3d"] = {"fp64": r64, "fp32c": r32,
+                            "workload": "3D3V Landau, x in [0,4pi)^3, (1+0.01 sum cos(0.5x_d)) M3(v), "
+                                        "v in [-6,6]^3, 16^3 x 32^3 nodes, dt=1/8, T=20 (161 steps)"}
+        if args.dump_energies:
+            np.save(os.path.join(args.dump_energies, "landau3d_energy_fp64.npy"), f64)
+            np.save(os.path.join(args.dump_energies, "landau3d_energy_fp32c.npy"), f32)
+    if args.dump_energies and e64 is not None:
+        np.save(os.path.join(args.dump_energies, "twostream_energy_fp64.npy"), e64)
+        if e32 is not None:
+            np.save(os.path.join(args.dump_energies, "twostream_energy_fp32c.npy"), e32)
 
     prim = out.get("fp64") or out.get("fp32c")
     prim_prec = "fp64" if "fp64" in out else "fp32c"

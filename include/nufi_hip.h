@@ -189,6 +189,13 @@ int nufi_set_trace_variant(nufi_ctx *ctx, int variant);
  * is not built or does not fit the LDS. */
 int nufi_set_launch_shape(nufi_ctx *ctx, int block, int nodes_per_thread, int chunk);
 int nufi_get_launch_shape(nufi_ctx *ctx, int *block, int *nodes_per_thread, int *chunk);
+/* Autotuner (no counterpart in the reference, whose launch shape is fixed at
+ * 64 threads, cuda_kernel.cu:123-124): times every built shape that fits the
+ * LDS on the trace kernel at step n_probe over all nodes (`repeats` launches
+ * each, HIP events on the context stream, history as currently on the device),
+ * keeps the fastest and returns it.  Results are overwritten partials only. */
+int nufi_autotune(nufi_ctx *ctx, uint64_t n_probe, int repeats, int *block, int *nodes_per_thread,
+                  int *chunk, double *best_ms);
 
 /* ---- timing of the trace kernel ---------------------------------------- */
 /*

@@ -56,6 +56,8 @@ SIGNATURES = {
     "nufi_set_trace_variant": (_int, [_vp, _int]),
     "nufi_set_launch_shape": (_int, [_vp, _int, _int, _int]),
     "nufi_get_launch_shape": (_int, [_vp, ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_int)]),
+    "nufi_autotune": (_int, [_vp, _u64, _int, ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_int),
+                             _dp]),
     "nufi_set_timing": (_int, [_vp, _int]),
     "nufi_trace_timing": (_int, [_vp, ctypes.POINTER(_u64), _dp, _dp]),
 }

@@ -11,6 +11,7 @@
 // (Poisson + fit).  Reference semantics are cited at each entry point.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -41,7 +42,9 @@ struct nufi_ctx
     double *hist = nullptr, *dhist = nullptr;
     float *dhist32 = nullptr, *hist32 = nullptr;
     double *partials = nullptr, *rho = nullptr, *energies = nullptr, *values = nullptr;
-    double *scratch_slice = nullptr, *metrics = nullptr;
+    double *scratch_slice = nullptr, *metrics = nullptr, *mpartials = nullptr, *energy_tmp = nullptr;
+    uint64_t m_space = 0, m_vel = 0, m_nodes = 0;   // metrics node grid
+    uint32_t m_wpx = 0;
     double2 *tw_x = nullptr, *tw_y = nullptr, *tw_z = nullptr;
     double2 *mult_step = nullptr, *mult_pois = nullptr, *mult_fit = nullptr;
     double *ewt = nullptr;
@@ -129,7 +132,7 @@ int dev_alloc( nufi_ctx *c, T **p, size_t count, const char *what )
 void free_all( nufi_ctx *c )
 {
     void *ptrs[] = { c->hist, c->dhist, c->dhist32, c->hist32, c->partials, c->rho, c->energies,
-                     c->values, c->scratch_slice, c->metrics, c->tw_x, c->tw_y, c->tw_z,
+                     c->values, c->scratch_slice, c->metrics, c->mpartials, c->energy_tmp, c->tw_x, c->tw_y, c->tw_z,
                      c->mult_step, c->mult_pois, c->mult_fit, c->ewt };
     for ( void *p : ptrs ) if ( p ) (void)hipFree( p );
     for ( auto &pr : c->ev ) { (void)hipEventDestroy( pr.first ); (void)hipEventDestroy( pr.second ); }
@@ -220,6 +223,9 @@ int build_tables( nufi_ctx *c )
     return NUFI_OK;
 }
 
+// Default launch shapes (the autotuner refines them): measured best shapes on
+// MI355X for the two-stream 1D grid (1024 threads, chunk 64) and 3D Landau 16^3
+// (FP64: 1024 threads; FP32C: 256 threads x 4 nodes).
 void default_shape( nufi_ctx *c )
 {
     const uint64_t waves = c->n_space * c->wpx;
@@ -237,11 +243,12 @@ void default_shape( nufi_ctx *c )
     }
     else
     {
-        int block = c->variant == TRACE_STRICT ? 256 : 256;
         const size_t per = size_t( c->st ) * elem;
         int chunk = int( std::min<size_t>( c->lds_cap, 64 * 1024 ) / per );
-        chunk = std::max( 1, std::min( chunk, 8 ) );
-        c->shape = launch_shape { block, 1, chunk };
+        chunk = std::max( 1, std::min( chunk, 1 ) );
+        if ( c->variant == TRACE_STRICT ) c->shape = launch_shape { 256, 1, chunk };
+        else if ( c->variant == TRACE_FP32C ) c->shape = launch_shape { 256, 4, chunk };
+        else c->shape = launch_shape { 1024, 1, chunk };
     }
 }
 
@@ -281,9 +288,10 @@ int launch_trace( nufi_ctx *c, uint64_t n, uint64_t q_begin, uint64_t q_end, uin
         p.Nx = uint32_t( c->conf.Nx ); p.Nu = uint32_t( c->conf.Nu ); p.stride_d = c->sd; p.wpx = c->wpx;
         p.ix_first = uint32_t( l0 ); p.n_tasks = uint32_t( tasks );
         p.order = uint32_t( c->order );
+        p.mode = MODE_RHO;
         p.x_min = c->conf.x_min; p.dx = c->conf.dx; p.dx_inv = c->conf.dx_inv;
-        p.Lx = c->conf.Lx; p.Lx_inv = c->conf.Lx_inv;
-        p.u_min = c->conf.u_min; p.du = c->conf.du; p.dt = c->conf.dt;
+        p.Lx = c->conf.Lx; p.Lx_inv = c->conf.Lx_inv; p.dt = c->conf.dt;
+        p.nx_min = c->conf.x_min; p.ndx = c->conf.dx; p.nu_min = c->conf.u_min; p.ndu = c->conf.du;
         p.f0 = c->conf.f0;
         e = launch_trace1d( p, c->variant, c->shape, c->stream );
     }
@@ -291,6 +299,7 @@ int launch_trace( nufi_ctx *c, uint64_t n, uint64_t q_begin, uint64_t q_end, uin
     {
         trace3_params p {};
         p.hist = c->hist; p.hist32 = c->hist32; p.partials = c->partials;
+        p.mode = MODE_RHO;
         p.n = n; p.q_begin = q_begin; p.q_end = q_end;
         p.Nx = uint32_t( c->conf.Nx ); p.Ny = uint32_t( c->conf.Ny ); p.Nz = uint32_t( c->conf.Nz );
         p.Nu = uint32_t( c->conf.Nu ); p.Nv = uint32_t( c->conf.Nv ); p.Nw = uint32_t( c->conf.Nw );
@@ -455,6 +464,13 @@ int nufi_create( const nufi_config *conf, const nufi_config *conf_metrics, int o
     if ( ( rc = dev_alloc( c, &c->energies, slices, "energies" ) ) ) return bail( rc );
     if ( ( rc = dev_alloc( c, &c->scratch_slice, c->st + 8, "scratch slice" ) ) ) return bail( rc );
     if ( ( rc = dev_alloc( c, &c->metrics, 8, "metrics" ) ) ) return bail( rc );
+    if ( ( rc = dev_alloc( c, &c->energy_tmp, 8, "energy scratch" ) ) ) return bail( rc );
+    // metrics node grid: conf_metrics in 1D (cuda_kernel.cu:97-109), conf in 3D
+    if ( c->dim == 1 ) { c->m_space = c->conf_m.Nx; c->m_vel = c->conf_m.Nu; }
+    else { c->m_space = c->n_space; c->m_vel = c->n_vel; }
+    c->m_nodes = c->m_space * c->m_vel;
+    c->m_wpx = uint32_t( ( c->m_vel + 63 ) / 64 );
+    if ( ( rc = dev_alloc( c, &c->mpartials, 4 * c->m_space * c->m_wpx, "metric partials" ) ) ) return bail( rc );
     if ( c->field_ok && ( rc = build_tables( c ) ) ) return bail( rc );
     c->host_rho.assign( c->n_space, 0.0 );
     default_shape( c );
@@ -545,16 +561,75 @@ int nufi_upload_phi( nufi_ctx *c, uint64_t n, const double *history_base )
     return NUFI_OK;
 }
 
-int nufi_compute_metrics( nufi_ctx *c, uint64_t, uint64_t, uint64_t )
+// cuda_kernel.cu:158-176 / 538-559: eval_f (with the initial half-kick from
+// slice n, rho.hpp:63-96) on the metrics grid, four weighted sums.  FP64 in
+// every precision (a diagnostic); the strict kernel in strict contexts.
+int nufi_compute_metrics( nufi_ctx *c, uint64_t n, uint64_t q_begin, uint64_t q_end )
 {
     if ( c == nullptr ) return NUFI_ERR_INVALID;
-    return fail( c, NUFI_ERR_UNSUPPORTED, "compute_metrics: not built yet" );
+    if ( n > c->conf.Nt ) return fail( c, NUFI_ERR_RANGE, "compute_metrics: time step out of range" );
+    if ( q_begin > q_end || q_end > c->m_nodes ) return fail( c, NUFI_ERR_RANGE, "compute_metrics: node range out of range" );
+    HIP_TRY( c, hipSetDevice( c->device ) );
+    uint64_t tasks = 0;
+    if ( q_begin < q_end )
+    {
+        const uint64_t l0 = q_begin / c->m_vel, l1 = ( q_end - 1 ) / c->m_vel;
+        tasks = ( l1 - l0 + 1 ) * c->m_wpx;
+        const int variant = c->variant == TRACE_STRICT ? TRACE_STRICT : TRACE_FAST;
+        hipError_t e;
+        if ( c->dim == 1 )
+        {
+            const nufi_config &g = c->conf, &m = c->conf_m;
+            trace1_params p {};
+            p.dhist = c->dhist; p.partials = c->mpartials; p.mode = MODE_METRICS;
+            p.n = n; p.q_begin = q_begin; p.q_end = q_end;
+            p.Nx = uint32_t( g.Nx ); p.Nu = uint32_t( m.Nu ); p.stride_d = c->sd; p.wpx = c->m_wpx;
+            p.ix_first = uint32_t( l0 ); p.n_tasks = uint32_t( tasks ); p.order = uint32_t( c->order );
+            p.x_min = g.x_min; p.dx = g.dx; p.dx_inv = g.dx_inv; p.Lx = g.Lx; p.Lx_inv = g.Lx_inv; p.dt = g.dt;
+            p.nx_min = m.x_min; p.ndx = m.dx; p.nu_min = m.u_min; p.ndu = m.du;
+            p.weight = m.du * m.dx;                                // cuda_kernel.cu:71
+            p.f0 = g.f0;
+            const int chunk = int( std::max<size_t>( 1, std::min<size_t>( 64, std::min<size_t>( c->lds_cap, 65536 ) / ( size_t( c->sd ) * 8 ) ) ) );
+            e = launch_trace1d( p, variant, launch_shape { 256, 1, chunk }, c->stream );
+        }
+        else
+        {
+            const nufi_config &g = c->conf;
+            trace3_params p {};
+            p.hist = c->hist; p.partials = c->mpartials; p.mode = MODE_METRICS;
+            p.weight = g.du * g.dv * g.dw;                         // cuda_kernel.cu:451 (no dx dy dz)
+            p.n = n; p.q_begin = q_begin; p.q_end = q_end;
+            p.Nx = uint32_t( g.Nx ); p.Ny = uint32_t( g.Ny ); p.Nz = uint32_t( g.Nz );
+            p.Nu = uint32_t( g.Nu ); p.Nv = uint32_t( g.Nv ); p.Nw = uint32_t( g.Nw );
+            p.sy = c->sy; p.sz = c->sz; p.st = c->st; p.wpx = c->m_wpx;
+            p.l_first = uint32_t( l0 ); p.n_tasks = uint32_t( tasks ); p.order = uint32_t( c->order );
+            p.x_min = g.x_min; p.y_min = g.y_min; p.z_min = g.z_min;
+            p.dx = g.dx; p.dy = g.dy; p.dz = g.dz;
+            p.dx_inv = g.dx_inv; p.dy_inv = g.dy_inv; p.dz_inv = g.dz_inv;
+            p.Lx = g.Lx; p.Ly = g.Ly; p.Lz = g.Lz;
+            p.Lx_inv = g.Lx_inv; p.Ly_inv = g.Ly_inv; p.Lz_inv = g.Lz_inv;
+            p.u_min = g.u_min; p.v_min = g.v_min; p.w_min = g.w_min;
+            p.du = g.du; p.dv = g.dv; p.dw = g.dw; p.dt = g.dt;
+            p.f0 = g.f0;
+            e = launch_trace3d( p, variant, launch_shape { 256, 1, 1 }, c->stream );
+        }
+        if ( e != hipSuccess ) return hip_fail( c, e, "metrics trace launch" );
+    }
+    hipError_t e = launch_reduce_metrics( c->mpartials, uint32_t( tasks ), c->metrics, c->stream );
+    if ( e != hipSuccess ) return hip_fail( c, e, "metrics reduce launch" );
+    return NUFI_OK;
 }
 
-int nufi_download_metrics( nufi_ctx *c, double * )
+// cuda_kernel.cu:178-189 — synchronise, metrics[k] += device metrics
+int nufi_download_metrics( nufi_ctx *c, double *metrics )
 {
-    if ( c == nullptr ) return NUFI_ERR_INVALID;
-    return fail( c, NUFI_ERR_UNSUPPORTED, "download_metrics: not built yet" );
+    if ( c == nullptr || metrics == nullptr ) return NUFI_ERR_INVALID;
+    HIP_TRY( c, hipSetDevice( c->device ) );
+    double m[4] = { 0, 0, 0, 0 };
+    HIP_TRY( c, hipMemcpyAsync( m, c->metrics, sizeof(m), hipMemcpyDeviceToHost, c->stream ) );
+    HIP_TRY( c, hipStreamSynchronize( c->stream ) );
+    for ( int k = 0; k < 4; ++k ) metrics[k] += m[k];
+    return NUFI_OK;
 }
 
 static int step_async( nufi_ctx *c, uint64_t n )
@@ -654,12 +729,12 @@ int nufi_poisson_solve( nufi_ctx *c, double *data, double *energy )
     HIP_TRY( c, hipSetDevice( c->device ) );
     HIP_TRY( c, hipMemcpyAsync( c->values, data, c->n_space * sizeof(double), hipMemcpyHostToDevice, c->stream ) );
     field_params f = base_field( c );
-    f.mode = FIELD_POISSON; f.values = c->values; f.mult = c->mult_pois; f.energy = c->metrics;
+    f.mode = FIELD_POISSON; f.values = c->values; f.mult = c->mult_pois; f.energy = c->energy_tmp;
     hipError_t e = launch_field( f, c->stream );
     if ( e != hipSuccess ) return hip_fail( c, e, "field kernel launch" );
     HIP_TRY( c, hipMemcpyAsync( data, c->values, c->n_space * sizeof(double), hipMemcpyDeviceToHost, c->stream ) );
     double en = 0;
-    HIP_TRY( c, hipMemcpyAsync( &en, c->metrics, sizeof(double), hipMemcpyDeviceToHost, c->stream ) );
+    HIP_TRY( c, hipMemcpyAsync( &en, c->energy_tmp, sizeof(double), hipMemcpyDeviceToHost, c->stream ) );
     HIP_TRY( c, hipStreamSynchronize( c->stream ) );
     if ( energy ) *energy = en;
     return NUFI_OK;
@@ -715,6 +790,67 @@ int nufi_get_launch_shape( nufi_ctx *c, int *block, int *npt, int *chunk )
     if ( block ) *block = c->shape.block;
     if ( npt ) *npt = c->shape.npt;
     if ( chunk ) *chunk = c->shape.chunk;
+    return NUFI_OK;
+}
+
+int nufi_autotune( nufi_ctx *c, uint64_t n_probe, int repeats, int *block, int *npt, int *chunk, double *best_ms )
+{
+    if ( c == nullptr ) return NUFI_ERR_INVALID;
+    if ( n_probe > c->conf.Nt ) return fail( c, NUFI_ERR_RANGE, "autotune: probe step out of range" );
+    if ( repeats < 1 ) repeats = 1;
+    HIP_TRY( c, hipSetDevice( c->device ) );
+    std::vector<launch_shape> cand;
+    const int blocks1[] = { 256, 512, 1024 }, npts[] = { 1, 2, 4 };
+    if ( c->dim == 1 )
+    {
+        if ( c->variant == TRACE_STRICT )
+        { for ( int b : { 256, 1024 } ) for ( int ch : { 16, 32, 64 } ) cand.push_back( { b, 1, ch } ); }
+        else
+            for ( int b : blocks1 ) for ( int np : { 1, 2 } ) for ( int ch : { 16, 32, 64 } ) cand.push_back( { b, np, ch } );
+    }
+    else
+    {
+        if ( c->variant == TRACE_STRICT ) cand.push_back( { 256, 1, 1 } );
+        else for ( int b : blocks1 ) for ( int np : npts ) for ( int ch : { 1, 2 } ) cand.push_back( { b, np, ch } );
+    }
+    const launch_shape saved = c->shape;
+    const bool saved_timing = c->timing;
+    c->timing = false;
+    hipEvent_t e0, e1;
+    HIP_TRY( c, hipEventCreate( &e0 ) );
+    HIP_TRY( c, hipEventCreate( &e1 ) );
+    double best = 1e300;
+    launch_shape best_s = saved;
+    const size_t elem = ( c->variant == TRACE_FP32C ) ? sizeof(float) : sizeof(double);
+    const size_t per = size_t( c->dim == 1 ? c->sd : c->st ) * elem;
+    int rc = NUFI_OK;
+    for ( const launch_shape &sh : cand )
+    {
+        const bool ok = c->dim == 1 ? trace1d_shape_ok( c->variant, c->order, sh ) : trace3d_shape_ok( c->variant, c->order, sh );
+        if ( !ok || size_t( sh.chunk ) * per > std::min<size_t>( c->lds_cap, 160 * 1024 ) ) continue;
+        c->shape = sh;
+        uint64_t f, n;
+        if ( ( rc = launch_trace( c, n_probe, 0, c->n_nodes, &f, &n ) ) ) break;    // warm-up
+        hipError_t he = hipEventRecord( e0, c->stream );
+        for ( int r = 0; r < repeats && he == hipSuccess && rc == NUFI_OK; ++r ) rc = launch_trace( c, n_probe, 0, c->n_nodes, &f, &n );
+        if ( he == hipSuccess ) he = hipEventRecord( e1, c->stream );
+        if ( he == hipSuccess ) he = hipEventSynchronize( e1 );
+        float ms = 0;
+        if ( he == hipSuccess ) he = hipEventElapsedTime( &ms, e0, e1 );
+        if ( he != hipSuccess ) { rc = hip_fail( c, he, "autotune timing" ); break; }
+        if ( rc ) break;
+        if ( ms / repeats < best ) { best = ms / repeats; best_s = sh; }
+    }
+    (void)hipEventDestroy( e0 );
+    (void)hipEventDestroy( e1 );
+    c->timing = saved_timing;
+    c->launches = 0; c->csteps = 0.0; c->ev_used = 0;
+    c->shape = ( rc == NUFI_OK ) ? best_s : saved;
+    if ( rc ) return rc;
+    if ( block ) *block = c->shape.block;
+    if ( npt ) *npt = c->shape.npt;
+    if ( chunk ) *chunk = c->shape.chunk;
+    if ( best_ms ) *best_ms = best;
     return NUFI_OK;
 }
 

@@ -243,7 +243,34 @@ __global__ void slice_tables_kernel( int dim, const double *slice, uint32_t len,
     }
 }
 
+// metrics[k] = sum over tasks of partials[4 g + k], compensated, fixed order
+__global__ __launch_bounds__( 256 )
+void reduce_metrics_kernel( const double *partials, uint32_t n_tasks, double *metrics )
+{
+    __shared__ double red_s[4][256], red_c[4][256];
+    csum acc[4] = { { 0, 0 }, { 0, 0 }, { 0, 0 }, { 0, 0 } };
+    for ( uint32_t g = threadIdx.x; g < n_tasks; g += 256 )
+        #pragma unroll
+        for ( int k = 0; k < 4; ++k ) acc[k] = csum_add( acc[k], csum { partials[ size_t( g ) * 4 + k ], 0.0 } );
+    #pragma unroll
+    for ( int k = 0; k < 4; ++k ) { red_s[k][threadIdx.x] = acc[k].s; red_c[k][threadIdx.x] = acc[k].c; }
+    __syncthreads();
+    if ( threadIdx.x < 4 )
+    {
+        const int k = threadIdx.x;
+        csum t { 0, 0 };
+        for ( int i = 0; i < 256; ++i ) t = csum_add( t, csum { red_s[k][i], red_c[k][i] } );
+        metrics[k] = t.s + t.c;
+    }
+}
+
 } // namespace
+
+hipError_t launch_reduce_metrics( const double *partials, uint32_t n_tasks, double *metrics, hipStream_t st )
+{
+    hipLaunchKernelGGL( reduce_metrics_kernel, dim3( 1 ), dim3( 256 ), 0, st, partials, n_tasks, metrics );
+    return hipGetLastError();
+}
 
 int field_supported( int, uint32_t Nx, uint32_t Ny, uint32_t Nz )
 {

@@ -27,18 +27,29 @@ enum trace_variant : int
 // wave-tasks: task g covers ix = ix_first + g / wpx and iu = (g % wpx)*64 +
 // lane; lanes with iu >= Nu or q outside [q_begin, q_end) contribute zero.
 // Each task writes one partial sum of f to partials[ix*wpx + g % wpx].
+// Trace kernel outputs.
+enum trace_mode : int
+{
+    MODE_RHO     = 0, // f without the initial half-kick (rho.hpp:31-61), one partial per wave
+    MODE_METRICS = 1  // f with it (rho.hpp:63-96), four weighted partials per wave
+                      // [f, f^2, |v|^2 f / 2, -f log f] (cuda_kernel.cu:53-79, 428-466)
+};
+
 struct trace1_params
 {
     const double *dhist;     // extended differences of slice m at dhist + m*stride_d (FP64)
     const float  *dhist32;   // FP32 copy (TRACE_FP32C)
-    double       *partials;
+    double       *partials;  // MODE_RHO: partials[ix*wpx + j]; MODE_METRICS: partials[4*g + k]
     uint64_t n;
     uint64_t q_begin, q_end;
-    uint32_t Nx, Nu, stride_d, wpx;
+    uint32_t Nx, Nu, stride_d, wpx;   // Nu / wpx: of the node grid
     uint32_t ix_first, n_tasks;
     uint32_t chunk;          // slices per LDS chunk
     uint32_t order;
-    double x_min, dx, dx_inv, Lx, Lx_inv, u_min, du, dt;
+    int      mode;           // trace_mode
+    double x_min, dx, dx_inv, Lx, Lx_inv, dt;    // the field grid
+    double nx_min, ndx, nu_min, ndu;             // the node grid (= field grid for rho)
+    double weight;           // MODE_METRICS quadrature weight
     nufi_f0 f0;
 };
 
@@ -49,7 +60,9 @@ struct trace3_params
 {
     const double *hist;      // coefficient slices (FP64), stride st
     const float  *hist32;    // FP32 copy (TRACE_FP32C)
-    double       *partials;
+    double       *partials;  // MODE_RHO: partials[l*wpx + j]; MODE_METRICS: partials[4*g + k]
+    int      mode;
+    double   weight;
     uint64_t n;
     uint64_t q_begin, q_end;
     uint32_t Nx, Ny, Nz, Nu, Nv, Nw;
@@ -114,6 +127,9 @@ struct field_params
 hipError_t launch_field( const field_params &p, hipStream_t st );
 size_t field_lds_bytes( const field_params &p );
 int field_supported( int dim, uint32_t Nx, uint32_t Ny, uint32_t Nz );
+
+// metrics[k] = sum_g partials[4*g + k] in a fixed order (one workgroup).
+hipError_t launch_reduce_metrics( const double *partials, uint32_t n_tasks, double *metrics, hipStream_t st );
 
 // Sum of per-wave partials into rho[ix] = weight * sum_j partials[ix*wpx+j]
 // for ix in [first, first+count), zero elsewhere (n_space entries).

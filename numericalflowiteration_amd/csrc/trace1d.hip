@@ -91,8 +91,34 @@ __device__ __forceinline__ void write_partial( const trace1_params &p, const nod
         p.partials[ size_t(id.ix) * p.wpx + id.j ] = v.s + v.c;
 }
 
+// The four metric integrands of one node (cuda_kernel.cu:73-78), wave-summed.
+__device__ __forceinline__ void write_metrics( double *partials, uint32_t g, bool task, bool active,
+                                               double w, double f, double v2 )
+{
+    double m[4];
+    m[0] = active ? w*f : 0.0;
+    m[1] = active ? w*f*f : 0.0;
+    m[2] = active ? w*(v2*f/2) : 0.0;
+    m[3] = ( active && f > 0 ) ? -w*f*log(f) : 0.0;
+    #pragma unroll
+    for ( int k = 0; k < 4; ++k )
+    {
+        const csum v = wave_csum( csum { m[k], 0.0 } );
+        if ( task && ( threadIdx.x % WAVE ) == 0 ) partials[ size_t(g) * 4 + k ] = v.s + v.c;
+    }
+}
+
+// first slice of the backward loop: n-1 for rho; n (the initial half-kick) for
+// metrics, none at n = 0 where both return f0 at the node
+template <int MODE>
+__device__ __forceinline__ int64_t top_slice( uint64_t n )
+{
+    if ( MODE == MODE_METRICS ) return n == 0 ? -1 : int64_t( n );
+    return int64_t( n ) - 1;
+}
+
 // ---- fast FP64 (order 4) ----------------------------------------------------
-template <int BLOCK, int NPT>
+template <int BLOCK, int NPT, int MODE>
 __global__ __launch_bounds__( BLOCK )
 void trace1d_fast_kernel( trace1_params p )
 {
@@ -105,8 +131,10 @@ void trace1d_fast_kernel( trace1_params p )
     for ( int r = 0; r < NPT; ++r )
     {
         id[r] = make_node<WPB,NPT>( p, r );
-        s[r] = double( id[r].ix );
-        u[r] = p.u_min + id[r].iu*p.du + p.du/2;   // cuda_kernel.cu:44
+        // node (cuda_kernel.cu:43-44); on the field grid its cell position is exactly ix
+        s[r] = ( MODE == MODE_RHO ) ? double( id[r].ix )
+                                    : ( ( p.nx_min + id[r].ix*p.ndx ) - p.x_min ) * p.dx_inv;
+        u[r] = p.nu_min + id[r].iu*p.ndu + p.ndu/2;
     }
 
     const double Nxd   = double( p.Nx );
@@ -115,7 +143,7 @@ void trace1d_fast_kernel( trace1_params p )
     const double ck    = -0.5 * p.dt * p.dx_inv;   // kick per unit 2 dphi/dt
     const uint32_t sd  = p.stride_d;
 
-    for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
+    for ( int64_t mhi = top_slice<MODE>( p.n ); mhi >= 0; mhi -= p.chunk )
     {
         const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
         const int cnt = int( mhi - mlo + 1 );
@@ -125,11 +153,13 @@ void trace1d_fast_kernel( trace1_params p )
         for ( int jj = cnt - 1; jj >= 0; --jj )
         {
             const double *D = lds + jj * sd + 1;      // D[-1] .. D[Nx+order-2]
-            const double kf = ( mlo + jj == 0 ) ? 0.5 * ck : ck;
+            const uint64_t m = uint64_t( mlo + jj );
+            const bool initial = ( MODE == MODE_METRICS ) && m == p.n;   // half-kick, no drift
+            const double kf = ( m == 0 || initial ) ? 0.5 * ck : ck;
             #pragma unroll
             for ( int r = 0; r < NPT; ++r )
             {
-                double x = fma( -cd, u[r], s[r] );
+                double x = initial ? s[r] : fma( -cd, u[r], s[r] );
                 const double w = floor( x * invNx );
                 x = fma( -Nxd, w, x );
                 const double k = floor( x );
@@ -148,7 +178,12 @@ void trace1d_fast_kernel( trace1_params p )
     for ( int r = 0; r < NPT; ++r )
     {
         const double f = id[r].active ? f0_1d( p.f0, p.x_min + s[r]*p.dx, u[r] ) : 0.0;
-        write_partial( p, id[r], f );
+        if ( MODE == MODE_RHO ) write_partial( p, id[r], f );
+        else
+        {
+            const double u0 = p.nu_min + id[r].iu*p.ndu + p.ndu/2;  // the node's velocity (cuda_kernel.cu:70)
+            write_metrics( p.partials, id[r].g, id[r].task, id[r].active, p.weight, f, u0*u0 );
+        }
     }
 }
 
@@ -185,7 +220,7 @@ __device__ __forceinline__ double efield_strict( double x, const double *D, cons
     return -( p.dx_inv * val );
 }
 
-template <int ORDER, int BLOCK>
+template <int ORDER, int BLOCK, int MODE>
 __global__ __launch_bounds__( BLOCK )
 void trace1d_strict_kernel( trace1_params p )
 {
@@ -194,12 +229,13 @@ void trace1d_strict_kernel( trace1_params p )
     constexpr int WPB = BLOCK / WAVE;
     const node_id id = make_node<WPB,1>( p, 0 );
 
-    double x = p.x_min + id.ix * p.dx;                      // cuda_kernel.cu:43
-    double u = p.u_min + id.iu * p.du + p.du / 2;           // cuda_kernel.cu:44
+    double x = p.nx_min + id.ix * p.ndx;                    // cuda_kernel.cu:43, 69
+    const double u0 = p.nu_min + id.iu * p.ndu + p.ndu / 2; // cuda_kernel.cu:44, 70
+    double u = u0;
     const uint32_t sd = p.stride_d;
     const double half_dt = 0.5 * p.dt;
 
-    for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
+    for ( int64_t mhi = top_slice<MODE>( p.n ); mhi >= 0; mhi -= p.chunk )
     {
         const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
         const int cnt = int( mhi - mlo + 1 );
@@ -208,13 +244,16 @@ void trace1d_strict_kernel( trace1_params p )
         __syncthreads();
         for ( int jj = cnt - 1; jj >= 0; --jj )
         {
-            x = x - p.dt * u;
+            const uint64_t m = uint64_t( mlo + jj );
+            const bool initial = ( MODE == MODE_METRICS ) && m == p.n;   // rho.hpp:76-79
+            if ( !initial ) x = x - p.dt * u;
             const double E = efield_strict<ORDER>( x, lds + jj * sd + 1, p );
-            u = ( mlo + jj == 0 ) ? u + half_dt * E : u + p.dt * E;
+            u = ( m == 0 || initial ) ? u + half_dt * E : u + p.dt * E;
         }
     }
     const double f = id.active ? f0_1d( p.f0, x, u ) : 0.0;
-    write_partial( p, id, f );
+    if ( MODE == MODE_RHO ) write_partial( p, id, f );
+    else write_metrics( p.partials, id.g, id.task, id.active, p.weight, f, u0*u0 );
 }
 
 // ---- FP32 compensated (order 4) --------------------------------------------
@@ -231,7 +270,7 @@ void trace1d_fp32c_kernel( trace1_params p )
     for ( int r = 0; r < NPT; ++r )
     {
         id[r] = make_node<WPB,NPT>( p, r );
-        const double u0 = p.u_min + id[r].iu*p.du + p.du/2;
+        const double u0 = p.nu_min + id[r].iu*p.ndu + p.ndu/2;
         sh[r] = float( id[r].ix ); sl[r] = 0.0f;
         uh[r] = float( u0 );       ul[r] = float( u0 - double( uh[r] ) );
     }
@@ -312,9 +351,11 @@ hipError_t launch_kernel( K kernel, uint32_t grid, int block, size_t lds, const 
 template <int BLOCK, int NPT>
 hipError_t launch_fast_or_32( const trace1_params &p, int variant, uint32_t grid, size_t lds, hipStream_t st )
 {
+    if ( p.mode == MODE_METRICS )
+        return launch_kernel( trace1d_fast_kernel<BLOCK,NPT,MODE_METRICS>, grid, BLOCK, lds, p, st );
     if ( variant == TRACE_FP32C )
         return launch_kernel( trace1d_fp32c_kernel<BLOCK,NPT>, grid, BLOCK, lds, p, st );
-    return launch_kernel( trace1d_fast_kernel<BLOCK,NPT>, grid, BLOCK, lds, p, st );
+    return launch_kernel( trace1d_fast_kernel<BLOCK,NPT,MODE_RHO>, grid, BLOCK, lds, p, st );
 }
 
 template <int BLOCK>
@@ -329,16 +370,23 @@ hipError_t launch_npt( const trace1_params &p, int variant, int npt, uint32_t gr
     }
 }
 
-template <int ORDER>
-hipError_t launch_strict( const trace1_params &p, int block, uint32_t grid, size_t lds, hipStream_t st )
+template <int ORDER, int MODE>
+hipError_t launch_strict_mode( const trace1_params &p, int block, uint32_t grid, size_t lds, hipStream_t st )
 {
     switch ( block )
     {
-    case 64:   return launch_kernel( trace1d_strict_kernel<ORDER,64>,   grid, 64,   lds, p, st );
-    case 256:  return launch_kernel( trace1d_strict_kernel<ORDER,256>,  grid, 256,  lds, p, st );
-    case 1024: return launch_kernel( trace1d_strict_kernel<ORDER,1024>, grid, 1024, lds, p, st );
+    case 64:   return launch_kernel( trace1d_strict_kernel<ORDER,64,MODE>,   grid, 64,   lds, p, st );
+    case 256:  return launch_kernel( trace1d_strict_kernel<ORDER,256,MODE>,  grid, 256,  lds, p, st );
+    case 1024: return launch_kernel( trace1d_strict_kernel<ORDER,1024,MODE>, grid, 1024, lds, p, st );
     default:   return hipErrorInvalidValue;
     }
+}
+
+template <int ORDER>
+hipError_t launch_strict( const trace1_params &p, int block, uint32_t grid, size_t lds, hipStream_t st )
+{
+    if ( p.mode == MODE_METRICS ) return launch_strict_mode<ORDER,MODE_METRICS>( p, block, grid, lds, st );
+    return launch_strict_mode<ORDER,MODE_RHO>( p, block, grid, lds, st );
 }
 
 } // namespace
@@ -354,10 +402,17 @@ bool trace1d_shape_ok( int variant, int order, const launch_shape &s )
     return bok && nok && s.chunk >= 1;
 }
 
+// slices the backward loop visits: n (rho: n-1 .. 0) or n+1 (metrics: n .. 0)
+static uint64_t visited( const trace1_params &p )
+{
+    return p.mode == MODE_METRICS ? ( p.n > 0 ? p.n + 1 : 0 ) : p.n;
+}
+
 size_t trace1d_lds_bytes( const trace1_params &p, int variant, const launch_shape &s )
 {
-    const size_t elem = ( variant == TRACE_FP32C ) ? sizeof(float) : sizeof(double);
-    const uint64_t c = p.n < uint64_t( s.chunk ) ? ( p.n > 0 ? p.n : 1 ) : uint64_t( s.chunk );
+    const size_t elem = ( variant == TRACE_FP32C && p.mode == MODE_RHO ) ? sizeof(float) : sizeof(double);
+    const uint64_t v = visited( p );
+    const uint64_t c = v < uint64_t( s.chunk ) ? ( v > 0 ? v : 1 ) : uint64_t( s.chunk );
     return size_t( c ) * p.stride_d * elem;
 }
 
@@ -369,7 +424,8 @@ hipError_t launch_trace1d( const trace1_params &p_in, int variant, const launch_
     const uint32_t tasks_per_block = uint32_t( s.block / WAVE ) * uint32_t( s.npt );
     const uint32_t grid = ( p.n_tasks + tasks_per_block - 1 ) / tasks_per_block;
     p.chunk = uint32_t( s.chunk );
-    if ( p.n < p.chunk ) p.chunk = p.n > 0 ? uint32_t( p.n ) : 1u;
+    const uint64_t v = visited( p );
+    if ( v < p.chunk ) p.chunk = v > 0 ? uint32_t( v ) : 1u;
     const size_t lds = trace1d_lds_bytes( p, variant, s );
 
     if ( variant == TRACE_STRICT )

@@ -74,6 +74,31 @@ __device__ __forceinline__ void write_partial3( const trace3_params &p, const no
         p.partials[ size_t( id.l ) * p.wpx + id.j ] = v.s + v.c;
 }
 
+// the four metric integrands (cuda_kernel.cu:453-465), wave-summed
+__device__ __forceinline__ void write_metrics3( const trace3_params &p, const node3 &id, double f, double v2 )
+{
+    const bool a = id.active;
+    const double w = p.weight;
+    double m[4];
+    m[0] = a ? w*f : 0.0;
+    m[1] = a ? w*f*f : 0.0;
+    m[2] = a ? w*v2*f/2 : 0.0;
+    m[3] = ( a && f > 0 ) ? -w*f*log(f) : 0.0;
+    #pragma unroll
+    for ( int k = 0; k < 4; ++k )
+    {
+        const csum v = wave_csum( csum { m[k], 0.0 } );
+        if ( id.task && ( threadIdx.x % WAVE ) == 0 ) p.partials[ size_t( id.g ) * 4 + k ] = v.s + v.c;
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ int64_t top_slice3( uint64_t n )
+{
+    if ( MODE == MODE_METRICS ) return n == 0 ? -1 : int64_t( n );
+    return int64_t( n ) - 1;
+}
+
 // node coordinates, cuda_kernel.cu:402-418
 __device__ __forceinline__ void node_coords( const trace3_params &p, const node3 &id,
                                              uint32_t &ix, uint32_t &iy, uint32_t &iz,
@@ -132,11 +157,11 @@ __device__ __forceinline__ void grad_window( const T *__restrict__ c, uint32_t s
     }
 }
 
-// one axis of the fast FP64 step: drift, exact wrap, locate
+// one axis of the fast FP64 step: drift (unless `still`), exact wrap, locate
 __device__ __forceinline__ int advance_axis( double &s, double vel, double cd, double N, double invN, int Ni,
-                                             double &t )
+                                             double &t, bool still = false )
 {
-    double x = fma( -cd, vel, s );
+    double x = still ? s : fma( -cd, vel, s );
     const double w = floor( x * invN );
     x = fma( -N, w, x );
     const double k = floor( x );
@@ -148,7 +173,7 @@ __device__ __forceinline__ int advance_axis( double &s, double vel, double cd, d
     return i;
 }
 
-template <int BLOCK, int NPT>
+template <int BLOCK, int NPT, int MODE>
 __global__ __launch_bounds__( BLOCK )
 void trace3d_fast_kernel( trace3_params p )
 {
@@ -172,7 +197,7 @@ void trace3d_fast_kernel( trace3_params p )
     const double kx = -p.dt * p.dx_inv, ky = -p.dt * p.dy_inv, kz = -p.dt * p.dz_inv;
     const uint32_t st = p.st, sy = p.sy, sz = p.sz;
 
-    for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
+    for ( int64_t mhi = top_slice3<MODE>( p.n ); mhi >= 0; mhi -= p.chunk )
     {
         const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
         const int cnt = int( mhi - mlo + 1 );
@@ -182,14 +207,16 @@ void trace3d_fast_kernel( trace3_params p )
         for ( int jj = cnt - 1; jj >= 0; --jj )
         {
             const double *C = lds + size_t( jj ) * st;
-            const double hf = ( mlo + jj == 0 ) ? 0.5 : 1.0;
+            const uint64_t m = uint64_t( mlo + jj );
+            const bool initial = ( MODE == MODE_METRICS ) && m == p.n;     // rho.hpp:386-394
+            const double hf = ( m == 0 || initial ) ? 0.5 : 1.0;
             #pragma unroll
             for ( int r = 0; r < NPT; ++r )
             {
                 double tx, ty, tz;
-                const int i = advance_axis( sx[r],  vu[r], cdx, Nxd, iNx, Nxi, tx );
-                const int j = advance_axis( sy_[r], vv[r], cdy, Nyd, iNy, Nyi, ty );
-                const int k = advance_axis( sz_[r], vw[r], cdz, Nzd, iNz, Nzi, tz );
+                const int i = advance_axis( sx[r],  vu[r], cdx, Nxd, iNx, Nxi, tx, initial );
+                const int j = advance_axis( sy_[r], vv[r], cdy, Nyd, iNy, Nyi, ty, initial );
+                const int k = advance_axis( sz_[r], vw[r], cdz, Nzd, iNz, Nzi, tz, initial );
                 double Bx[4], Dx[4], By[4], Dy[4], Bz[4], Dz[4];
                 cubic_bases( tx, Bx, Dx );
                 cubic_bases( ty, By, Dy );
@@ -210,7 +237,13 @@ void trace3d_fast_kernel( trace3_params p )
             ? f0_3d( p.f0, p.x_min + sx[r]*p.dx, p.y_min + sy_[r]*p.dy, p.z_min + sz_[r]*p.dz,
                      vu[r], vv[r], vw[r] )
             : 0.0;
-        write_partial3( p, id[r], f );
+        if ( MODE == MODE_RHO ) write_partial3( p, id[r], f );
+        else
+        {
+            uint32_t ix, iy, iz; double u0, v0, w0;
+            node_coords( p, id[r], ix, iy, iz, u0, v0, w0 );
+            write_metrics3( p, id[r], f, u0*u0 + v0*v0 + w0*w0 );
+        }
     }
 }
 
@@ -424,7 +457,7 @@ __device__ __forceinline__ double field3_strict( double x, double y, double z, c
     return factor * deboor_strict<ORDER,DZ>( z, cz );
 }
 
-template <int BLOCK>
+template <int BLOCK, int MODE>
 __global__ __launch_bounds__( BLOCK )
 void trace3d_strict_kernel( trace3_params p )
 {
@@ -434,11 +467,12 @@ void trace3d_strict_kernel( trace3_params p )
     const node3 id = make_node3<WPB,1>( p, 0 );
     uint32_t ix, iy, iz; double u, v, w;
     node_coords( p, id, ix, iy, iz, u, v, w );
+    const double v2 = u*u + v*v + w*w;
     double x = p.x_min + ix * p.dx, y = p.y_min + iy * p.dy, z = p.z_min + iz * p.dz;
     const uint32_t st = p.st;
     const double half_dt = 0.5 * p.dt;
 
-    for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
+    for ( int64_t mhi = top_slice3<MODE>( p.n ); mhi >= 0; mhi -= p.chunk )
     {
         const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
         const int cnt = int( mhi - mlo + 1 );
@@ -448,16 +482,19 @@ void trace3d_strict_kernel( trace3_params p )
         for ( int jj = cnt - 1; jj >= 0; --jj )
         {
             const double *C = lds + size_t( jj ) * st;
-            x = x - p.dt * u; y = y - p.dt * v; z = z - p.dt * w;
+            const uint64_t m = uint64_t( mlo + jj );
+            const bool initial = ( MODE == MODE_METRICS ) && m == p.n;
+            if ( !initial ) { x = x - p.dt * u; y = y - p.dt * v; z = z - p.dt * w; }
             const double Ex = -field3_strict<4,1,0,0>( x, y, z, C, p );
             const double Ey = -field3_strict<4,0,1,0>( x, y, z, C, p );
             const double Ez = -field3_strict<4,0,0,1>( x, y, z, C, p );
-            const double h = ( mlo + jj == 0 ) ? half_dt : p.dt;
+            const double h = ( m == 0 || initial ) ? half_dt : p.dt;
             u = u + h * Ex; v = v + h * Ey; w = w + h * Ez;
         }
     }
     const double f = id.active ? f0_3d( p.f0, x, y, z, u, v, w ) : 0.0;
-    write_partial3( p, id, f );
+    if ( MODE == MODE_RHO ) write_partial3( p, id, f );
+    else write_metrics3( p, id, f, v2 );
 }
 
 template <typename K>
@@ -476,6 +513,16 @@ hipError_t launch3( K kernel, uint32_t grid, int block, size_t lds, const trace3
 template <int BLOCK>
 hipError_t launch3_npt( const trace3_params &p, int variant, int npt, uint32_t grid, size_t lds, hipStream_t st )
 {
+    if ( p.mode == MODE_METRICS )
+    {
+        switch ( npt )
+        {
+        case 1: return launch3( trace3d_fast_kernel<BLOCK,1,MODE_METRICS>, grid, BLOCK, lds, p, st );
+        case 2: return launch3( trace3d_fast_kernel<BLOCK,2,MODE_METRICS>, grid, BLOCK, lds, p, st );
+        case 4: return launch3( trace3d_fast_kernel<BLOCK,4,MODE_METRICS>, grid, BLOCK, lds, p, st );
+        default: return hipErrorInvalidValue;
+        }
+    }
     if ( variant == TRACE_FP32C )
     {
         switch ( npt )
@@ -488,9 +535,9 @@ hipError_t launch3_npt( const trace3_params &p, int variant, int npt, uint32_t g
     }
     switch ( npt )
     {
-    case 1: return launch3( trace3d_fast_kernel<BLOCK,1>, grid, BLOCK, lds, p, st );
-    case 2: return launch3( trace3d_fast_kernel<BLOCK,2>, grid, BLOCK, lds, p, st );
-    case 4: return launch3( trace3d_fast_kernel<BLOCK,4>, grid, BLOCK, lds, p, st );
+    case 1: return launch3( trace3d_fast_kernel<BLOCK,1,MODE_RHO>, grid, BLOCK, lds, p, st );
+    case 2: return launch3( trace3d_fast_kernel<BLOCK,2,MODE_RHO>, grid, BLOCK, lds, p, st );
+    case 4: return launch3( trace3d_fast_kernel<BLOCK,4,MODE_RHO>, grid, BLOCK, lds, p, st );
     default: return hipErrorInvalidValue;
     }
 }
@@ -506,10 +553,16 @@ bool trace3d_shape_ok( int variant, int order, const launch_shape &s )
            ( s.npt == 1 || s.npt == 2 || s.npt == 4 );
 }
 
+static uint64_t visited3( const trace3_params &p )
+{
+    return p.mode == MODE_METRICS ? ( p.n > 0 ? p.n + 1 : 0 ) : p.n;
+}
+
 size_t trace3d_lds_bytes( const trace3_params &p, int variant, const launch_shape &s )
 {
-    const size_t elem = ( variant == TRACE_FP32C ) ? sizeof(float) : sizeof(double);
-    const uint64_t c = p.n < uint64_t( s.chunk ) ? ( p.n > 0 ? p.n : 1 ) : uint64_t( s.chunk );
+    const size_t elem = ( variant == TRACE_FP32C && p.mode == MODE_RHO ) ? sizeof(float) : sizeof(double);
+    const uint64_t v = visited3( p );
+    const uint64_t c = v < uint64_t( s.chunk ) ? ( v > 0 ? v : 1 ) : uint64_t( s.chunk );
     return size_t( c ) * p.st * elem;
 }
 
@@ -521,14 +574,20 @@ hipError_t launch_trace3d( const trace3_params &p_in, int variant, const launch_
     const uint32_t tasks_per_block = uint32_t( s.block / WAVE ) * uint32_t( s.npt );
     const uint32_t grid = ( p.n_tasks + tasks_per_block - 1 ) / tasks_per_block;
     p.chunk = uint32_t( s.chunk );
-    if ( p.n < p.chunk ) p.chunk = p.n > 0 ? uint32_t( p.n ) : 1u;
+    const uint64_t v = visited3( p );
+    if ( v < p.chunk ) p.chunk = v > 0 ? uint32_t( v ) : 1u;
     const size_t lds = trace3d_lds_bytes( p, variant, s );
     if ( lds > 160u * 1024u ) return hipErrorInvalidValue;
 
     if ( variant == TRACE_STRICT )
     {
-        if ( s.block == 64 )  return launch3( trace3d_strict_kernel<64>,  grid, 64,  lds, p, st );
-        return launch3( trace3d_strict_kernel<256>, grid, 256, lds, p, st );
+        if ( p.mode == MODE_METRICS )
+        {
+            if ( s.block == 64 ) return launch3( trace3d_strict_kernel<64,MODE_METRICS>, grid, 64, lds, p, st );
+            return launch3( trace3d_strict_kernel<256,MODE_METRICS>, grid, 256, lds, p, st );
+        }
+        if ( s.block == 64 )  return launch3( trace3d_strict_kernel<64,MODE_RHO>,  grid, 64,  lds, p, st );
+        return launch3( trace3d_strict_kernel<256,MODE_RHO>, grid, 256, lds, p, st );
     }
     switch ( s.block )
     {

@@ -159,6 +159,15 @@ class GpuScheduler:
         self._check(self._lib.nufi_get_launch_shape(self._ctx, ctypes.byref(b), ctypes.byref(n), ctypes.byref(c)))
         return b.value, n.value, c.value
 
+    def autotune(self, n_probe: int | None = None, repeats: int = 3):
+        """Time the built launch shapes of the trace kernel at step n_probe and keep the fastest."""
+        n_probe = self.conf.Nt if n_probe is None else n_probe
+        b, n, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms = np.zeros(1)
+        self._check(self._lib.nufi_autotune(self._ctx, n_probe, repeats, ctypes.byref(b), ctypes.byref(n),
+                                            ctypes.byref(c), _ptr(ms)))
+        return (b.value, n.value, c.value), float(ms[0])
+
     # ---- timing ----------------------------------------------------------
     def set_timing(self, enabled: bool):
         self._check(self._lib.nufi_set_timing(self._ctx, int(enabled)))

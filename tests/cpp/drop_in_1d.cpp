@@ -56,6 +56,9 @@ int run( double tol )
 
 int main( int argc, char **argv )
 {
-    if ( argc > 1 && std::strcmp( argv[1], "float" ) == 0 ) return run<float>( 1e-5 );
+    // float: the host loop rounds rho (~ -1 + 0.01 cos kx) and the coefficients to
+    // float every step, as the reference's float path does; that alone costs
+    // ~1e-5 of the perturbation per step, so the bound is 5e-4 over 40 steps.
+    if ( argc > 1 && std::strcmp( argv[1], "float" ) == 0 ) return run<float>( 5e-4 );
     return run<double>( 1e-12 );
 }

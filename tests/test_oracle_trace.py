@@ -1,0 +1,193 @@
+"""Pins the oracle's characteristic trace, rho, time loop and metrics
+(rho.hpp:31-182, 318-462; cuda_kernel.cu:53-79, 428-466) independently of its
+own code: closed-form free streaming, an independent long-double
+Stormer-Verlet with its own spline evaluator, the 3D trace reducing to the 1D
+trace, the linear Landau damping rate, and zero-field metric closed forms."""
+import math
+
+import numpy as np
+import pytest
+
+import numericalflowiteration_amd as nufi
+from numericalflowiteration_amd import config as C
+from tests import oracle_lib as O
+
+INV_SQRT_2PI = 0.39894228040143267793994
+
+
+def f0_landau(x, u, alpha, k):
+    return INV_SQRT_2PI * (1 + alpha * math.cos(k * x)) * math.exp(-u * u / 2)
+
+
+def f0_twostream(x, u, alpha, k):
+    return INV_SQRT_2PI * (1 + alpha * math.cos(k * x)) * math.exp(-u * u / 2) * u * u
+
+
+@pytest.mark.parametrize("kind", ["landau", "twostream"])
+def test_free_streaming_zero_field(kind):
+    conf = nufi.landau_1d(Nx=16, Nu=32, T=2.0) if kind == "landau" else nufi.twostream_1d(Nx=16, Nu=32, T=2.0)
+    f0 = f0_landau if kind == "landau" else f0_twostream
+    hist = O.new_history(conf, conf.Nt + 1)
+    for n in [0, 1, 5, 17]:
+        for x, u in [(0.3, 1.7), (5.0, -3.2), (1.0, 0.05)]:
+            got = O.trace1d(conf, n, x, u, hist)
+            ref = f0(x - n * conf.dt * u, u, conf.f0.alpha, conf.f0.k)
+            assert abs(got - ref) <= 1e-13 * max(abs(ref), 1e-300), (n, x, u)
+
+
+def test_rho_zero_field_closed_form():
+    conf = nufi.landau_1d(Nx=16, Nu=64, T=2.0)
+    hist = O.new_history(conf, 20)
+    n = 9
+    rho = O.rho(conf, n, hist)
+    du = (conf.u_max - conf.u_min) / conf.Nu
+    for i in range(conf.Nx):
+        x = conf.x_min + i * conf.dx
+        s = sum(f0_landau(x - n * conf.dt * u, u, conf.f0.alpha, conf.f0.k)
+                for u in (conf.u_min + 0.5 * du + ii * du for ii in range(conf.Nu)))
+        assert abs(rho[i] - (1 - du * s)) < 1e-13
+
+
+# ---- an independent static-field integrator ------------------------------
+
+def cubic_basis(t):
+    a = 1 - t
+    return [a ** 3 / 6, (3 * t ** 3 - 6 * t ** 2 + 4) / 6, (-3 * t ** 3 + 3 * t ** 2 + 3 * t + 1) / 6, t ** 3 / 6]
+
+
+def cubic_slope(t):
+    a = 1 - t
+    return [-a ** 2 / 2, (3 * t ** 2 - 4 * t) / 2, (-3 * t ** 2 + 2 * t + 1) / 2, t ** 2 / 2]
+
+
+def spline_coeffs_of(values, N):
+    """Dense solve of the periodic collocation system (independent of the oracle's fit)."""
+    A = np.zeros((N, N))
+    w = [1 / 6, 4 / 6, 1 / 6, 0]
+    for i in range(N):
+        for j in range(4):
+            A[i, (i + j) % N] += w[j]
+    c = np.linalg.solve(A, values)
+    return np.concatenate([c, c[:3]])
+
+
+def efield(slice_, x, L, N):
+    dx = L / N
+    s = (x % L) / dx
+    i = int(math.floor(s))
+    t = s - i
+    if i >= N:
+        i -= N
+    d = cubic_slope(t)
+    return -sum(slice_[i + j] * d[j] for j in range(4)) / dx
+
+
+def test_static_single_mode_field_against_independent_sv():
+    conf = nufi.landau_1d(Nx=32, Nu=16, T=3.0)          # 48 steps
+    N, L = conf.Nx, conf.Lx
+    kap = 2 * math.pi * 2 / L
+    phi = 0.3 * np.cos(kap * np.arange(N) * conf.dx)
+    sl = spline_coeffs_of(phi, N)
+    st = conf.stride_t()
+    hist = O.new_history(conf, conf.Nt + 1)
+    for m in range(conf.Nt + 1):
+        hist[m * st:(m + 1) * st] = sl
+    dt = conf.dt
+    for n in [1, 2, 13, 48]:
+        for x0, u0 in [(0.7, 0.4), (9.1, -1.3), (3.3, 2.2)]:
+            x, u = x0, u0
+            for _ in range(n - 1):                      # rho.hpp:46-52 (no initial half-kick)
+                x = x - dt * u
+                u = u + dt * efield(sl, x, L, N)
+            x = x - dt * u                              # rho.hpp:55-58
+            u = u + 0.5 * dt * efield(sl, x, L, N)
+            ref = f0_landau(x, u, conf.f0.alpha, conf.f0.k)
+            got = O.trace1d(conf, n, x0, u0, hist)
+            assert abs(got - ref) <= 1e-11 * abs(ref), (n, x0, u0, got, ref)
+            # with the initial half-kick (rho.hpp:63-96)
+            x, u = x0, u0 + 0.5 * dt * efield(sl, x0, L, N)
+            for _ in range(n - 1):
+                x = x - dt * u
+                u = u + dt * efield(sl, x, L, N)
+            x = x - dt * u
+            u = u + 0.5 * dt * efield(sl, x, L, N)
+            ref = f0_landau(x, u, conf.f0.alpha, conf.f0.k)
+            got = O.trace1d(conf, n, x0, u0, hist, half_kick=True)
+            assert abs(got - ref) <= 1e-11 * abs(ref)
+
+
+def test_3d_trace_with_an_x_only_field_matches_independent_sv():
+    """c_ijk = c_i: the 3D field is the 1D spline field in x and zero in y, z;
+    the 3D trace must follow the independent 1D integrator in x, u and leave
+    y, z, v = w = 0 to free streaming."""
+    c3 = nufi.config_3d((16, 8, 8), (4, 4, 4), 1 / 8, 2.0, (0.0, 4 * math.pi), (-3.0, 3.0), C.F0_LANDAU_3D,
+                        alpha=0.01, k=0.5)
+    c1 = nufi.config_1d(16, 4, 1 / 8, 2.0, 0.0, 4 * math.pi, -3.0, 3.0, C.F0_LANDAU_1D, alpha=0.01, k=0.5)
+    rng = np.random.default_rng(4)
+    s1, st3 = c1.stride_t(), c3.stride_t()
+    h1 = O.new_history(c1, c3.Nt + 1)
+    h3 = O.new_history(c3, c3.Nt + 1)
+    slices = []
+    for m in range(c3.Nt + 1):
+        cx = spline_coeffs_of(0.2 * rng.standard_normal(16), 16)
+        slices.append(cx)
+        h1[m * s1:(m + 1) * s1] = cx
+        h3[m * st3:(m + 1) * st3] = np.tile(cx, 11 * 11)      # c_ijk = c_i
+    L, dt, a, k = c3.Lx, c3.dt, 0.01, 0.5
+    for n in [1, 7, 16]:
+        x0, y0, z0, u0 = 2.3, 1.1, 0.4, 0.9
+        x, u = x0, u0
+        for m in range(n - 1, 0, -1):
+            x = x - dt * u
+            u = u + dt * efield(slices[m], x, L, 16)
+        x = x - dt * u
+        u = u + 0.5 * dt * efield(slices[0], x, L, 16)
+        ref = 0.06349363593424096978576330493464 * (1 + a * math.cos(k * x) + a * math.cos(k * y0)
+                                                    + a * math.cos(k * z0)) * math.exp(-u * u / 2)
+        got = O.trace3d(c3, n, [x0, y0, z0, u0, 0.0, 0.0], h3)
+        assert abs(got - ref) <= 1e-11 * ref, (n, got, ref)
+    for (x, y, z) in [(0.3, 1.0, 2.0), (7.7, 3.3, 0.1)]:
+        e1 = O.field1d(c1, h1[:s1], x, 1)
+        assert abs(O.field3d(c3, h3[:st3], x, y, z, (1, 0, 0)) - e1) < 1e-13 * max(1, abs(e1))
+        assert abs(O.field3d(c3, h3[:st3], x, y, z, (0, 1, 0))) < 1e-13
+        assert abs(O.field3d(c3, h3[:st3], x, y, z, (0, 0, 1))) < 1e-13
+
+
+def test_3d_free_streaming_zero_field():
+    c3 = nufi.landau_3d(N=8, V=4, T=1.0)
+    h = O.new_history(c3, c3.Nt + 1)
+    a, k = c3.f0.alpha, c3.f0.k
+    for n in [0, 1, 5]:
+        xv = [0.3, 2.0, 5.5, 1.2, -0.7, 2.9]
+        got = O.trace3d(c3, n, xv, h)
+        x, y, z = (xv[i] - n * c3.dt * xv[i + 3] for i in range(3))
+        ref = 0.06349363593424096978576330493464 * (1 + a * math.cos(k * x) + a * math.cos(k * y) + a * math.cos(k * z)) \
+            * math.exp(-(xv[3] ** 2 + xv[4] ** 2 + xv[5] ** 2) / 2)
+        assert abs(got - ref) <= 1e-13 * ref
+
+
+def test_linear_landau_damping_rate():
+    """k = 0.5: E decays at gamma = -0.1533 (textbook, e.g. Sonnendruecker's lecture notes)."""
+    conf = nufi.landau_1d(Nx=64, Nu=256, T=15.0)
+    hist = O.new_history(conf, conf.Nt + 1)
+    e = O.run(conf, 0, conf.Nt + 1, hist)
+    t = np.arange(e.size) * conf.dt
+    pk = [i for i in range(1, e.size - 1) if e[i] > e[i - 1] and e[i] > e[i + 1]]
+    gamma = np.polyfit(t[pk], np.log(e[pk]), 1)[0] / 2
+    assert -0.158 < gamma < -0.149, gamma
+    # n = 0 energy in closed form: rho = -alpha cos(kx) -> L alpha^2 / (4 k^2)
+    assert abs(e[0] - conf.Lx * 0.01 ** 2 / (4 * 0.25)) < 1e-6 * e[0]
+
+
+def test_metrics_zero_field_closed_forms():
+    conf = nufi.config_1d(32, 400, 1 / 16, 1.0, 0.0, 4 * math.pi, -10.0, 10.0, C.F0_LANDAU_1D, alpha=0.0, k=0.5)
+    h = O.new_history(conf, conf.Nt + 1)
+    L = conf.Lx
+    m = O.metrics(conf, 0, h)
+    assert abs(m[0] - L) < 1e-12 * L                                        # mass
+    assert abs(m[1] - L / (2 * math.sqrt(math.pi))) < 1e-12 * L             # int f^2
+    assert abs(m[2] - L / 2) < 1e-12 * L                                    # kinetic energy
+    assert abs(m[3] - L * 0.5 * (1 + math.log(2 * math.pi))) < 1e-12 * L    # entropy
+    # after free streaming of a spatially uniform f0 nothing changes
+    m5 = O.metrics(conf, 5, h)
+    assert np.allclose(m5, m, rtol=1e-13, atol=0)
