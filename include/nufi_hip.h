@@ -207,6 +207,45 @@ int nufi_set_timing(nufi_ctx *ctx, int enabled);
 int nufi_trace_timing(nufi_ctx *ctx, uint64_t *launches, double *kernel_ms,
                       double *char_steps);
 
+/* ---- adjoint ------------------------------------------------------------ */
+/*
+ * No counterpart in the reference.  Runs the whole forward loop (steps 0..Nt)
+ * and a reverse sweep through the time steps that recomputes every trace
+ * (nothing of the foot-point paths is stored), giving the gradient of
+ * J = sum_{n=0}^{Nt} dt * energy_n (energy_n as returned by the Poisson solve,
+ * poisson.cpp:73-88) with respect to the parameters of f0: a_1..a_n_modes for
+ * NUFI_F0_MULTIMODE_1D, alpha for the Landau and two-stream families.
+ * dJ_da receives n_par >= that count values; *J_out (may be NULL) receives J.
+ * 1D, order 4, FP64 contexts; the history is overwritten by the forward run.
+ */
+int nufi_adjoint_grad(nufi_ctx *ctx, double *dJ_da, int n_par, double *J_out);
+
+/* ---- batched sweep: many independent 1D runs on one device --------------- */
+/*
+ * No counterpart in the reference (which runs one configuration per
+ * programme): n_runs independent 1D configs (any Nx, Nu, domain, f0; one dt)
+ * stepped together.  Each time step is one trace launch over the wave-tasks of
+ * every run plus one field launch (a workgroup per run); nufi_batch_run with
+ * use_graph != 0 captures the step range into a hipGraph once and replays it.
+ * Each run's history and energies are bit-identical to a single-run context of
+ * the same config and precision.  Order 4; runs past their own Nt idle.
+ */
+typedef struct nufi_batch nufi_batch;
+int nufi_batch_create(const nufi_config *confs, uint64_t n_runs, int order, int precision, int device,
+                      nufi_batch **out);
+int nufi_batch_destroy(nufi_batch *b);
+const char *nufi_batch_last_error(const nufi_batch *b);
+/* Steps [n_begin, n_end) of every run (asynchronous). */
+int nufi_batch_run(nufi_batch *b, uint64_t n_begin, uint64_t n_end, int use_graph);
+int nufi_batch_synchronize(nufi_batch *b);
+int nufi_batch_energies(nufi_batch *b, uint64_t run, uint64_t n_begin, uint64_t n_end, double *out);
+int nufi_batch_history(nufi_batch *b, uint64_t run, uint64_t n_begin, uint64_t n_end, double *out);
+int nufi_batch_set_launch_shape(nufi_batch *b, int block, int nodes_per_thread, int chunk);
+int nufi_batch_get_launch_shape(nufi_batch *b, int *block, int *nodes_per_thread, int *chunk);
+/* Autotuner over the batch trace launch at step n_probe (all runs). */
+int nufi_batch_autotune(nufi_batch *b, uint64_t n_probe, int repeats, int *block, int *nodes_per_thread,
+                        int *chunk, double *best_ms);
+
 #ifdef __cplusplus
 }
 #endif

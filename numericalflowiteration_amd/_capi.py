@@ -60,6 +60,18 @@ SIGNATURES = {
                              _dp]),
     "nufi_set_timing": (_int, [_vp, _int]),
     "nufi_trace_timing": (_int, [_vp, ctypes.POINTER(_u64), _dp, _dp]),
+    "nufi_adjoint_grad": (_int, [_vp, _dp, _int, _dp]),
+    "nufi_batch_create": (_int, [_cp, _u64, _int, _int, _int, ctypes.POINTER(_vp)]),
+    "nufi_batch_destroy": (_int, [_vp]),
+    "nufi_batch_last_error": (ctypes.c_char_p, [_vp]),
+    "nufi_batch_run": (_int, [_vp, _u64, _u64, _int]),
+    "nufi_batch_synchronize": (_int, [_vp]),
+    "nufi_batch_energies": (_int, [_vp, _u64, _u64, _u64, _dp]),
+    "nufi_batch_history": (_int, [_vp, _u64, _u64, _u64, _dp]),
+    "nufi_batch_set_launch_shape": (_int, [_vp, _int, _int, _int]),
+    "nufi_batch_get_launch_shape": (_int, [_vp, ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_int)]),
+    "nufi_batch_autotune": (_int, [_vp, _u64, _int, ctypes.POINTER(_int), ctypes.POINTER(_int),
+                                   ctypes.POINTER(_int), _dp]),
 }
 
 
@@ -100,10 +112,11 @@ def load():
     return lib
 
 
-def check(rc: int, ctx=None):
+def check(rc: int, ctx=None, batch: bool = False):
     if rc == OK:
         return
-    msg = load().nufi_last_error(ctx).decode(errors="replace")
+    lib = load()
+    msg = (lib.nufi_batch_last_error(ctx) if batch else lib.nufi_last_error(ctx)).decode(errors="replace")
     if rc == ERR_RANGE:
         raise NufiRangeError(rc, msg)
     raise NufiError(rc, msg)

@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "host_tables.hpp"
 #include "internal.hpp"
 
 using namespace nufi_impl;
@@ -43,6 +44,7 @@ struct nufi_ctx
     float *dhist32 = nullptr, *hist32 = nullptr;
     double *partials = nullptr, *rho = nullptr, *energies = nullptr, *values = nullptr;
     double *scratch_slice = nullptr, *metrics = nullptr, *mpartials = nullptr, *energy_tmp = nullptr;
+    double *rho_hist = nullptr, *lambda = nullptr, *gvec = nullptr, *grad_part = nullptr, *grad = nullptr;
     uint64_t m_space = 0, m_vel = 0, m_nodes = 0;   // metrics node grid
     uint32_t m_wpx = 0;
     double2 *tw_x = nullptr, *tw_y = nullptr, *tw_z = nullptr;
@@ -86,38 +88,6 @@ uint32_t log2_or_none( uint64_t n )
     return l;
 }
 
-// B-spline values at t = 0 of the `order` bases that are non-zero on [0,1)
-// (the collocation weights of fields.hpp:70-75), by the Cox-de Boor recurrence.
-std::vector<long double> basis_at_zero( int order )
-{
-    std::vector<long double> w( order, 0.0L );
-    w[order-1] = 1.0L;
-    const long double t = 0.0L;
-    for ( int m = 1; m < order; ++m )
-    {
-        w[order-m-1] = ( 1.0L - t ) * w[order-m];
-        for ( int s = 1 - m; s < 0; ++s )
-            w[order-1+s] = ( t - s ) * w[order-1+s] + ( ( m + 1 + s ) - t ) * w[order+s];
-        w[order-1] *= t;
-    }
-    long double f = 1.0L;
-    for ( int i = 2; i <= order - 1; ++i ) f *= i;
-    for ( auto &v : w ) v /= f;
-    return w;
-}
-
-std::vector<double2> twiddles( uint64_t L )
-{
-    std::vector<double2> tw( L ? L : 1 );
-    const long double two_pi = 6.283185307179586476925286766559L;
-    for ( uint64_t m = 0; m < L; ++m )
-    {
-        const long double a = -two_pi * (long double)m / (long double)L;
-        tw[m] = make_double2( (double)std::cos( a ), (double)std::sin( a ) );
-    }
-    return tw;
-}
-
 template <typename T>
 int dev_alloc( nufi_ctx *c, T **p, size_t count, const char *what )
 {
@@ -132,7 +102,8 @@ int dev_alloc( nufi_ctx *c, T **p, size_t count, const char *what )
 void free_all( nufi_ctx *c )
 {
     void *ptrs[] = { c->hist, c->dhist, c->dhist32, c->hist32, c->partials, c->rho, c->energies,
-                     c->values, c->scratch_slice, c->metrics, c->mpartials, c->energy_tmp, c->tw_x, c->tw_y, c->tw_z,
+                     c->values, c->scratch_slice, c->metrics, c->mpartials, c->energy_tmp,
+                     c->rho_hist, c->lambda, c->gvec, c->grad_part, c->grad, c->tw_x, c->tw_y, c->tw_z,
                      c->mult_step, c->mult_pois, c->mult_fit, c->ewt };
     for ( void *p : ptrs ) if ( p ) (void)hipFree( p );
     for ( auto &pr : c->ev ) { (void)hipEventDestroy( pr.first ); (void)hipEventDestroy( pr.second ); }
@@ -155,71 +126,26 @@ int validate( const nufi_config *c, int order )
     return NUFI_OK;
 }
 
-// Host precomputation of the Fourier tables (long double).
+// Host precomputation of the Fourier tables (long double), uploaded once.
 int build_tables( nufi_ctx *c )
 {
-    const nufi_config &g = c->conf;
-    const uint64_t Nx = g.Nx, Ny = c->dim == 3 ? g.Ny : 1, Nz = c->dim == 3 ? g.Nz : 1;
-    const uint64_t N = Nx * Ny * Nz;
-    const std::vector<long double> B = basis_at_zero( c->order );
-    const long double two_pi = 6.283185307179586476925286766559L;
-    using cld = std::complex<long double>;
-    auto symbol = [&]( uint64_t k, uint64_t L ) {
-        cld s = 0;
-        for ( int j = 0; j < c->order; ++j )
-        {
-            const long double a = two_pi * (long double)( ( uint64_t( j ) * k ) % L ) / (long double)L;
-            s += B[j] * cld( std::cos( a ), std::sin( a ) );
-        }
-        return s;
-    };
-    auto fold = []( uint64_t i, uint64_t L ) { return (long double)( ( 2*i < L ) ? i : L - i ); };
-    const long double Lx = g.Lx, Ly = c->dim == 3 ? g.Ly : 1.0L, Lz = c->dim == 3 ? g.Lz : 1.0L;
-    const long double fx = two_pi / Lx, fy = two_pi / Ly, fz = two_pi / Lz;
-    const long double vol = Lx * Ly * Lz;
-
-    std::vector<double2> ms( N ), mp( N ), mf( N );
-    std::vector<double> ew( N );
-    for ( uint64_t k = 0; k < Nz; ++k )
-    for ( uint64_t j = 0; j < Ny; ++j )
-    for ( uint64_t i = 0; i < Nx; ++i )
-    {
-        const uint64_t idx = ( k * Ny + j ) * Nx + i;
-        const cld s = symbol( i, Nx ) * symbol( j, Ny ) * symbol( k, Nz );
-        const long double ii = fold( i, Nx ) * fx, jj = fold( j, Ny ) * fy, kk = fold( k, Nz ) * fz;
-        const long double kap2 = ii*ii + jj*jj + kk*kk;
-        const bool singular = std::abs( s ) < 1e-12L;
-        const cld fit = singular ? cld( 0 ) : cld( 1 ) / ( (long double)N * s );
-        mf[idx] = make_double2( (double)fit.real(), (double)fit.imag() );
-        if ( idx == 0 )
-        {
-            ms[idx] = make_double2( 0.0, 0.0 );
-            mp[idx] = make_double2( 0.0, 0.0 );
-            ew[idx] = 0.0;
-            continue;
-        }
-        const long double pois = 1.0L / ( (long double)N * kap2 );
-        mp[idx] = make_double2( (double)pois, 0.0 );
-        const cld st = singular ? cld( 0 ) : cld( 1 ) / ( (long double)N * kap2 * s );
-        ms[idx] = make_double2( (double)st.real(), (double)st.imag() );
-        ew[idx] = (double)( ( vol / 2.0L ) / ( (long double)N * (long double)N * kap2 ) );
-    }
-    const auto tx = twiddles( Nx ), ty = twiddles( Ny ), tz = twiddles( Nz );
+    const fourier_tables t = make_tables( c->conf, c->dim, c->order );
+    const size_t N = t.ms.size();
     int rc;
-    if ( ( rc = dev_alloc( c, &c->tw_x, tx.size(), "twiddles" ) ) ) return rc;
-    if ( ( rc = dev_alloc( c, &c->tw_y, ty.size(), "twiddles" ) ) ) return rc;
-    if ( ( rc = dev_alloc( c, &c->tw_z, tz.size(), "twiddles" ) ) ) return rc;
+    if ( ( rc = dev_alloc( c, &c->tw_x, t.tx.size(), "twiddles" ) ) ) return rc;
+    if ( ( rc = dev_alloc( c, &c->tw_y, t.ty.size(), "twiddles" ) ) ) return rc;
+    if ( ( rc = dev_alloc( c, &c->tw_z, t.tz.size(), "twiddles" ) ) ) return rc;
     if ( ( rc = dev_alloc( c, &c->mult_step, N, "multipliers" ) ) ) return rc;
     if ( ( rc = dev_alloc( c, &c->mult_pois, N, "multipliers" ) ) ) return rc;
     if ( ( rc = dev_alloc( c, &c->mult_fit, N, "multipliers" ) ) ) return rc;
     if ( ( rc = dev_alloc( c, &c->ewt, N, "energy weights" ) ) ) return rc;
-    HIP_TRY( c, hipMemcpy( c->tw_x, tx.data(), tx.size() * sizeof(double2), hipMemcpyHostToDevice ) );
-    HIP_TRY( c, hipMemcpy( c->tw_y, ty.data(), ty.size() * sizeof(double2), hipMemcpyHostToDevice ) );
-    HIP_TRY( c, hipMemcpy( c->tw_z, tz.data(), tz.size() * sizeof(double2), hipMemcpyHostToDevice ) );
-    HIP_TRY( c, hipMemcpy( c->mult_step, ms.data(), N * sizeof(double2), hipMemcpyHostToDevice ) );
-    HIP_TRY( c, hipMemcpy( c->mult_pois, mp.data(), N * sizeof(double2), hipMemcpyHostToDevice ) );
-    HIP_TRY( c, hipMemcpy( c->mult_fit, mf.data(), N * sizeof(double2), hipMemcpyHostToDevice ) );
-    HIP_TRY( c, hipMemcpy( c->ewt, ew.data(), N * sizeof(double), hipMemcpyHostToDevice ) );
+    HIP_TRY( c, hipMemcpy( c->tw_x, t.tx.data(), t.tx.size() * sizeof(double2), hipMemcpyHostToDevice ) );
+    HIP_TRY( c, hipMemcpy( c->tw_y, t.ty.data(), t.ty.size() * sizeof(double2), hipMemcpyHostToDevice ) );
+    HIP_TRY( c, hipMemcpy( c->tw_z, t.tz.data(), t.tz.size() * sizeof(double2), hipMemcpyHostToDevice ) );
+    HIP_TRY( c, hipMemcpy( c->mult_step, t.ms.data(), N * sizeof(double2), hipMemcpyHostToDevice ) );
+    HIP_TRY( c, hipMemcpy( c->mult_pois, t.mp.data(), N * sizeof(double2), hipMemcpyHostToDevice ) );
+    HIP_TRY( c, hipMemcpy( c->mult_fit, t.mf.data(), N * sizeof(double2), hipMemcpyHostToDevice ) );
+    HIP_TRY( c, hipMemcpy( c->ewt, t.ew.data(), N * sizeof(double), hipMemcpyHostToDevice ) );
     return NUFI_OK;
 }
 
@@ -632,7 +558,16 @@ int nufi_download_metrics( nufi_ctx *c, double *metrics )
     return NUFI_OK;
 }
 
-static int step_async( nufi_ctx *c, uint64_t n )
+static int step_async( nufi_ctx *c, uint64_t n, double *rho_out = nullptr );
+
+int adjoint_npar( const nufi_f0 &f )
+{
+    if ( f.kind == NUFI_F0_MULTIMODE_1D ) return f.n_modes;
+    if ( f.kind == NUFI_F0_LANDAU_1D || f.kind == NUFI_F0_TWOSTREAM_1D ) return 1;
+    return 0;
+}
+
+static int step_async( nufi_ctx *c, uint64_t n, double *rho_out )
 {
     if ( n > c->conf.Nt ) return fail( c, NUFI_ERR_RANGE, "step: time step out of range" );
     if ( !c->field_ok ) return fail( c, NUFI_ERR_UNSUPPORTED, "step: grid too large for the single-workgroup field solve" );
@@ -648,7 +583,7 @@ static int step_async( nufi_ctx *c, uint64_t n )
     f.dslice32 = c->dhist32 ? c->dhist32 + n * c->sd : nullptr;
     f.slice32 = c->hist32 ? c->hist32 + n * c->st : nullptr;
     f.energy = c->energies + n;
-    f.rho_out = c->rho;
+    f.rho_out = rho_out ? rho_out : c->rho;
     hipError_t e = launch_field( f, c->stream );
     if ( e != hipSuccess ) return hip_fail( c, e, "field kernel launch" );
     return NUFI_OK;
@@ -683,6 +618,74 @@ int nufi_run( nufi_ctx *c, uint64_t n_begin, uint64_t n_end, double *energies )
         HIP_TRY( c, hipMemcpyAsync( energies, c->energies + n_begin, ( n_end - n_begin ) * sizeof(double),
                                     hipMemcpyDeviceToHost, c->stream ) );
         HIP_TRY( c, hipStreamSynchronize( c->stream ) );
+    }
+    return NUFI_OK;
+}
+
+// Gradient of J = sum_{n=0}^{Nt} dt * energy_n with respect to the initial
+// condition's parameters (adjoint1d.hip): forward run storing rho_n, then the
+// reverse sweep.  1D, order 4, FP64.
+int nufi_adjoint_grad( nufi_ctx *c, double *dJ_da, int n_par, double *J_out )
+{
+    if ( c == nullptr ) return NUFI_ERR_INVALID;
+    if ( c->dim != 1 || c->order != 4 || c->precision != NUFI_FP64 )
+        return fail( c, NUFI_ERR_UNSUPPORTED, "adjoint_grad: built for 1D, order 4, FP64" );
+    const int npar = adjoint_npar( c->conf.f0 );
+    if ( npar == 0 ) return fail( c, NUFI_ERR_UNSUPPORTED, "adjoint_grad: no parameters for this f0 family" );
+    if ( n_par < npar || dJ_da == nullptr ) return fail( c, NUFI_ERR_INVALID, "adjoint_grad: output too short" );
+    if ( !c->field_ok || c->conf.Nx > 2048 ) return fail( c, NUFI_ERR_UNSUPPORTED, "adjoint_grad: grid too large" );
+    HIP_TRY( c, hipSetDevice( c->device ) );
+    const uint64_t Nt = c->conf.Nt, Nx = c->conf.Nx;
+    const uint32_t n_tasks = uint32_t( c->n_space * c->wpx );
+    int rc;
+    if ( c->rho_hist == nullptr )
+    {
+        if ( ( rc = dev_alloc( c, &c->rho_hist, ( Nt + 1 ) * Nx, "adjoint rho history" ) ) ) return rc;
+        if ( ( rc = dev_alloc( c, &c->lambda, ( Nt + 1 ) * Nx, "adjoint slice adjoints" ) ) ) return rc;
+        if ( ( rc = dev_alloc( c, &c->gvec, Nx, "adjoint g" ) ) ) return rc;
+        if ( ( rc = dev_alloc( c, &c->grad_part, size_t( n_tasks ) * NUFI_MAX_MODES, "adjoint partials" ) ) ) return rc;
+        if ( ( rc = dev_alloc( c, &c->grad, NUFI_MAX_MODES, "adjoint gradient" ) ) ) return rc;
+    }
+    // forward run (the FP64 fast trace, as nufi_run), keeping rho_n
+    const int saved_variant = c->variant;
+    c->variant = TRACE_FAST;
+    for ( uint64_t n = 0; n <= Nt && ( rc = step_async( c, n, c->rho_hist + n * Nx ) ) == NUFI_OK; ++n ) {}
+    c->variant = saved_variant;
+    if ( rc ) return rc;
+    HIP_TRY( c, hipMemsetAsync( c->lambda, 0, ( Nt + 1 ) * Nx * sizeof(double), c->stream ) );
+    HIP_TRY( c, hipMemsetAsync( c->grad_part, 0, size_t( n_tasks ) * NUFI_MAX_MODES * sizeof(double), c->stream ) );
+
+    adjoint_field_params af {};
+    af.N = uint32_t( Nx ); af.log2x = log2_or_none( Nx );
+    af.g = c->gvec; af.tw = c->tw_x; af.mult = c->mult_step; af.ewt = c->ewt; af.dt = c->conf.dt;
+    adjoint1_params ap {};
+    ap.dhist = c->dhist; ap.g = c->gvec; ap.lambda = c->lambda; ap.grad_part = c->grad_part;
+    ap.Nx = uint32_t( Nx ); ap.Nu = uint32_t( c->conf.Nu ); ap.stride_d = c->sd; ap.wpx = c->wpx;
+    ap.n_tasks = n_tasks; ap.npar = uint32_t( npar );
+    ap.x_min = c->conf.x_min; ap.dx = c->conf.dx; ap.dx_inv = c->conf.dx_inv; ap.dt = c->conf.dt;
+    ap.u_min = c->conf.u_min; ap.du = c->conf.du; ap.f0 = c->conf.f0;
+    ap.chunk = uint32_t( std::max<size_t>( 1, std::min<size_t>( 32, std::min<size_t>( c->lds_cap, 65536 ) /
+                                                                  ( ( size_t( c->sd ) + Nx ) * 8 ) ) ) );
+    for ( uint64_t n = Nt + 1; n-- > 0; )
+    {
+        af.rho = c->rho_hist + n * Nx; af.lambda = c->lambda + n * Nx;
+        hipError_t e = launch_adjoint_field( af, c->stream );
+        if ( e != hipSuccess ) return hip_fail( c, e, "adjoint field launch" );
+        ap.n = n;
+        e = launch_adjoint1d( ap, c->stream );
+        if ( e != hipSuccess ) return hip_fail( c, e, "adjoint trace launch" );
+    }
+    hipError_t e = launch_reduce_grad( c->grad_part, n_tasks, uint32_t( npar ), c->grad, c->stream );
+    if ( e != hipSuccess ) return hip_fail( c, e, "adjoint reduce launch" );
+    std::vector<double> en( Nt + 1 );
+    HIP_TRY( c, hipMemcpyAsync( dJ_da, c->grad, npar * sizeof(double), hipMemcpyDeviceToHost, c->stream ) );
+    HIP_TRY( c, hipMemcpyAsync( en.data(), c->energies, ( Nt + 1 ) * sizeof(double), hipMemcpyDeviceToHost, c->stream ) );
+    HIP_TRY( c, hipStreamSynchronize( c->stream ) );
+    if ( J_out )
+    {
+        double J = 0;
+        for ( double v : en ) J += c->conf.dt * v;
+        *J_out = J;
     }
     return NUFI_OK;
 }

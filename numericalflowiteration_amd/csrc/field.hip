@@ -19,6 +19,7 @@
 // is not a power of two), ping-ponged between two LDS buffers.
 #include "device_common.hpp"
 #include "internal.hpp"
+#include "fft_lds.hpp"
 
 namespace nufi_impl
 {
@@ -27,82 +28,6 @@ using namespace nufi_dev;
 
 namespace
 {
-
-constexpr int FIELD_BLOCK = 512;
-
-__device__ __forceinline__ double2 cmul( double2 a, double2 b )
-{
-    return make_double2( a.x*b.x - a.y*b.y, a.x*b.y + a.y*b.x );
-}
-
-struct axis_desc
-{
-    uint32_t L, log2L, S;       // length, log2 (or ~0u), stride
-    uint32_t nlines;
-    uint32_t mode;              // 0: x lines, 1: y lines, 2: z lines
-    uint32_t Nx, Ny;
-};
-
-__device__ __forceinline__ uint32_t line_base( const axis_desc &a, uint32_t line )
-{
-    if ( a.mode == 0 ) return line * a.Nx;
-    if ( a.mode == 1 ) return ( line / a.Nx ) * a.Nx * a.Ny + ( line % a.Nx );
-    return line;
-}
-
-// Transform all lines along one axis; returns the buffer holding the result.
-__device__ double2 *fft_axis( double2 *in, double2 *out, const axis_desc &a, const double2 *tw, bool inverse )
-{
-    const uint32_t L = a.L;
-    if ( L <= 1 ) return in;
-    if ( a.log2L == 0xffffffffu )
-    {
-        // direct DFT, O(L^2) per line, exact angle reduction j*k mod L
-        const uint32_t total = a.nlines * L;
-        for ( uint32_t e = threadIdx.x; e < total; e += FIELD_BLOCK )
-        {
-            const uint32_t line = e / L, k = e % L;
-            const uint32_t base = line_base( a, line );
-            double2 acc = make_double2( 0.0, 0.0 );
-            uint32_t m = 0;
-            for ( uint32_t j = 0; j < L; ++j )
-            {
-                double2 w = tw[m];
-                if ( inverse ) w.y = -w.y;
-                const double2 v = cmul( in[ base + j*a.S ], w );
-                acc.x += v.x; acc.y += v.y;
-                m += k; if ( m >= L ) m -= L;
-            }
-            out[ base + k*a.S ] = acc;
-        }
-        __syncthreads();
-        return out;
-    }
-    const uint32_t half = L >> 1, lh = a.log2L - 1;
-    const uint32_t total = a.nlines * half;
-    uint32_t log2Ns = 0;
-    for ( uint32_t Ns = 1; Ns < L; Ns <<= 1, ++log2Ns )
-    {
-        const uint32_t twstep = L >> ( log2Ns + 1 );       // L / (2 Ns)
-        for ( uint32_t e = threadIdx.x; e < total; e += FIELD_BLOCK )
-        {
-            const uint32_t line = e >> lh, j = e & ( half - 1 );
-            const uint32_t base = line_base( a, line );
-            const double2 x0 = in[ base + j*a.S ];
-            double2 x1 = in[ base + ( j + half )*a.S ];
-            const uint32_t k = j & ( Ns - 1 );
-            double2 w = tw[ k * twstep ];
-            if ( inverse ) w.y = -w.y;
-            x1 = cmul( x1, w );
-            const uint32_t o = ( ( j >> log2Ns ) << ( log2Ns + 1 ) ) + k;
-            out[ base + o*a.S ]        = make_double2( x0.x + x1.x, x0.y + x1.y );
-            out[ base + ( o + Ns )*a.S ] = make_double2( x0.x - x1.x, x0.y - x1.y );
-        }
-        __syncthreads();
-        double2 *t = in; in = out; out = t;
-    }
-    return in;
-}
 
 __device__ double2 *fft3( double2 *A, double2 *B, const field_params &p, bool inverse )
 {
@@ -117,10 +42,8 @@ __device__ double2 *fft3( double2 *A, double2 *B, const field_params &p, bool in
     return r;
 }
 
-__global__ __launch_bounds__( FIELD_BLOCK )
-void field_kernel( field_params p )
+__device__ __forceinline__ void field_body( const field_params &p, double2 *lds2 )
 {
-    extern __shared__ double2 lds2[];
     double2 *A = lds2, *B = lds2 + p.N;
     double *red = reinterpret_cast<double*>( lds2 + 2*p.N );
     const uint32_t N = p.N;
@@ -211,6 +134,31 @@ void field_kernel( field_params p )
     }
 }
 
+__global__ __launch_bounds__( FIELD_BLOCK )
+void field_kernel( field_params p )
+{
+    extern __shared__ double2 lds2[];
+    field_body( p, lds2 );
+}
+
+// Batched sweep: workgroup r solves run r's field at step n (runs past their
+// Nt skip it); the per-step outputs are the run's slice n, its differences and
+// energies[n].
+__global__ __launch_bounds__( FIELD_BLOCK )
+void field_batch_kernel( const batch_field *__restrict__ runs, uint64_t n )
+{
+    extern __shared__ double2 lds2b[];
+    const batch_field &b = runs[ blockIdx.x ];
+    if ( n > b.Nt ) return;
+    field_params f = b.f;
+    f.slice = b.f.slice + n * b.st;
+    if ( f.dslice )   f.dslice   = b.f.dslice + n * b.sd;
+    if ( f.dslice32 ) f.dslice32 = b.f.dslice32 + n * b.sd;
+    if ( f.slice32 )  f.slice32  = b.f.slice32 + n * b.st;
+    f.energy = b.f.energy + n;
+    field_body( f, lds2b );
+}
+
 __global__ void reduce_partials_kernel( const double *partials, uint32_t wpx, double weight,
                                         uint32_t first, uint32_t count, uint32_t n_space, double *rho )
 {
@@ -296,6 +244,18 @@ hipError_t launch_field( const field_params &p, hipStream_t st )
     }
     hipLaunchKernelGGL( field_kernel, dim3( 1 ), dim3( FIELD_BLOCK ), lds, st, p );
     return hipGetLastError();
+}
+
+hipError_t launch_field_batch( const batch_field *runs, uint32_t n_runs, uint64_t n, size_t lds, hipStream_t st )
+{
+    if ( n_runs == 0 ) return hipSuccess;
+    hipLaunchKernelGGL( field_batch_kernel, dim3( n_runs ), dim3( FIELD_BLOCK ), lds, st, runs, n );
+    return hipGetLastError();
+}
+
+size_t field_batch_lds_bytes( uint32_t max_N )
+{
+    return size_t( max_N ) * 2 * sizeof(double2) + ( FIELD_BLOCK / WAVE ) * sizeof(double);
 }
 
 hipError_t launch_reduce_partials( const double *partials, uint32_t wpx, double weight,

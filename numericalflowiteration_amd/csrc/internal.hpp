@@ -77,6 +77,14 @@ struct trace3_params
     nufi_f0 f0;
 };
 
+// One run of a batched sweep (device-resident parameter block).
+struct batch_run
+{
+    trace1_params p;          // n is set per step
+    uint64_t Nt;
+    uint32_t wg_begin, n_wg;  // the run's workgroups in the batch trace launch
+};
+
 struct launch_shape
 {
     int block;   // threads per workgroup
@@ -88,9 +96,39 @@ struct launch_shape
 hipError_t launch_trace1d( const trace1_params &p, int variant, const launch_shape &s, hipStream_t st );
 hipError_t launch_trace3d( const trace3_params &p, int variant, const launch_shape &s, hipStream_t st );
 size_t trace1d_lds_bytes( const trace1_params &p, int variant, const launch_shape &s );
+hipError_t launch_trace1d_batch( const batch_run *runs, const uint32_t *wg_run, uint32_t n_wg, uint64_t n,
+                                 int variant, const launch_shape &s, size_t lds, hipStream_t st );
 size_t trace3d_lds_bytes( const trace3_params &p, int variant, const launch_shape &s );
 bool trace1d_shape_ok( int variant, int order, const launch_shape &s );
 bool trace3d_shape_ok( int variant, int order, const launch_shape &s );
+
+// ---- adjoint (adjoint1d.hip) ---------------------------------------------------
+struct adjoint1_params
+{
+    const double *dhist;      // extended differences (FP64), stride stride_d
+    const double *g;          // dJ/drho_n (Nx)
+    double       *lambda;     // folded slice adjoints, slice m at lambda + m*Nx
+    double       *grad_part;  // per wave-task parameter-gradient accumulators [g][NUFI_MAX_MODES]
+    uint64_t n;
+    uint32_t Nx, Nu, stride_d, wpx, n_tasks, chunk, npar;
+    double x_min, dx, dx_inv, dt, u_min, du;
+    nufi_f0 f0;
+};
+
+struct adjoint_field_params
+{
+    uint32_t N, log2x;
+    const double  *rho, *lambda;   // rho_n and lambda_n (Nx)
+    double        *g;              // out: dJ/drho_n
+    const double2 *tw, *mult;      // twiddles, step multiplier
+    const double  *ewt;
+    double dt;
+};
+
+hipError_t launch_adjoint1d( const adjoint1_params &p, hipStream_t st );
+size_t adjoint1d_lds_bytes( const adjoint1_params &p );
+hipError_t launch_adjoint_field( const adjoint_field_params &p, hipStream_t st );
+hipError_t launch_reduce_grad( const double *part, uint32_t n_tasks, uint32_t npar, double *grad, hipStream_t st );
 
 // ---- field solve (field.hip) -------------------------------------------------
 // Input rho either from per-wave partials (sum_j partials[i*wpx + j], times
@@ -125,6 +163,17 @@ struct field_params
 };
 
 hipError_t launch_field( const field_params &p, hipStream_t st );
+
+// One run of a batched sweep for the field kernel: f holds the bases of the
+// run's history (slice, dslice, dslice32, slice32 at slice 0; energy at step 0).
+struct batch_field
+{
+    field_params f;
+    uint64_t Nt;
+    uint32_t st, sd;
+};
+hipError_t launch_field_batch( const batch_field *runs, uint32_t n_runs, uint64_t n, size_t lds, hipStream_t st );
+size_t field_batch_lds_bytes( uint32_t max_N );
 size_t field_lds_bytes( const field_params &p );
 int field_supported( int dim, uint32_t Nx, uint32_t Ny, uint32_t Nz );
 

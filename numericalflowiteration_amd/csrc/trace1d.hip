@@ -32,6 +32,7 @@
 //            rho sums stay FP64 (order 4).
 #include "device_common.hpp"
 #include "internal.hpp"
+#include "step1d.hpp"
 
 namespace nufi_impl
 {
@@ -69,11 +70,11 @@ struct node_id
 };
 
 template <int WPB, int NPT>
-__device__ __forceinline__ node_id make_node( const trace1_params &p, int r )
+__device__ __forceinline__ node_id make_node( const trace1_params &p, int r, uint32_t wg )
 {
     node_id id;
     const uint32_t lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
-    id.g = ( blockIdx.x * NPT + r ) * WPB + wave;
+    id.g = ( wg * NPT + r ) * WPB + wave;
     id.task = id.g < p.n_tasks;
     const uint32_t ixl = id.task ? id.g / p.wpx : 0;
     id.j  = id.task ? id.g % p.wpx : 0;
@@ -119,10 +120,8 @@ __device__ __forceinline__ int64_t top_slice( uint64_t n )
 
 // ---- fast FP64 (order 4) ----------------------------------------------------
 template <int BLOCK, int NPT, int MODE>
-__global__ __launch_bounds__( BLOCK )
-void trace1d_fast_kernel( trace1_params p )
+__device__ __forceinline__ void trace1d_fast_body( const trace1_params &p, uint32_t wg, double *lds )
 {
-    extern __shared__ double lds[];
     constexpr int WPB = BLOCK / WAVE;
 
     node_id id[NPT];
@@ -130,7 +129,7 @@ void trace1d_fast_kernel( trace1_params p )
     #pragma unroll
     for ( int r = 0; r < NPT; ++r )
     {
-        id[r] = make_node<WPB,NPT>( p, r );
+        id[r] = make_node<WPB,NPT>( p, r, wg );
         // node (cuda_kernel.cu:43-44); on the field grid its cell position is exactly ix
         s[r] = ( MODE == MODE_RHO ) ? double( id[r].ix )
                                     : ( ( p.nx_min + id[r].ix*p.ndx ) - p.x_min ) * p.dx_inv;
@@ -159,17 +158,7 @@ void trace1d_fast_kernel( trace1_params p )
             #pragma unroll
             for ( int r = 0; r < NPT; ++r )
             {
-                double x = initial ? s[r] : fma( -cd, u[r], s[r] );
-                const double w = floor( x * invNx );
-                x = fma( -Nxd, w, x );
-                const double k = floor( x );
-                const double t = x - k;
-                const int i = int( k );                   // in [-1, Nx]: D is extended
-                const double D0 = D[i], D1 = D[i+1], D2 = D[i+2];
-                const double a = 1.0 - t;
-                const double r2 = fma( a*a, D0 - D1, fma( t*t, D2 - D1, D1 + D1 ) );
-                u[r] = fma( kf, r2, u[r] );
-                s[r] = x;
+                sv_step( s[r], u[r], D, cd, kf, Nxd, invNx, !initial );
             }
         }
     }
@@ -185,6 +174,14 @@ void trace1d_fast_kernel( trace1_params p )
             write_metrics( p.partials, id[r].g, id[r].task, id[r].active, p.weight, f, u0*u0 );
         }
     }
+}
+
+template <int BLOCK, int NPT, int MODE>
+__global__ __launch_bounds__( BLOCK )
+void trace1d_fast_kernel( trace1_params p )
+{
+    extern __shared__ double lds[];
+    trace1d_fast_body<BLOCK,NPT,MODE>( p, blockIdx.x, lds );
 }
 
 // ---- strict FP64 (reference operation order, any order 3..8) ---------------
@@ -227,7 +224,7 @@ void trace1d_strict_kernel( trace1_params p )
 #pragma clang fp contract(off)
     extern __shared__ double lds[];
     constexpr int WPB = BLOCK / WAVE;
-    const node_id id = make_node<WPB,1>( p, 0 );
+    const node_id id = make_node<WPB,1>( p, 0, blockIdx.x );
 
     double x = p.nx_min + id.ix * p.ndx;                    // cuda_kernel.cu:43, 69
     const double u0 = p.nu_min + id.iu * p.ndu + p.ndu / 2; // cuda_kernel.cu:44, 70
@@ -258,10 +255,8 @@ void trace1d_strict_kernel( trace1_params p )
 
 // ---- FP32 compensated (order 4) --------------------------------------------
 template <int BLOCK, int NPT>
-__global__ __launch_bounds__( BLOCK )
-void trace1d_fp32c_kernel( trace1_params p )
+__device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint32_t wg, float *ldsf )
 {
-    extern __shared__ float ldsf[];
     constexpr int WPB = BLOCK / WAVE;
 
     node_id id[NPT];
@@ -269,7 +264,7 @@ void trace1d_fp32c_kernel( trace1_params p )
     #pragma unroll
     for ( int r = 0; r < NPT; ++r )
     {
-        id[r] = make_node<WPB,NPT>( p, r );
+        id[r] = make_node<WPB,NPT>( p, r, wg );
         const double u0 = p.nu_min + id[r].iu*p.ndu + p.ndu/2;
         sh[r] = float( id[r].ix ); sl[r] = 0.0f;
         uh[r] = float( u0 );       ul[r] = float( u0 - double( uh[r] ) );
@@ -334,6 +329,32 @@ void trace1d_fp32c_kernel( trace1_params p )
     }
 }
 
+template <int BLOCK, int NPT>
+__global__ __launch_bounds__( BLOCK )
+void trace1d_fp32c_kernel( trace1_params p )
+{
+    extern __shared__ float ldsf[];
+    trace1d_fp32c_body<BLOCK,NPT>( p, blockIdx.x, ldsf );
+}
+
+// ---- batched sweep: one launch over the wave-tasks of many independent runs --
+// Workgroup b belongs to run wg_run[b]; its parameter block lives in device
+// memory (runs[r].p, with n filled in here) and its workgroup index within the
+// run is b - runs[r].wg_begin.  Runs past their own Nt skip the step.
+template <int BLOCK, int NPT, int VARIANT>
+__global__ __launch_bounds__( BLOCK )
+void trace1d_batch_kernel( const batch_run *__restrict__ runs, const uint32_t *__restrict__ wg_run, uint64_t n )
+{
+    extern __shared__ double lds_b[];
+    const uint32_t r = wg_run[ blockIdx.x ];
+    if ( n > runs[r].Nt ) return;                     // uniform over the workgroup
+    trace1_params p = runs[r].p;
+    p.n = n;
+    const uint32_t wg = blockIdx.x - runs[r].wg_begin;
+    if ( VARIANT == TRACE_FP32C ) trace1d_fp32c_body<BLOCK,NPT>( p, wg, reinterpret_cast<float*>( lds_b ) );
+    else                          trace1d_fast_body<BLOCK,NPT,MODE_RHO>( p, wg, lds_b );
+}
+
 template <typename K>
 hipError_t launch_kernel( K kernel, uint32_t grid, int block, size_t lds, const trace1_params &p,
                           hipStream_t st )
@@ -389,7 +410,49 @@ hipError_t launch_strict( const trace1_params &p, int block, uint32_t grid, size
     return launch_strict_mode<ORDER,MODE_RHO>( p, block, grid, lds, st );
 }
 
+template <int BLOCK, int NPT>
+hipError_t launch_batch_npt( const batch_run *runs, const uint32_t *wg_run, uint32_t n_wg, uint64_t n,
+                             int variant, size_t lds, hipStream_t st )
+{
+    if ( variant == TRACE_FP32C )
+        hipLaunchKernelGGL( ( trace1d_batch_kernel<BLOCK,NPT,TRACE_FP32C> ), dim3( n_wg ), dim3( BLOCK ), lds, st,
+                            runs, wg_run, n );
+    else
+        hipLaunchKernelGGL( ( trace1d_batch_kernel<BLOCK,NPT,TRACE_FAST> ), dim3( n_wg ), dim3( BLOCK ), lds, st,
+                            runs, wg_run, n );
+    return hipGetLastError();
+}
+
+template <int BLOCK>
+hipError_t launch_batch_block( const batch_run *runs, const uint32_t *wg_run, uint32_t n_wg, uint64_t n,
+                               int variant, int npt, size_t lds, hipStream_t st )
+{
+    switch ( npt )
+    {
+    case 1: return launch_batch_npt<BLOCK,1>( runs, wg_run, n_wg, n, variant, lds, st );
+    case 2: return launch_batch_npt<BLOCK,2>( runs, wg_run, n_wg, n, variant, lds, st );
+    case 4: return launch_batch_npt<BLOCK,4>( runs, wg_run, n_wg, n, variant, lds, st );
+    default: return hipErrorInvalidValue;
+    }
+}
+
 } // namespace
+
+hipError_t launch_trace1d_batch( const batch_run *runs, const uint32_t *wg_run, uint32_t n_wg, uint64_t n,
+                                 int variant, const launch_shape &s, size_t lds, hipStream_t st )
+{
+    if ( variant != TRACE_FAST && variant != TRACE_FP32C ) return hipErrorInvalidValue;
+    if ( n_wg == 0 ) return hipSuccess;
+    switch ( s.block )
+    {
+    case 64:   return launch_batch_block<64>  ( runs, wg_run, n_wg, n, variant, s.npt, lds, st );
+    case 128:  return launch_batch_block<128> ( runs, wg_run, n_wg, n, variant, s.npt, lds, st );
+    case 256:  return launch_batch_block<256> ( runs, wg_run, n_wg, n, variant, s.npt, lds, st );
+    case 512:  return launch_batch_block<512> ( runs, wg_run, n_wg, n, variant, s.npt, lds, st );
+    case 1024: return launch_batch_block<1024>( runs, wg_run, n_wg, n, variant, s.npt, lds, st );
+    default:   return hipErrorInvalidValue;
+    }
+}
 
 bool trace1d_shape_ok( int variant, int order, const launch_shape &s )
 {

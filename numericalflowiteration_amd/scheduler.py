@@ -147,6 +147,17 @@ class GpuScheduler:
         self._check(self._lib.nufi_interpolate(self._ctx, _ptr(out), _ptr(v)))
         return out
 
+    # ---- adjoint ---------------------------------------------------------
+    def adjoint_grad(self):
+        """(J, dJ/da): J = sum_n dt*energy_n over steps 0..Nt and its gradient with
+        respect to the f0 parameters (multimode a_m, else alpha).  Runs the forward
+        loop and the reverse sweep on the device."""
+        g = np.zeros(8)
+        J = np.zeros(1)
+        self._check(self._lib.nufi_adjoint_grad(self._ctx, _ptr(g), 8, _ptr(J)))
+        npar = self.conf.f0.n_modes if self.conf.f0.kind == 3 else 1
+        return float(J[0]), g[:npar].copy()
+
     # ---- kernel selection ------------------------------------------------
     def set_trace_variant(self, variant: int):
         self._check(self._lib.nufi_set_trace_variant(self._ctx, variant))
@@ -181,4 +192,89 @@ class GpuScheduler:
         return int(n.value), float(ms[0]), float(cs[0])
 
 
-__all__ = ["GpuScheduler", "FP64", "FP32C"]
+
+
+class BatchScheduler:
+    """Many independent 1D runs stepped together on one device (nufi_batch_*).
+
+    Each time step is one trace launch over every run's nodes and one field
+    launch (a workgroup per run); `run(graph=True)` replays a captured hipGraph
+    of the whole step range.  Each run's history equals a single-run
+    `GpuScheduler` of the same config bit for bit.
+    """
+
+    def __init__(self, confs, order: int = 4, precision: int = FP64, device: int = 0):
+        self._lib = _capi.load()
+        self.confs = [c.copy() for c in confs]
+        arr = (Config * len(self.confs))(*self.confs)
+        self._arr = arr
+        ctx = ctypes.c_void_p()
+        rc = self._lib.nufi_batch_create(arr, len(self.confs), order, precision, device, ctypes.byref(ctx))
+        _capi.check(rc, None, batch=True)
+        self._b = ctx
+        self.order = order
+        self.precision = precision
+
+    def close(self):
+        if getattr(self, "_b", None):
+            self._lib.nufi_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        _capi.check(rc, self._b, batch=True)
+
+    @property
+    def n_runs(self) -> int:
+        return len(self.confs)
+
+    def run(self, n_begin: int = 0, n_end: int | None = None, graph: bool = True):
+        n_end = max(c.Nt for c in self.confs) + 1 if n_end is None else n_end
+        self._check(self._lib.nufi_batch_run(self._b, n_begin, n_end, int(graph)))
+
+    def synchronize(self):
+        self._check(self._lib.nufi_batch_synchronize(self._b))
+
+    def energies(self, run: int) -> np.ndarray:
+        c = self.confs[run]
+        out = np.zeros(c.Nt + 1)
+        self._check(self._lib.nufi_batch_energies(self._b, run, 0, c.Nt + 1, _ptr(out)))
+        return out
+
+    def history(self, run: int) -> np.ndarray:
+        c = self.confs[run]
+        out = np.zeros((c.Nt + 1) * c.stride_t(self.order))
+        self._check(self._lib.nufi_batch_history(self._b, run, 0, c.Nt + 1, _ptr(out)))
+        return out
+
+    def set_launch_shape(self, block: int, nodes_per_thread: int, chunk: int):
+        self._check(self._lib.nufi_batch_set_launch_shape(self._b, block, nodes_per_thread, chunk))
+
+    def launch_shape(self):
+        b, n, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self._lib.nufi_batch_get_launch_shape(self._b, ctypes.byref(b), ctypes.byref(n), ctypes.byref(c)))
+        return b.value, n.value, c.value
+
+    def autotune(self, n_probe: int, repeats: int = 2):
+        b, n, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms = np.zeros(1)
+        self._check(self._lib.nufi_batch_autotune(self._b, n_probe, repeats, ctypes.byref(b), ctypes.byref(n),
+                                                  ctypes.byref(c), _ptr(ms)))
+        return (b.value, n.value, c.value), float(ms[0])
+
+    def char_steps(self, n_begin: int, n_end: int) -> int:
+        return sum(c.char_steps(n_begin, min(n_end, c.Nt + 1)) for c in self.confs if n_begin <= c.Nt)
+
+
+__all__ = ["GpuScheduler", "BatchScheduler", "FP64", "FP32C"]

@@ -58,13 +58,17 @@ struct grid1
     real dt, x_min, x_max, u_min, u_max;
     real dx, dx_inv, Lx, Lx_inv, du;
     nufi_f0 f0;
+    real alpha, k, a[NUFI_MAX_MODES];   // f0 parameters in `real` (seeded for tangents)
 
     explicit grid1( const nufi_config &c ):
         Nx { c.Nx }, Nu { c.Nu }, Nt { c.Nt },
         dt { c.dt }, x_min { c.x_min }, x_max { c.x_max },
         u_min { c.u_min }, u_max { c.u_max },
         dx { c.dx }, dx_inv { c.dx_inv }, Lx { c.Lx }, Lx_inv { c.Lx_inv },
-        du { c.du }, f0 ( c.f0 ) {}
+        du { c.du }, f0 ( c.f0 ), alpha { c.f0.alpha }, k { c.f0.k }
+    {
+        for ( int m = 0; m < NUFI_MAX_MODES; ++m ) a[m] = real( c.f0.a[m] );
+    }
 };
 
 template <typename real>
@@ -100,10 +104,11 @@ constexpr double inv_2pi_3_2    = 0.06349363593424096978576330493464;
 constexpr double twostream3_c   = 0.03174681796712048489288165246732;
 
 template <typename real>
-real f0_1d( const nufi_f0 &p, real x, real u )
+real f0_1d( const grid1<real> &g, real x, real u )
 {
     using std::cos; using std::exp;
-    const real alpha = real_of<real>(p.alpha), k = real_of<real>(p.k);
+    const nufi_f0 &p = g.f0;
+    const real alpha = g.alpha, k = g.k;
     switch ( p.kind )
     {
     case NUFI_F0_LANDAU_1D:      /* config.hpp:83 (the commented variant) */
@@ -114,7 +119,7 @@ real f0_1d( const nufi_f0 &p, real x, real u )
     {
         real s = real(1);
         for ( int m = 1; m <= p.n_modes; ++m )
-            s = s + real_of<real>(p.a[m-1]) * cos( real_of<real>( p.k*m ) * x );
+            s = s + g.a[m-1] * cos( real_of<real>( p.k*m ) * x );
         return real_of<real>(inv_sqrt_2pi) * s * exp( -u*u/real(2) );
     }
     default:
@@ -291,7 +296,7 @@ real field3d( real x, real y, real z, const real *slice, const grid3<real> &g )
 template <typename real, int order>
 real ftilda1d( size_t n, real x, real u, const real *hist, const grid1<real> &g )
 {
-    if ( n == 0 ) return f0_1d<real>( g.f0, x, u );
+    if ( n == 0 ) return f0_1d<real>( g, x, u );
     const size_t stride_t = g.Nx + order - 1;
     // No initial half-kick (rho.hpp:44): E at t_n is not known yet.
     while ( --n )
@@ -303,14 +308,14 @@ real ftilda1d( size_t n, real x, real u, const real *hist, const grid1<real> &g 
     x = x - g.dt*u;
     const real E = -field1d<real,order,1>( x, hist, g );
     u = u + real_of<real>(0.5)*g.dt*E;
-    return f0_1d<real>( g.f0, x, u );
+    return f0_1d<real>( g, x, u );
 }
 
 /* With the initial half-kick from slice n (rho.hpp:63-96). */
 template <typename real, int order>
 real f1d( size_t n, real x, real u, const real *hist, const grid1<real> &g )
 {
-    if ( n == 0 ) return f0_1d<real>( g.f0, x, u );
+    if ( n == 0 ) return f0_1d<real>( g, x, u );
     const size_t stride_t = g.Nx + order - 1;
     real E = -field1d<real,order,1>( x, hist + n*stride_t, g );
     u = u + real_of<real>(0.5)*g.dt*E;
@@ -323,7 +328,7 @@ real f1d( size_t n, real x, real u, const real *hist, const grid1<real> &g )
     x = x - g.dt*u;
     E = -field1d<real,order,1>( x, hist, g );
     u = u + real_of<real>(0.5)*g.dt*E;
-    return f0_1d<real>( g.f0, x, u );
+    return f0_1d<real>( g, x, u );
 }
 
 template <typename real, int order>

@@ -10,6 +10,7 @@
  * slice read (the t = 0 edge read of fields.hpp:49-58 may touch it).
  */
 #include "nufi_oracle.hpp"
+#include "dual.hpp"
 
 #include <omp.h>
 #include <cstdio>
@@ -274,6 +275,87 @@ struct metrics3d_op
     }
 };
 
+/* ---- forward-mode tangent of the 1D time loop --------------------------
+ * J = sum_{n} dt * energy_n and dJ/d(parameter) for up to NUFI_MAX_MODES
+ * directions (multimode a_m; alpha for Landau / two-stream), by dual numbers
+ * through the trace and componentwise (linear) Poisson solve and fit. */
+using D8 = dual<NUFI_MAX_MODES>;
+
+/* scaled DHT coefficients d_i = H_i / (N kappa_i^2) of one real array (poisson.cpp:68-82) */
+std::vector<double> scaled_spectrum( const grid1<double> &g, std::vector<double> v )
+{
+    std::vector<long double> tmp;
+    cas_table T( g.Nx );
+    dht_line( T, v.data(), 1, tmp );
+    const double fac_N = double(1) / double( g.Nx ), fac_x = ( 2*M_PI*g.Lx_inv );
+    for ( size_t i = 1; i < g.Nx; ++i )
+    {
+        const double ii = ( 2*i < g.Nx ) ? double(i) : double(g.Nx - i);
+        v[i] *= fac_N / ( ii*ii*fac_x*fac_x );
+    }
+    v[0] = 0;
+    return v;
+}
+
+int tangent_run1d( const nufi_config *c, uint64_t n_end, double *J, double *dJ, int nthreads )
+{
+    constexpr int K = NUFI_MAX_MODES;
+    constexpr int order = 4;
+    grid1<double> g( *c );
+    grid1<D8> gd( *c );
+    int npar = 0;
+    if ( c->f0.kind == NUFI_F0_MULTIMODE_1D )
+    { npar = c->f0.n_modes; for ( int m = 0; m < npar; ++m ) gd.a[m].d[m] = 1.0; }
+    else if ( c->f0.kind == NUFI_F0_LANDAU_1D || c->f0.kind == NUFI_F0_TWOSTREAM_1D )
+    { npar = 1; gd.alpha.d[0] = 1.0; }
+    else return NUFI_ERR_UNSUPPORTED;
+
+    const size_t st = g.Nx + order - 1, N = g.Nx;
+    std::vector<D8> hist( ( n_end + 1 ) * st + order + 1 );
+    std::vector<D8> rho( N );
+    double Jv = 0, dJv[K] = {};
+    for ( uint64_t n = 0; n < n_end; ++n )
+    {
+        const long long NN = (long long)N;
+        #pragma omp parallel for num_threads(nthreads) schedule(static)
+        for ( long long i = 0; i < NN; ++i ) rho[i] = rho1d<D8,order>( n, (size_t)i, hist.data(), gd );
+        // value: exactly the oracle's step (poisson1d + fit1d)
+        std::vector<double> v( N );
+        for ( size_t i = 0; i < N; ++i ) v[i] = rho[i].v;
+        std::vector<double> phi = v;
+        const double e = poisson1d( g, phi.data() );
+        std::vector<double> sl( st );
+        fit1d<order>( g, sl.data(), phi.data() );
+        for ( size_t i = 0; i < st; ++i ) hist[n*st + i] = D8( sl[i] );
+        // tangents: the same linear maps per component, energy by the product rule
+        const std::vector<double> d0 = scaled_spectrum( g, v );
+        const double fac_x = ( 2*M_PI*g.Lx_inv );
+        for ( int k = 0; k < npar; ++k )
+        {
+            std::vector<double> vk( N );
+            for ( size_t i = 0; i < N; ++i ) vk[i] = rho[i].d[k];
+            const std::vector<double> dk = scaled_spectrum( g, vk );
+            double de = 0;
+            for ( size_t i = 1; i < N; ++i )
+            {
+                const double ii = ( 2*i < N ) ? double(i) : double(N - i);
+                de += 2.0 * ( ii*fac_x*d0[i] ) * ( ii*fac_x*dk[i] );
+            }
+            de *= g.Lx/2;
+            dJv[k] += g.dt * de;
+            std::vector<double> phik = vk;
+            poisson1d( g, phik.data() );
+            std::vector<double> slk( st );
+            fit1d<order>( g, slk.data(), phik.data() );
+            for ( size_t i = 0; i < st; ++i ) hist[n*st + i].d[k] = slk[i];
+        }
+        Jv += g.dt * e;
+    }
+    *J = Jv;
+    for ( int k = 0; k < npar; ++k ) dJ[k] = dJv[k];
+    return 0;
+}
+
 bool bad( const nufi_config *c, int dim )
 {
     return c == nullptr || c->dim != dim || c->Nx == 0 || c->Nu == 0 ||
@@ -394,5 +476,12 @@ int oracle_metrics3d( const nufi_config *c, int order, uint64_t n, const double 
 }
 
 int oracle_max_threads( void ) { return omp_get_max_threads(); }
+
+/* J = sum_{n<n_end} dt*energy_n and its forward-mode tangent dJ/d(parameters). */
+int oracle_tangent_run1d( const nufi_config *c, uint64_t n_end, double *J, double *dJ, int nthreads )
+{
+    if ( bad( c, 1 ) || J == nullptr || dJ == nullptr ) return NUFI_ERR_INVALID;
+    return tangent_run1d( c, n_end, J, dJ, nthreads );
+}
 
 } // extern "C"

@@ -58,6 +58,7 @@ def lib():
             "oracle_run3d": [_cp, _int, _u64, _u64, _dp, _dp, _int],
             "oracle_metrics1d": [_cp, _cp, _int, _u64, _dp, _dp, _int],
             "oracle_metrics3d": [_cp, _int, _u64, _dp, _dp, _int],
+            "oracle_tangent_run1d": [_cp, _u64, _dp, _dp, _int],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -186,3 +187,13 @@ def metrics(conf: Config, n: int, hist, conf_metrics: Config | None = None, orde
         _check(lib().oracle_metrics3d(ctypes.byref(conf), order, n, _p(hist), _p(out),
                                       nthreads or nthreads_default()), "metrics3d")
     return out
+
+
+def tangent_run(conf: Config, n_end: int | None = None, nthreads: int | None = None):
+    """(J, dJ/dparams) over steps 0..n_end-1 by forward-mode dual numbers (8 directions)."""
+    n_end = conf.Nt + 1 if n_end is None else n_end
+    J = np.zeros(1)
+    dJ = np.zeros(8)
+    _check(lib().oracle_tangent_run1d(ctypes.byref(conf), n_end, _p(J), _p(dJ), nthreads or nthreads_default()),
+           "tangent_run1d")
+    return float(J[0]), dJ

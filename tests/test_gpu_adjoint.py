@@ -1,0 +1,85 @@
+"""Device adjoint (nufi_adjoint_grad) of J = sum_n dt * energy_n with respect to
+the f0 parameters: J against the oracle, the gradient against central finite
+differences (h = 1e-6) of the oracle (the reference arithmetic) and of the
+device forward model, and through the torch.autograd wrapper.
+Tolerance: <= 1e-5 relative (the bar of the adjoint case) against the oracle."""
+import numpy as np
+import pytest
+
+import numericalflowiteration_amd as nufi
+from numericalflowiteration_amd import adjoint as A
+from numericalflowiteration_amd import config as C
+from tests import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def J_oracle(conf):
+    h = O.new_history(conf, conf.Nt + 1)
+    e = O.run(conf, 0, conf.Nt + 1, h)
+    return conf.dt * float(np.sum(e))
+
+
+def J_device(conf):
+    with nufi.GpuScheduler(conf) as s:
+        return conf.dt * float(np.sum(s.run()))
+
+
+def small_multimode():
+    return nufi.multimode_1d(nufi.adjoint_modes(), Nx=32, Nu=128, T=2.0)      # 33 steps, 8 parameters
+
+
+def test_adjoint_multimode_against_fd():
+    conf = small_multimode()
+    J, g = A.energy_and_grad(conf, nufi.adjoint_modes())
+    Jo = J_oracle(conf)
+    print(f"[adjoint] J device {J:.15e} oracle {Jo:.15e}", flush=True)
+    assert abs(J - Jo) < 1e-11 * abs(Jo)
+    h = 1e-6
+    a0 = nufi.adjoint_modes()
+    for m in range(8):
+        ap, am = list(a0), list(a0)
+        ap[m] += h
+        am[m] -= h
+        fd_o = (J_oracle(A._with_params(conf, ap)) - J_oracle(A._with_params(conf, am))) / (2 * h)
+        fd_d = (J_device(A._with_params(conf, ap)) - J_device(A._with_params(conf, am))) / (2 * h)
+        print(f"  a_{m+1}: adjoint {g[m]:.10e} fd(oracle) {fd_o:.10e} fd(device) {fd_d:.10e}", flush=True)
+        scale = max(abs(fd_o), 1e-3 * np.max(np.abs(g)))
+        assert abs(g[m] - fd_o) <= 1e-5 * scale
+        assert abs(g[m] - fd_d) <= 1e-6 * scale
+
+
+def test_adjoint_against_forward_tangent():
+    conf = small_multimode()
+    J, g = A.energy_and_grad(conf, nufi.adjoint_modes())
+    Jt, dJ = O.tangent_run(conf)
+    err = np.max(np.abs(g - dJ[:8]) / np.max(np.abs(dJ[:8])))
+    print(f"[adjoint vs tangent] max rel err {err:.3e}", flush=True)
+    assert abs(J - Jt) < 1e-11 * abs(Jt)
+    assert err < 1e-8
+
+
+@pytest.mark.parametrize("kind", [C.F0_LANDAU_1D, C.F0_TWOSTREAM_1D])
+def test_adjoint_alpha(kind):
+    conf = C.config_1d(32, 128, 1 / 16, 1.5, 0.0, 4 * np.pi, -8.0, 8.0, kind, alpha=0.05, k=0.5)
+    J, g = A.energy_and_grad(conf, [0.05])
+    h = 1e-6
+    fd = (J_oracle(A._with_params(conf, [0.05 + h])) - J_oracle(A._with_params(conf, [0.05 - h]))) / (2 * h)
+    print(f"[adjoint alpha] kind={kind} adjoint {g[0]:.10e} fd {fd:.10e}", flush=True)
+    assert abs(g[0] - fd) <= 1e-5 * abs(fd)
+
+
+def test_torch_autograd_wrapper():
+    import torch
+    conf = small_multimode()
+    a = torch.tensor(nufi.adjoint_modes(), dtype=torch.float64, requires_grad=True)
+    J = A.time_integrated_energy(a, conf)
+    (2.0 * J).backward()
+    _, g = A.energy_and_grad(conf, nufi.adjoint_modes())
+    assert np.allclose(a.grad.numpy(), 2.0 * g, rtol=1e-12, atol=0)
+
+
+def test_adjoint_rejects_unsupported():
+    with nufi.GpuScheduler(nufi.landau_3d(N=8, V=8, T=1.0)) as s:
+        with pytest.raises(nufi.NufiError):
+            s.adjoint_grad()
