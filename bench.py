@@ -9,8 +9,9 @@ dt=1/32, T=50 (1600 steps).  It runs in both forms: the plain FP64 method and
 the FP32-compensated trace; the reported `value` is the FP64 form's
 characteristic steps per second end to end (time-to-T), the FP32C form and
 its electric-energy error against the FP64 run are reported beside it.
-With --with-3d, the 3D3V Landau case (16^3 x 32^3, dt=1/8, T=20) is run once
-per form as well.
+The 3D3V Landau case (16^3 x 32^3, dt=1/8, T=20) runs once per form as well
+(--no-3d skips it), and so do the 512-run batched sweep and the adjoint case
+(--no-extra skips them); their figures are reported in `forms`.
 
 Prints ONE JSON line (rank 0, N = 1).
 """
@@ -42,9 +43,11 @@ def sync_all(sched):
         pass
 
 
-def timed_runs(conf, precision, steps, warmup, flops_per_cs, tune_n=None):
+def timed_runs(conf, precision, steps, warmup, flops_per_cs, tune_n=None, shape=None):
     import numericalflowiteration_amd as nufi
     with nufi.GpuScheduler(conf, precision=precision) as s:
+        if shape is not None:
+            s.set_launch_shape(*shape)
         for _ in range(warmup):
             s.run()
         tuned = None
@@ -73,6 +76,24 @@ def timed_runs(conf, precision, steps, warmup, flops_per_cs, tune_n=None):
     ), energies
 
 
+def landau3d_forms(args):
+    """The 3D3V Landau case, one run to T per form (FP64, then FP32C)."""
+    import numpy as np
+    import numericalflowiteration_amd as nufi
+    c5 = nufi.landau_3d()
+    t3 = None if args.no_autotune else 16
+    r64, f64 = timed_runs(c5, nufi.FP64, 1, 0, FLOPS_3D, t3)
+    r32, f32 = timed_runs(c5, nufi.FP32C, 1, 0, FLOPS_3D, t3)
+    r32.update(energy_errors(f32, f64))
+    r32["speedup_vs_fp64"] = r32["cs_per_s_e2e"] / r64["cs_per_s_e2e"]
+    if args.dump_energies:
+        np.save(os.path.join(args.dump_energies, "landau3d_energy_fp64.npy"), f64)
+        np.save(os.path.join(args.dump_energies, "landau3d_energy_fp32c.npy"), f32)
+    return {"fp64": r64, "fp32c": r32,
+            "workload": "3D3V Landau, x in [0,4pi)^3, (1+0.01 sum cos(0.5x_d)) M3(v), "
+                        "v in [-6,6]^3, 16^3 x 32^3 nodes, dt=1/8, T=20 (161 steps)"}
+
+
 def energy_errors(e, ref):
     """Electric-energy error of a run against the FP64 run: the pointwise maximum
     of |e-ref|/ref, and the time-normwise ||e-ref||_2/||ref||_2.  The pointwise
@@ -83,6 +104,65 @@ def energy_errors(e, ref):
     return {"energy_max_rel_err_vs_fp64": float(np.max(d / np.abs(ref))),
             "energy_normwise_rel_err_vs_fp64": float(np.linalg.norm(e - ref) / np.linalg.norm(ref)),
             "energy_err_at_max_pointwise": int(np.argmax(d / np.abs(ref)))}
+
+
+def batch_sweep():
+    """The 512-run sweep (alpha~U[0.001,0.5], k~U[0.3,0.6], L=2pi/k, Nx in {32..256},
+    Nv=4Nx, v in [-10,10], Maxwellian or two-stream, dt=1/16, T=20) as one batch:
+    shared launches per step, hipGraph replay, autotuned shape; FP32C and FP64."""
+    import numpy as np
+    import numericalflowiteration_amd as nufi
+    confs = [nufi.sweep_config(r) for r in nufi.sweep_runs()]
+    res = {"workload": "512 independent 1D runs (seeded sweep), dt=1/16, T=20, one batch",
+           "runs": len(confs), "nx_histogram": {str(n): sum(c.Nx == n for c in confs) for n in (32, 64, 128, 256)}}
+    energies = {}
+    for name, prec in (("fp32c", nufi.FP32C), ("fp64", nufi.FP64)):
+        with nufi.BatchScheduler(confs, precision=prec) as b:
+            b.run(graph=True)                          # warm-up, captures the graph
+            b.synchronize()
+            shape, _ = b.autotune(n_probe=confs[0].Nt, repeats=1)
+            b.run(graph=True)                          # recapture for the tuned shape
+            b.synchronize()
+            t0 = time.perf_counter()
+            b.run(graph=True)
+            b.synchronize()
+            wall = time.perf_counter() - t0
+            cs = b.char_steps(0, max(c.Nt for c in confs) + 1)
+            energies[name] = [b.energies(r) for r in range(len(confs))]
+        res[name] = {"wall_s": wall, "char_steps": cs, "cs_per_s_e2e": cs / wall, "launch_shape": list(shape)}
+    errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(energies["fp32c"], energies["fp64"])]
+    res["fp32c"]["energy_normwise_rel_err_vs_fp64_max_over_runs"] = max(errs)
+    res["fp32c"]["speedup_vs_fp64"] = res["fp32c"]["cs_per_s_e2e"] / res["fp64"]["cs_per_s_e2e"]
+    return res
+
+
+def adjoint_case():
+    """Gradient of J = sum_n dt*energy_n w.r.t. the 8 multimode amplitudes (Nx=128,
+    Nv=512, x in [0,4pi), v in [-10,10], dt=1/16, T=20) by the device adjoint,
+    checked against central differences (h=1e-6) of the device forward model."""
+    import numpy as np
+    import numericalflowiteration_amd as nufi
+    from numericalflowiteration_amd import adjoint as A
+    a0 = nufi.adjoint_modes()
+    conf = nufi.multimode_1d(a0)
+    A.energy_and_grad(conf, a0)                           # warm-up
+    t0 = time.perf_counter()
+    J, g = A.energy_and_grad(conf, a0)
+    wall = time.perf_counter() - t0
+    h, fd = 1e-6, []
+    for m in range(len(a0)):
+        ap, am = list(a0), list(a0)
+        ap[m] += h
+        am[m] -= h
+        Jp = A.energy_and_grad(conf, ap)[0]
+        Jm = A.energy_and_grad(conf, am)[0]
+        fd.append((Jp - Jm) / (2 * h))
+    fd = np.array(fd)
+    return {"workload": "adjoint: 1D multimode (8 amplitudes), Nx=128, Nv=512, dt=1/16, T=20 (321 steps)",
+            "J": J, "grad": g.tolist(), "fd_device_h1e-6": fd.tolist(),
+            "max_rel_err_vs_fd": float(np.max(np.abs(g - fd)) / np.max(np.abs(fd))),
+            "forward_plus_adjoint_s": wall,
+            "forward_char_steps": conf.char_steps(0, conf.Nt + 1)}
 
 
 def read_traffic(name):
@@ -192,8 +272,13 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--with-3d", action="store_true", help="also run the 3D3V Landau case once per form")
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3D3V Landau case")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched sweep and the adjoint case")
+    ap.add_argument("--shape", default="", help="B,N,C: fixed launch shape of the primary FP64 run (no autotune)")
     ap.add_argument("--only", choices=["fp64", "fp32c", "both"], default="both")
+    ap.add_argument("--case", choices=["all", "twostream", "landau3d"], default="all",
+                    help="twostream: only the 1D two-stream runs; landau3d: only the 3D3V Landau runs "
+                         "(for profiling one kernel at a time; the JSON line then carries that case)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--dump-energies", default="", help="directory for the per-step energy arrays (.npy)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -231,9 +316,14 @@ def main():
     out = {}
     e64 = e32 = None
     hist = None
-    tune = None if args.no_autotune else conf.Nt
+    tune = None if ( args.no_autotune or args.shape ) else conf.Nt
+    shape = tuple( int( v ) for v in args.shape.split( "," ) ) if args.shape else None
+    if args.case == "landau3d":
+        print(json.dumps({"case": "landau3d", "forms": landau3d_forms(args)}))
+        return
+
     if args.only in ("fp64", "both"):
-        out["fp64"], e64 = timed_runs(conf, nufi.FP64, args.steps, args.warmup, FLOPS_1D, tune)
+        out["fp64"], e64 = timed_runs(conf, nufi.FP64, args.steps, args.warmup, FLOPS_1D, tune, shape)
         with nufi.GpuScheduler(conf) as s:
             s.run()
             hist = s.history()
@@ -243,23 +333,16 @@ def main():
             out["fp32c"].update(energy_errors(e32, e64))
             out["fp32c"]["speedup_vs_fp64"] = out["fp32c"]["cs_per_s_e2e"] / out["fp64"]["cs_per_s_e2e"]
 
-    if args.with_3d:
-        c5 = nufi.landau_3d()
-        t3 = None if args.no_autotune else 16
-        r64, f64 = timed_runs(c5, nufi.FP64, 1, 0, FLOPS_3D, t3)
-        r32, f32 = timed_runs(c5, nufi.FP32C, 1, 0, FLOPS_3D, t3)
-        r32.update(energy_errors(f32, f64))
-        r32["speedup_vs_fp64"] = r32["cs_per_s_e2e"] / r64["cs_per_s_e2e"]
-        out["landau_3d"] = {"fp64": r64, "fp32c": r32,
-                            "workload": "3D3V Landau, x in [0,4pi)^3, (1+0.01 sum cos(0.5x_d)) M3(v), "
-                                        "v in [-6,6]^3, 16^3 x 32^3 nodes, dt=1/8, T=20 (161 steps)"}
-        if args.dump_energies:
-            np.save(os.path.join(args.dump_energies, "landau3d_energy_fp64.npy"), f64)
-            np.save(os.path.join(args.dump_energies, "landau3d_energy_fp32c.npy"), f32)
+    if not args.no_3d and args.only == "both" and args.case == "all":
+        out["landau_3d"] = landau3d_forms(args)
     if args.dump_energies and e64 is not None:
         np.save(os.path.join(args.dump_energies, "twostream_energy_fp64.npy"), e64)
         if e32 is not None:
             np.save(os.path.join(args.dump_energies, "twostream_energy_fp32c.npy"), e32)
+
+    if not args.no_extra and args.only == "both" and args.case == "all":
+        out["batch_sweep"] = batch_sweep()
+        out["adjoint"] = adjoint_case()
 
     prim = out.get("fp64") or out.get("fp32c")
     prim_prec = "fp64" if "fp64" in out else "fp32c"
@@ -269,7 +352,7 @@ def main():
         "frac": prim["achieved_tflops"] / peak,
         "traffic": read_traffic(f"trace1d_twostream_{prim_prec}_pmc.json"),
         "traffic_note": "HBM bytes per trace launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary",
-        "kernel": "trace1d_fast_kernel" if prim_prec == "fp64" else "trace1d_fp32c_kernel",
+        "kernel": "trace1d_poly_kernel" if prim_prec == "fp64" else "trace1d_fp32c_kernel",
         "flops_per_char_step": FLOPS_1D,
     }
     cpu = None

@@ -10,10 +10,11 @@
 //   g_n = 2 dt Re F^H( w . F rho_n ) + Re F^H( conj(m) . F lambda_n )  (= dJ/d rho_n)
 //   per node: fbar = -du g_n[i]; trace forward to the foot; dJ/da += fbar df0/da;
 //   then walk back through slices 0 .. n-1, undoing each Stormer-Verlet step
-//   exactly as it was taken (the states are recomputed, never stored):
-//     forward  s_a = wrap(s_b - cd u_b),  u_a = u_b + kf r2(s_a; S_m)
-//     reverse  rbar = kf ubar;  sbar += rbar dr2/dt;  lambda_m += rbar dr2/dS_m;
-//              u_b = u_a - kf r2;  ubar += -cd sbar;  s_b = wrap(s_a + cd u_b)
+//   as it was taken (the states are recomputed, never stored), with the
+//   kick polynomial p(t) = Q0 + Q1 t + Q2 t^2 of the forward kernel:
+//     forward  s_a = s_b - cd u_b,  u_a = u_b + h p(s_a; S_m)
+//     reverse  pbar = h ubar;  sbar += pbar dp/dt;  lambda_m += pbar dp/dS_m;
+//              u_b = u_a - h p;  ubar += -cd sbar;  s_b = s_a + cd u_b
 // lambda_m accumulates in LDS per workgroup (all nodes of a workgroup visit the
 // same slice at the same time) and is flushed with global atomics once per
 // chunk of slices; its summation order therefore varies at the rounding level.
@@ -75,13 +76,13 @@ __device__ inline void f0_grad_1d( const nufi_f0 &p, double x, double u, double 
     }
 }
 
-template <int BLOCK>
+template <int BLOCK, bool POW2>
 __global__ __launch_bounds__( BLOCK )
 void adjoint1d_kernel( adjoint1_params p )
 {
     extern __shared__ double lds[];
-    double *Dl  = lds;                                   // chunk of difference slices
-    double *lam = lds + size_t( p.chunk ) * p.stride_d;  // chunk of folded slice adjoints
+    double *Ql  = lds;                                   // chunk of kick-polynomial slices
+    double *lam = lds + size_t( p.chunk ) * p.stride_q;  // chunk of folded slice adjoints
     constexpr int WPB = BLOCK / WAVE;
     const uint32_t lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
     const uint32_t g = blockIdx.x * WPB + wave;
@@ -90,26 +91,26 @@ void adjoint1d_kernel( adjoint1_params p )
     const uint32_t iu = j * WAVE + lane;
     const bool active = task && iu < p.Nu;
 
-    const int Nx = int( p.Nx );
+    const int Nx = int( p.Nx ), mask = Nx - 1;
     const double Nxd = p.Nx, invNx = 1.0 / Nxd;
-    const double cd = p.dt * p.dx_inv, ck = -0.5 * p.dt * p.dx_inv;
-    const uint32_t sd = p.stride_d;
+    const double cd = p.dt * p.dx_inv, kick = -0.5 * p.dt * p.dx_inv;
+    const uint32_t sq = p.stride_q, q2 = p.q2_off;
 
     double s = double( ix );
     double u = p.u_min + iu*p.du + p.du/2;
 
-    // forward: slices n-1 .. 0 (the trace of trace1d_fast_body)
+    // forward: slices n-1 .. 0, the step of trace1d_poly_body
     for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
     {
         const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
         const int cnt = int( mhi - mlo + 1 );
         __syncthreads();
-        for ( uint32_t q = threadIdx.x; q < uint32_t( cnt ) * sd; q += BLOCK ) Dl[q] = p.dhist[ size_t( mlo ) * sd + q ];
+        build_poly<BLOCK>( Ql, p.dhist, p.stride_d, sq, q2, mlo, cnt, p.Nx + 2, kick );
         __syncthreads();
         for ( int jj = cnt - 1; jj >= 0; --jj )
         {
-            const double kf = ( mlo + jj == 0 ) ? 0.5 * ck : ck;
-            sv_step( s, u, Dl + jj * sd + 1, cd, kf, Nxd, invNx );
+            if ( POW2 && poly_wrap_due( mlo + jj ) ) rewrap( s, Nxd, invNx );
+            poly_step<POW2>( s, u, Ql + jj * sq, q2, cd, ( mlo + jj == 0 ) ? 0.5 : 1.0, mask, Nxd, invNx );
         }
     }
 
@@ -131,27 +132,25 @@ void adjoint1d_kernel( adjoint1_params p )
         const int64_t mhi = ( mlo + p.chunk - 1 < int64_t( p.n ) - 1 ) ? mlo + p.chunk - 1 : int64_t( p.n ) - 1;
         const int cnt = int( mhi - mlo + 1 );
         __syncthreads();
-        for ( uint32_t q = threadIdx.x; q < uint32_t( cnt ) * sd; q += BLOCK ) Dl[q] = p.dhist[ size_t( mlo ) * sd + q ];
+        build_poly<BLOCK>( Ql, p.dhist, p.stride_d, sq, q2, mlo, cnt, p.Nx + 2, kick );
         for ( uint32_t q = threadIdx.x; q < uint32_t( cnt ) * p.Nx; q += BLOCK ) lam[q] = 0.0;
         __syncthreads();
         for ( int jj = 0; jj < cnt; ++jj )
         {
-            const double *D = Dl + jj * sd + 1;
-            const double kf = ( mlo + jj == 0 ) ? 0.5 * ck : ck;
-            // s is the wrapped position the forward step produced: the same cell and t
+            const double h = ( mlo + jj == 0 ) ? 0.5 : 1.0;
+            // the cell and t the forward step used at this position
             const double k = floor( s );
             const double t = s - k;
-            const int i = int( k );
-            const double D0 = D[i], D1 = D[i+1], D2 = D[i+2];
-            const double a = 1.0 - t;
-            const double r2 = fma( a*a, D0 - D1, fma( t*t, D2 - D1, D1 + D1 ) );
-            const double dr2 = 2.0 * ( a * ( D1 - D0 ) + t * ( D2 - D1 ) );
-            const double rbar = kf * ubar;
-            sbar = fma( rbar, dr2, sbar );
+            const int i = POW2 ? ( int( k ) & mask ) : int( k );
+            double dpdt;
+            const double pv = poly_eval( Ql + jj * sq, q2, i, t, dpdt );
+            const double pbar = h * ubar;
+            sbar = fma( pbar, dpdt, sbar );
             if ( active )
             {
-                // r2 = a^2 D0 + (2 - a^2 - t^2) D1 + t^2 D2, D_l = c[i+l+1] - c[i+l]
-                const double d0 = rbar * ( a*a ), d1 = rbar * ( 2.0 - a*a - t*t ), d2 = rbar * ( t*t );
+                // Q = kick (q0, q1, q2); q0 = D0 + D1, q1 = 2 (D1 - D0), q2 = D0 - 2 D1 + D2
+                const double q0 = kick * pbar, q1 = kick * pbar * t, q2 = kick * pbar * ( t*t );
+                const double d0 = q0 - 2.0*q1 + q2, d1 = q0 + 2.0*q1 - 2.0*q2, d2 = q2;
                 double *L = lam + jj * p.Nx;
                 const int i0 = ( i + Nx ) % Nx;
                 atomicAdd( L + i0, -d0 );
@@ -160,11 +159,10 @@ void adjoint1d_kernel( adjoint1_params p )
                 atomicAdd( L + ( i0 + 3 ) % Nx, d2 );
             }
             // undo the step
-            u = fma( -kf, r2, u );
+            u = fma( -h, pv, u );
             ubar = fma( -cd, sbar, ubar );
             s = fma( cd, u, s );
-            double tt;
-            (void)wrap_locate( s, Nxd, invNx, tt );
+            if ( !POW2 || poly_wrap_due( mlo + jj ) ) rewrap( s, Nxd, invNx );
         }
         __syncthreads();
         for ( uint32_t q = threadIdx.x; q < uint32_t( cnt ) * p.Nx; q += BLOCK )
@@ -229,7 +227,7 @@ void reduce_grad_kernel( const double *part, uint32_t n_tasks, uint32_t npar, do
 
 size_t adjoint1d_lds_bytes( const adjoint1_params &p )
 {
-    return size_t( p.chunk ) * ( p.stride_d + p.Nx ) * sizeof(double);
+    return size_t( p.chunk ) * ( p.stride_q + p.Nx ) * sizeof(double);
 }
 
 hipError_t launch_adjoint1d( const adjoint1_params &p, hipStream_t st )
@@ -238,13 +236,16 @@ hipError_t launch_adjoint1d( const adjoint1_params &p, hipStream_t st )
     constexpr int BLOCK = 256;
     const uint32_t grid = ( p.n_tasks + BLOCK / WAVE - 1 ) / ( BLOCK / WAVE );
     const size_t lds = adjoint1d_lds_bytes( p );
+    const bool pow2 = ( p.Nx & ( p.Nx - 1 ) ) == 0;
+    const void *k = pow2 ? reinterpret_cast<const void*>( adjoint1d_kernel<BLOCK,true> )
+                         : reinterpret_cast<const void*>( adjoint1d_kernel<BLOCK,false> );
     if ( lds > 65536 )
     {
-        hipError_t e = hipFuncSetAttribute( reinterpret_cast<const void*>( adjoint1d_kernel<BLOCK> ),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, int( lds ) );
+        hipError_t e = hipFuncSetAttribute( k, hipFuncAttributeMaxDynamicSharedMemorySize, int( lds ) );
         if ( e != hipSuccess ) return e;
     }
-    hipLaunchKernelGGL( adjoint1d_kernel<BLOCK>, dim3( grid ), dim3( BLOCK ), lds, st, p );
+    if ( pow2 ) hipLaunchKernelGGL( ( adjoint1d_kernel<BLOCK,true> ), dim3( grid ), dim3( BLOCK ), lds, st, p );
+    else        hipLaunchKernelGGL( ( adjoint1d_kernel<BLOCK,false> ), dim3( grid ), dim3( BLOCK ), lds, st, p );
     return hipGetLastError();
 }
 

@@ -25,10 +25,9 @@ struct nufi_batch
     std::vector<nufi_config> confs;
     uint64_t n_runs = 0, Nt_max = 0;
     std::vector<size_t> off_hist, off_dhist, off_part, off_en, off_tab;
-    std::vector<uint32_t> st, sd, wpx;
+    std::vector<uint32_t> st, sd, sq, wpx;
 
     double *hist = nullptr, *dhist = nullptr, *partials = nullptr, *energies = nullptr;
-    float *dhist32 = nullptr;
     double2 *tw = nullptr, *mult = nullptr;
     double *ewt = nullptr;
     batch_run *d_runs = nullptr;
@@ -36,7 +35,7 @@ struct nufi_batch
     uint32_t *d_wg_run = nullptr;
     size_t d_wg_cap = 0;
     std::vector<batch_run> h_runs;
-    uint32_t n_wg = 0, max_N = 0, max_sd = 0;
+    uint32_t n_wg = 0, max_N = 0, max_sq = 0;
     launch_shape shape { 1024, 1, 16 };
     size_t trace_lds = 0, field_lds = 0;
 
@@ -81,7 +80,7 @@ int balloc( nufi_batch *b, T **p, size_t count, const char *what )
 void bfree_all( nufi_batch *b )
 {
     if ( b->graph ) (void)hipGraphExecDestroy( b->graph );
-    void *ptrs[] = { b->hist, b->dhist, b->partials, b->energies, b->dhist32, b->tw, b->mult, b->ewt,
+    void *ptrs[] = { b->hist, b->dhist, b->partials, b->energies, b->tw, b->mult, b->ewt,
                      b->d_runs, b->d_fields, b->d_wg_run };
     for ( void *p : ptrs ) if ( p ) (void)hipFree( p );
     if ( b->stream ) (void)hipStreamDestroy( b->stream );
@@ -91,19 +90,24 @@ void bfree_all( nufi_batch *b )
 int apply_shape( nufi_batch *b, const launch_shape &s )
 {
     const size_t elem = b->variant == TRACE_FP32C ? sizeof(float) : sizeof(double);
-    const size_t lds = size_t( s.chunk ) * b->max_sd * elem;
-    if ( !trace1d_shape_ok( b->variant, b->order, s ) || lds > 65536 )
-        return bfail( b, NUFI_ERR_UNSUPPORTED, "batch: launch shape not built or LDS over 64 KiB" );
-    const uint32_t per_wg = uint32_t( s.block / 64 ) * uint32_t( s.npt );
+    if ( !trace1d_shape_ok( b->variant, b->order, s ) )
+        return bfail( b, NUFI_ERR_UNSUPPORTED, "batch: launch shape not built" );
+    // per run: the table chunk (bounded by the register prefetch) and its LDS,
+    // or the x-major f buffer of the workgroup, whichever is larger
+    size_t lds = size_t( s.block ) * size_t( s.npt ) * sizeof(double);
     std::vector<uint32_t> wg_run;
     for ( uint64_t r = 0; r < b->n_runs; ++r )
     {
         batch_run &h = b->h_runs[r];
-        h.p.chunk = uint32_t( s.chunk );
+        trace1_params q = h.p;
+        q.n = h.Nt;                                    // the chunk bound of the longest step
+        h.p.chunk = trace1d_chunk( q, b->variant, s );
+        lds = std::max( lds, size_t( h.p.chunk ) * b->sq[r] * elem );
         h.wg_begin = uint32_t( wg_run.size() );
-        h.n_wg = ( h.p.n_tasks + per_wg - 1 ) / per_wg;
+        h.n_wg = trace1d_grid( h.p, b->variant, s );
         wg_run.insert( wg_run.end(), h.n_wg, uint32_t( r ) );
     }
+    if ( lds > 65536 ) return bfail( b, NUFI_ERR_UNSUPPORTED, "batch: launch shape needs more than 64 KiB of LDS" );
     if ( wg_run.size() >= ( size_t( 1 ) << 31 ) ) return bfail( b, NUFI_ERR_RANGE, "batch: too many workgroups" );
     if ( wg_run.size() > b->d_wg_cap )
     {
@@ -175,8 +179,9 @@ int nufi_batch_create( const nufi_config *confs, uint64_t n_runs, int order, int
         const nufi_config &c = confs[r];
         const uint32_t st = uint32_t( c.Nx + order - 1 );
         const uint32_t sd = uint32_t( ( c.Nx + order + 3 ) & ~uint64_t( 3 ) );
-        const uint32_t wpx = uint32_t( ( c.Nu + 63 ) / 64 );
-        b->st.push_back( st ); b->sd.push_back( sd ); b->wpx.push_back( wpx );
+        const uint32_t wpx = xm_ppx( uint32_t( c.Nu ) );     // x-major partials per x
+        const uint32_t sq = poly_stride( uint32_t( c.Nx ) );
+        b->st.push_back( st ); b->sd.push_back( sd ); b->sq.push_back( sq ); b->wpx.push_back( wpx );
         b->off_hist.push_back( n_hist );  n_hist  = align_up( n_hist + ( c.Nt + 1 ) * st + 8, 32 );
         b->off_dhist.push_back( n_dhist ); n_dhist = align_up( n_dhist + ( c.Nt + 1 ) * sd + 8, 32 );
         b->off_part.push_back( n_part );  n_part  = align_up( n_part + c.Nx * wpx, 32 );
@@ -184,12 +189,11 @@ int nufi_batch_create( const nufi_config *confs, uint64_t n_runs, int order, int
         b->off_tab.push_back( n_tab );    n_tab   = align_up( n_tab + c.Nx, 16 );
         b->Nt_max = std::max<uint64_t>( b->Nt_max, c.Nt );
         b->max_N = std::max<uint32_t>( b->max_N, uint32_t( c.Nx ) );
-        b->max_sd = std::max<uint32_t>( b->max_sd, sd );
+        b->max_sq = std::max<uint32_t>( b->max_sq, sq );
     }
     int rc;
     if ( ( rc = balloc( b, &b->hist, n_hist, "batch history" ) ) ) return bail( rc );
     if ( ( rc = balloc( b, &b->dhist, n_dhist, "batch differences" ) ) ) return bail( rc );
-    if ( precision == NUFI_FP32C && ( rc = balloc( b, &b->dhist32, n_dhist, "batch FP32 differences" ) ) ) return bail( rc );
     if ( ( rc = balloc( b, &b->partials, n_part, "batch partials" ) ) ) return bail( rc );
     if ( ( rc = balloc( b, &b->energies, n_en, "batch energies" ) ) ) return bail( rc );
     if ( ( rc = balloc( b, &b->tw, n_tab, "batch twiddles" ) ) ) return bail( rc );
@@ -213,12 +217,13 @@ int nufi_batch_create( const nufi_config *confs, uint64_t n_runs, int order, int
         trace1_params &p = b->h_runs[r].p;
         p = trace1_params {};
         p.dhist = b->dhist + b->off_dhist[r];
-        p.dhist32 = b->dhist32 ? b->dhist32 + b->off_dhist[r] : nullptr;
+        p.stride_q = b->sq[r]; p.q2_off = poly_q2( uint32_t( c.Nx ) ); p.kick = -0.5 * c.dt * c.dx_inv;
         p.partials = b->partials + b->off_part[r];
         p.q_begin = 0; p.q_end = c.Nx * c.Nu;
-        p.Nx = uint32_t( c.Nx ); p.Nu = uint32_t( c.Nu ); p.stride_d = b->sd[r]; p.wpx = b->wpx[r];
-        p.ix_first = 0; p.n_tasks = uint32_t( c.Nx * b->wpx[r] );
+        p.Nx = uint32_t( c.Nx ); p.Nu = uint32_t( c.Nu ); p.stride_d = b->sd[r];
+        p.ix_first = 0; p.ix_count = uint32_t( c.Nx );
         p.order = uint32_t( order ); p.mode = MODE_RHO;
+        trace1d_xm_setup( p );
         p.x_min = c.x_min; p.dx = c.dx; p.dx_inv = c.dx_inv; p.Lx = c.Lx; p.Lx_inv = c.Lx_inv; p.dt = c.dt;
         p.nx_min = c.x_min; p.ndx = c.dx; p.nu_min = c.u_min; p.ndu = c.du;
         p.f0 = c.f0;
@@ -236,7 +241,6 @@ int nufi_batch_create( const nufi_config *confs, uint64_t n_runs, int order, int
         f.mult = b->mult + b->off_tab[r]; f.ewt = b->ewt + b->off_tab[r];
         f.slice = b->hist + b->off_hist[r];
         f.dslice = b->dhist + b->off_dhist[r];
-        f.dslice32 = b->dhist32 ? b->dhist32 + b->off_dhist[r] : nullptr;
         f.energy = b->energies + b->off_en[r];
         f.sy = b->st[r]; f.sz = b->st[r]; f.stride_d = b->sd[r];
         fields[r].Nt = c.Nt; fields[r].st = b->st[r]; fields[r].sd = b->sd[r];
@@ -249,7 +253,7 @@ int nufi_batch_create( const nufi_config *confs, uint64_t n_runs, int order, int
     b->field_lds = field_batch_lds_bytes( b->max_N );
 
     const size_t elem = b->variant == TRACE_FP32C ? sizeof(float) : sizeof(double);
-    const int chunk = int( std::max<size_t>( 1, std::min<size_t>( 64, 65536 / ( size_t( b->max_sd ) * elem ) ) ) );
+    const int chunk = int( std::max<size_t>( 1, std::min<size_t>( 64, 65536 / ( size_t( b->max_sq ) * elem ) ) ) );
     if ( ( rc = apply_shape( b, launch_shape { 1024, 1, chunk } ) ) ) return bail( rc );
     *out = b;
     return NUFI_OK;
@@ -351,7 +355,7 @@ int nufi_batch_autotune( nufi_batch *b, uint64_t n_probe, int repeats, int *bloc
     BHIP( b, hipSetDevice( b->device ) );
     BHIP( b, hipStreamSynchronize( b->stream ) );
     const size_t elem = b->variant == TRACE_FP32C ? sizeof(float) : sizeof(double);
-    const int cmax = int( std::max<size_t>( 1, std::min<size_t>( 64, 65536 / ( size_t( b->max_sd ) * elem ) ) ) );
+    const int cmax = int( std::max<size_t>( 1, std::min<size_t>( 64, 65536 / ( size_t( b->max_sq ) * elem ) ) ) );
     const launch_shape saved = b->shape;
     hipEvent_t e0, e1;
     BHIP( b, hipEventCreate( &e0 ) );
@@ -360,7 +364,7 @@ int nufi_batch_autotune( nufi_batch *b, uint64_t n_probe, int repeats, int *bloc
     launch_shape best_s = saved;
     int rc = NUFI_OK;
     for ( int bl : { 256, 512, 1024 } )
-    for ( int np : { 1, 2 } )
+    for ( int np : { 1, 2, 4 } )
     for ( int ch : { std::max( 1, cmax / 4 ), std::max( 1, cmax / 2 ), cmax } )
     {
         if ( ( rc = apply_shape( b, launch_shape { bl, np, ch } ) ) ) { rc = NUFI_OK; continue; }

@@ -3,9 +3,11 @@
 // A context owns, on one device and one HIP stream (default priority):
 //   hist     (Nt+1) coefficient slices of (Nx+order-1)^dim reals   FP64
 //   dhist    1D: first differences of every slice (stride_d)        FP64
-//   dhist32  1D FP32C: the differences rounded to FP32
+//            (the fast, FP32C and adjoint kernels build the per-cell kick
+//            polynomials from it in LDS)
 //   hist32   3D FP32C: the coefficients rounded to FP32
-//   partials one compensated partial sum per wave of 64 velocity nodes
+//   partials compensated partial sums of f: per wave of 64 velocity nodes
+//            (strict), per x-major velocity group (fast, FP32C)
 //   rho, energies, the FFT twiddles and Fourier multipliers.
 // Kernels: trace1d.hip / trace3d.hip (the characteristic trace), field.hip
 // (Poisson + fit).  Reference semantics are cited at each entry point.
@@ -34,14 +36,17 @@ struct nufi_ctx
     hipStream_t stream = nullptr;
     uint32_t st = 0, sd = 0, sy = 0, sz = 0;
     uint64_t n_space = 0, n_vel = 0, n_nodes = 0;
-    uint32_t wpx = 0;
+    uint32_t wpx = 0;          // wave-task partials per spatial point (strict 1D, 3D)
+    uint32_t ppx = 0;          // x-major partials per spatial point (fast and FP32C)
+    uint32_t vgroup = 0;       // 3D x-major: velocities per partial
     int field_ok = 0;
     int n_cu = 256;
     size_t lds_cap = 65536;
     launch_shape shape { 256, 1, 1 };
 
     double *hist = nullptr, *dhist = nullptr;
-    float *dhist32 = nullptr, *hist32 = nullptr;
+    float *hist32 = nullptr;
+    uint32_t sq = 0;           // 1D: LDS kick-polynomial table stride (poly_stride)
     double *partials = nullptr, *rho = nullptr, *energies = nullptr, *values = nullptr;
     double *scratch_slice = nullptr, *metrics = nullptr, *mpartials = nullptr, *energy_tmp = nullptr;
     double *rho_hist = nullptr, *lambda = nullptr, *gvec = nullptr, *grad_part = nullptr, *grad = nullptr;
@@ -101,7 +106,7 @@ int dev_alloc( nufi_ctx *c, T **p, size_t count, const char *what )
 
 void free_all( nufi_ctx *c )
 {
-    void *ptrs[] = { c->hist, c->dhist, c->dhist32, c->hist32, c->partials, c->rho, c->energies,
+    void *ptrs[] = { c->hist, c->dhist, c->hist32, c->partials, c->rho, c->energies,
                      c->values, c->scratch_slice, c->metrics, c->mpartials, c->energy_tmp,
                      c->rho_hist, c->lambda, c->gvec, c->grad_part, c->grad, c->tw_x, c->tw_y, c->tw_z,
                      c->mult_step, c->mult_pois, c->mult_fit, c->ewt };
@@ -149,6 +154,12 @@ int build_tables( nufi_ctx *c )
     return NUFI_OK;
 }
 
+// partials per spatial point of the rho kernel in use
+uint32_t rho_ppx( const nufi_ctx *c )
+{
+    return c->variant != TRACE_STRICT ? c->ppx : c->wpx;
+}
+
 // Default launch shapes (the autotuner refines them): measured best shapes on
 // MI355X for the two-stream 1D grid (1024 threads, chunk 64) and 3D Landau 16^3
 // (FP64: 1024 threads; FP32C: 256 threads x 4 nodes).
@@ -158,14 +169,19 @@ void default_shape( nufi_ctx *c )
     const size_t elem = ( c->variant == TRACE_FP32C ) ? sizeof(float) : sizeof(double);
     if ( c->dim == 1 )
     {
-        int block = 1024;
-        while ( block > 64 && waves / uint64_t( block / 64 ) < uint64_t( c->n_cu ) ) block /= 2;
-        if ( c->variant == TRACE_STRICT )
-            block = block >= 1024 ? 1024 : ( block >= 256 ? 256 : 64 );
         const size_t budget = std::min<size_t>( c->lds_cap, 64 * 1024 );
-        int chunk = int( budget / ( size_t( c->sd ) * elem ) );
+        const size_t row = ( c->variant == TRACE_STRICT ) ? c->sd : c->sq;
+        int chunk = int( budget / ( row * elem ) );
         chunk = std::max( 1, std::min( chunk, 64 ) );
-        c->shape = launch_shape { block, 1, chunk };
+        if ( c->variant == TRACE_STRICT )
+        {
+            int block = 1024;
+            while ( block > 64 && waves / uint64_t( block / 64 ) < uint64_t( c->n_cu ) ) block /= 2;
+            block = block >= 1024 ? 1024 : ( block >= 256 ? 256 : 64 );
+            c->shape = launch_shape { block, 1, chunk };
+        }
+        else
+            c->shape = launch_shape { 1024, 1, chunk };   // x-major: 16 velocity rows per workgroup
     }
     else
     {
@@ -209,10 +225,11 @@ int launch_trace( nufi_ctx *c, uint64_t n, uint64_t q_begin, uint64_t q_end, uin
     if ( c->dim == 1 )
     {
         trace1_params p {};
-        p.dhist = c->dhist; p.dhist32 = c->dhist32; p.partials = c->partials;
+        p.dhist = c->dhist; p.partials = c->partials;
+        p.stride_q = c->sq; p.q2_off = poly_q2( uint32_t( c->conf.Nx ) ); p.kick = -0.5 * c->conf.dt * c->conf.dx_inv;
         p.n = n; p.q_begin = q_begin; p.q_end = q_end;
         p.Nx = uint32_t( c->conf.Nx ); p.Nu = uint32_t( c->conf.Nu ); p.stride_d = c->sd; p.wpx = c->wpx;
-        p.ix_first = uint32_t( l0 ); p.n_tasks = uint32_t( tasks );
+        p.ix_first = uint32_t( l0 ); p.n_tasks = uint32_t( tasks ); p.ix_count = uint32_t( l1 - l0 + 1 );
         p.order = uint32_t( c->order );
         p.mode = MODE_RHO;
         p.x_min = c->conf.x_min; p.dx = c->conf.dx; p.dx_inv = c->conf.dx_inv;
@@ -231,6 +248,7 @@ int launch_trace( nufi_ctx *c, uint64_t n, uint64_t q_begin, uint64_t q_end, uin
         p.Nu = uint32_t( c->conf.Nu ); p.Nv = uint32_t( c->conf.Nv ); p.Nw = uint32_t( c->conf.Nw );
         p.sy = c->sy; p.sz = c->sz; p.st = c->st; p.wpx = c->wpx;
         p.l_first = uint32_t( l0 ); p.n_tasks = uint32_t( tasks );
+        p.n_count = uint32_t( l1 - l0 + 1 ); p.vgroup = c->vgroup; p.ppx = c->ppx;
         p.order = uint32_t( c->order );
         const nufi_config &g = c->conf;
         p.x_min = g.x_min; p.y_min = g.y_min; p.z_min = g.z_min;
@@ -266,7 +284,7 @@ field_params base_field( nufi_ctx *c )
     f.Nz = c->dim == 3 ? uint32_t( c->conf.Nz ) : 1u;
     f.N = f.Nx * f.Ny * f.Nz;
     f.log2x = log2_or_none( f.Nx ); f.log2y = log2_or_none( f.Ny ); f.log2z = log2_or_none( f.Nz );
-    f.wpx = c->wpx;
+    f.wpx = rho_ppx( c );
     f.weight = rho_weight( c );
     f.tw_x = c->tw_x; f.tw_y = c->tw_y; f.tw_z = c->tw_z;
     f.ewt = c->ewt;
@@ -374,17 +392,21 @@ int nufi_create( const nufi_config *conf, const nufi_config *conf_metrics, int o
     }
     c->n_nodes = c->n_space * c->n_vel;
     c->wpx = uint32_t( ( c->n_vel + 63 ) / 64 );
+    if ( c->dim == 1 ) c->ppx = xm_ppx( uint32_t( conf->Nu ) );
+    else
+    {
+        c->vgroup = xm3_vgroup( c->n_space, c->n_vel );
+        c->ppx = uint32_t( ( c->n_vel + c->vgroup - 1 ) / c->vgroup );
+    }
     c->field_ok = field_supported( c->dim, uint32_t( conf->Nx ), c->dim == 3 ? uint32_t( conf->Ny ) : 1u,
                                    c->dim == 3 ? uint32_t( conf->Nz ) : 1u );
     const uint64_t slices = conf->Nt + 1;
     if ( ( rc = dev_alloc( c, &c->hist, slices * c->st + 8, "history" ) ) ) return bail( rc );
     if ( c->dim == 1 && ( rc = dev_alloc( c, &c->dhist, slices * c->sd + 8, "difference history" ) ) ) return bail( rc );
-    if ( precision == NUFI_FP32C )
-    {
-        if ( c->dim == 1 && ( rc = dev_alloc( c, &c->dhist32, slices * c->sd + 8, "FP32 history" ) ) ) return bail( rc );
-        if ( c->dim == 3 && ( rc = dev_alloc( c, &c->hist32, slices * c->st + 8, "FP32 history" ) ) ) return bail( rc );
-    }
-    if ( ( rc = dev_alloc( c, &c->partials, c->n_space * c->wpx, "partials" ) ) ) return bail( rc );
+    if ( c->dim == 1 ) c->sq = poly_stride( uint32_t( conf->Nx ) );
+    if ( precision == NUFI_FP32C && c->dim == 3 && ( rc = dev_alloc( c, &c->hist32, slices * c->st + 8, "FP32 history" ) ) )
+        return bail( rc );
+    if ( ( rc = dev_alloc( c, &c->partials, c->n_space * std::max( c->wpx, c->ppx ), "partials" ) ) ) return bail( rc );
     if ( ( rc = dev_alloc( c, &c->rho, c->n_space, "rho" ) ) ) return bail( rc );
     if ( ( rc = dev_alloc( c, &c->values, c->n_space, "values" ) ) ) return bail( rc );
     if ( ( rc = dev_alloc( c, &c->energies, slices, "energies" ) ) ) return bail( rc );
@@ -445,14 +467,14 @@ int nufi_compute_rho( nufi_ctx *c, uint64_t n, uint64_t q_begin, uint64_t q_end 
     if ( q_begin == q_end )
     {
         // empty range: the device rho becomes zero, so download_rho adds nothing
-        hipError_t e = launch_reduce_partials( c->partials, c->wpx, 0.0, 0, 0, uint32_t( c->n_space ), c->rho, c->stream );
+        hipError_t e = launch_reduce_partials( c->partials, rho_ppx( c ), 0.0, 0, 0, uint32_t( c->n_space ), c->rho, c->stream );
         if ( e != hipSuccess ) return hip_fail( c, e, "reduce launch" );
         return NUFI_OK;
     }
     uint64_t first, count;
     int rc = launch_trace( c, n, q_begin, q_end, &first, &count );
     if ( rc ) return rc;
-    hipError_t e = launch_reduce_partials( c->partials, c->wpx, rho_weight( c ), uint32_t( first ), uint32_t( count ),
+    hipError_t e = launch_reduce_partials( c->partials, rho_ppx( c ), rho_weight( c ), uint32_t( first ), uint32_t( count ),
                                            uint32_t( c->n_space ), c->rho, c->stream );
     if ( e != hipSuccess ) return hip_fail( c, e, "reduce launch" );
     return NUFI_OK;
@@ -479,7 +501,6 @@ int nufi_upload_phi( nufi_ctx *c, uint64_t n, const double *history_base )
     HIP_TRY( c, hipMemcpyAsync( dst, history_base + n * c->st, c->st * sizeof(double), hipMemcpyHostToDevice, c->stream ) );
     hipError_t e = launch_slice_tables( c->dim, dst, c->st, uint32_t( c->conf.Nx ),
                                         c->dhist ? c->dhist + n * c->sd : nullptr,
-                                        c->dhist32 ? c->dhist32 + n * c->sd : nullptr,
                                         c->hist32 ? c->hist32 + n * c->st : nullptr, c->stream );
     if ( e != hipSuccess ) return hip_fail( c, e, "slice table launch" );
     // the copy reads host memory the caller may reuse: complete it before returning
@@ -580,7 +601,6 @@ static int step_async( nufi_ctx *c, uint64_t n, double *rho_out )
     f.mult = c->mult_step;
     f.slice = c->hist + n * c->st;
     f.dslice = c->dhist ? c->dhist + n * c->sd : nullptr;
-    f.dslice32 = c->dhist32 ? c->dhist32 + n * c->sd : nullptr;
     f.slice32 = c->hist32 ? c->hist32 + n * c->st : nullptr;
     f.energy = c->energies + n;
     f.rho_out = rho_out ? rho_out : c->rho;
@@ -660,12 +680,14 @@ int nufi_adjoint_grad( nufi_ctx *c, double *dJ_da, int n_par, double *J_out )
     af.g = c->gvec; af.tw = c->tw_x; af.mult = c->mult_step; af.ewt = c->ewt; af.dt = c->conf.dt;
     adjoint1_params ap {};
     ap.dhist = c->dhist; ap.g = c->gvec; ap.lambda = c->lambda; ap.grad_part = c->grad_part;
-    ap.Nx = uint32_t( Nx ); ap.Nu = uint32_t( c->conf.Nu ); ap.stride_d = c->sd; ap.wpx = c->wpx;
+    ap.Nx = uint32_t( Nx ); ap.Nu = uint32_t( c->conf.Nu ); ap.stride_d = c->sd;
+    ap.stride_q = c->sq; ap.q2_off = poly_q2( uint32_t( Nx ) );
+    ap.wpx = c->wpx;
     ap.n_tasks = n_tasks; ap.npar = uint32_t( npar );
     ap.x_min = c->conf.x_min; ap.dx = c->conf.dx; ap.dx_inv = c->conf.dx_inv; ap.dt = c->conf.dt;
     ap.u_min = c->conf.u_min; ap.du = c->conf.du; ap.f0 = c->conf.f0;
     ap.chunk = uint32_t( std::max<size_t>( 1, std::min<size_t>( 32, std::min<size_t>( c->lds_cap, 65536 ) /
-                                                                  ( ( size_t( c->sd ) + Nx ) * 8 ) ) ) );
+                                                                  ( ( size_t( c->sq ) + Nx ) * 8 ) ) ) );
     for ( uint64_t n = Nt + 1; n-- > 0; )
     {
         af.rho = c->rho_hist + n * Nx; af.lambda = c->lambda + n * Nx;
@@ -780,7 +802,8 @@ int nufi_set_launch_shape( nufi_ctx *c, int block, int npt, int chunk )
     const bool ok = c->dim == 1 ? trace1d_shape_ok( c->variant, c->order, s )
                                 : trace3d_shape_ok( c->variant, c->order, s );
     const size_t elem = ( c->variant == TRACE_FP32C ) ? sizeof(float) : sizeof(double);
-    const size_t lds = size_t( chunk ) * ( c->dim == 1 ? c->sd : c->st ) * elem;
+    const size_t row = c->dim == 1 ? ( c->variant == TRACE_STRICT ? c->sd : c->sq ) : c->st;
+    const size_t lds = size_t( chunk ) * row * elem;
     if ( !ok || lds > c->lds_cap || lds > 160u * 1024u )
         return fail( c, NUFI_ERR_UNSUPPORTED, "set_launch_shape: shape not built or LDS too large" );
     c->shape = s;
@@ -808,8 +831,8 @@ int nufi_autotune( nufi_ctx *c, uint64_t n_probe, int repeats, int *block, int *
     {
         if ( c->variant == TRACE_STRICT )
         { for ( int b : { 256, 1024 } ) for ( int ch : { 16, 32, 64 } ) cand.push_back( { b, 1, ch } ); }
-        else
-            for ( int b : blocks1 ) for ( int np : { 1, 2 } ) for ( int ch : { 16, 32, 64 } ) cand.push_back( { b, np, ch } );
+        else    // x-major: trace1d_shape_ok keeps the shapes with whole summation groups
+            for ( int b : blocks1 ) for ( int np : npts ) for ( int ch : { 4, 8, 16, 32, 64 } ) cand.push_back( { b, np, ch } );
     }
     else
     {
@@ -825,7 +848,7 @@ int nufi_autotune( nufi_ctx *c, uint64_t n_probe, int repeats, int *block, int *
     double best = 1e300;
     launch_shape best_s = saved;
     const size_t elem = ( c->variant == TRACE_FP32C ) ? sizeof(float) : sizeof(double);
-    const size_t per = size_t( c->dim == 1 ? c->sd : c->st ) * elem;
+    const size_t per = size_t( c->dim == 1 ? ( c->variant == TRACE_STRICT ? c->sd : c->sq ) : c->st ) * elem;
     int rc = NUFI_OK;
     for ( const launch_shape &sh : cand )
     {

@@ -105,20 +105,11 @@ __device__ __forceinline__ void field_body( const field_params &p, double2 *lds2
     if ( p.dim == 1 )
     {
         const uint32_t len = p.Nx + o1;
-        for ( uint32_t i = threadIdx.x; i < len; i += FIELD_BLOCK )
-        {
-            const double c = R[ i % p.Nx ].x;
-            p.slice[i] = c;
-            if ( p.slice32 ) p.slice32[i] = float( c );
-        }
+        for ( uint32_t i = threadIdx.x; i < len; i += FIELD_BLOCK ) p.slice[i] = R[ i % p.Nx ].x;
         // extended differences: entry j holds D[j-1] = c[j] - c[j-1] (periodic)
-        for ( uint32_t j = threadIdx.x; j < p.stride_d; j += FIELD_BLOCK )
-        {
-            double d = 0.0;
-            if ( j < len + 1 ) d = R[ j % p.Nx ].x - R[ ( j + p.Nx - 1 ) % p.Nx ].x;
-            if ( p.dslice )   p.dslice[j] = d;
-            if ( p.dslice32 ) p.dslice32[j] = float( d );
-        }
+        if ( p.dslice )
+            for ( uint32_t j = threadIdx.x; j < p.stride_d; j += FIELD_BLOCK )
+                p.dslice[j] = ( j < len + 1 ) ? R[ j % p.Nx ].x - R[ ( j + p.Nx - 1 ) % p.Nx ].x : 0.0;
     }
     else
     {
@@ -153,8 +144,6 @@ void field_batch_kernel( const batch_field *__restrict__ runs, uint64_t n )
     field_params f = b.f;
     f.slice = b.f.slice + n * b.st;
     if ( f.dslice )   f.dslice   = b.f.dslice + n * b.sd;
-    if ( f.dslice32 ) f.dslice32 = b.f.dslice32 + n * b.sd;
-    if ( f.slice32 )  f.slice32  = b.f.slice32 + n * b.st;
     f.energy = b.f.energy + n;
     field_body( f, lds2b );
 }
@@ -178,17 +167,12 @@ __global__ void reduce_partials_kernel( const double *partials, uint32_t wpx, do
 // FP32 copy of an uploaded slice of `len` reals and, in 1D (period Nx), the
 // extended differences dslice[j] = c[j mod Nx] - c[(j-1) mod Nx], j <= len.
 __global__ void slice_tables_kernel( int dim, const double *slice, uint32_t len, uint32_t Nx, double *dslice,
-                                     float *dslice32, float *slice32 )
+                                     float *slice32 )
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if ( i > len ) return;
     if ( slice32 && i < len ) slice32[i] = float( slice[i] );
-    if ( dim == 1 )
-    {
-        const double d = slice[ i % Nx ] - slice[ ( i + Nx - 1 ) % Nx ];
-        if ( dslice )   dslice[i] = d;
-        if ( dslice32 ) dslice32[i] = float( d );
-    }
+    if ( dim == 1 && dslice ) dslice[i] = slice[ i % Nx ] - slice[ ( i + Nx - 1 ) % Nx ];
 }
 
 // metrics[k] = sum over tasks of partials[4 g + k], compensated, fixed order
@@ -269,11 +253,11 @@ hipError_t launch_reduce_partials( const double *partials, uint32_t wpx, double 
 }
 
 hipError_t launch_slice_tables( int dim, const double *slice, uint32_t len, uint32_t Nx, double *dslice,
-                                float *dslice32, float *slice32, hipStream_t st )
+                                float *slice32, hipStream_t st )
 {
     const uint32_t block = 256, grid = ( len + 1 + block - 1 ) / block;
     hipLaunchKernelGGL( slice_tables_kernel, dim3( grid ), dim3( block ), 0, st,
-                        dim, slice, len, Nx, dslice, dslice32, slice32 );
+                        dim, slice, len, Nx, dslice, slice32 );
     return hipGetLastError();
 }
 

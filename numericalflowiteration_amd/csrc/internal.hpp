@@ -23,10 +23,28 @@ enum trace_variant : int
 };
 
 // ---- 1D trace ----------------------------------------------------------------
-// Node q = ix*Nu + iu (cuda_kernel.cu:39-44).  The launch is cut into
-// wave-tasks: task g covers ix = ix_first + g / wpx and iu = (g % wpx)*64 +
-// lane; lanes with iu >= Nu or q outside [q_begin, q_end) contribute zero.
-// Each task writes one partial sum of f to partials[ix*wpx + g % wpx].
+// Node q = ix*Nu + iu (cuda_kernel.cu:39-44); nodes with q outside
+// [q_begin, q_end) contribute zero.  Two decompositions:
+//
+// velocity-major (strict and metrics kernels): the launch is cut into
+// wave-tasks; task g covers ix = ix_first + g / wpx and iu = (g % wpx)*64 +
+// lane, and writes one partial sum of f to partials[ix*wpx + g % wpx].
+//
+// x-major (fast and FP32C rho kernels): the 64 lanes of a wave take xw
+// consecutive x nodes (xw = min(Nx, 64)) at vpw = 64/xw velocities; lanes at
+// one velocity and neighbouring x stay close along their characteristics, so
+// the table reads of a wave are near-contiguous in LDS.  Workgroup b covers
+// x-tile b % n_xt (x nodes ix_first + xw*(b % n_xt) + 0..xw-1) and velocity
+// block b / n_xt of vb = NPT*WPB*vpw velocities; thread (wave w, lane) takes
+// the velocity rows (r*WPB + w)*vpw + lane/xw, r < NPT.  The f values are
+// staged in LDS and summed over fixed groups of XM_GROUP consecutive
+// velocities (a sequential compensated sum in iu order) into
+// partials[ix*ppx + iu/XM_GROUP], ppx = ceil(Nu/XM_GROUP): the partials do not
+// depend on the launch shape.
+constexpr uint32_t XM_GROUP = 16;
+inline uint32_t xm_xw( uint32_t Nx ) { return Nx < 64 ? Nx : 64; }
+inline uint32_t xm_ppx( uint32_t Nu ) { return ( Nu + XM_GROUP - 1 ) / XM_GROUP; }
+
 // Trace kernel outputs.
 enum trace_mode : int
 {
@@ -38,12 +56,15 @@ enum trace_mode : int
 struct trace1_params
 {
     const double *dhist;     // extended differences of slice m at dhist + m*stride_d (FP64)
-    const float  *dhist32;   // FP32 copy (TRACE_FP32C)
-    double       *partials;  // MODE_RHO: partials[ix*wpx + j]; MODE_METRICS: partials[4*g + k]
+    uint32_t stride_q, q2_off;        // LDS kick-polynomial table: slice stride, Q2 row (poly_stride, poly_q2)
+    double   kick;                    // -dt/(2 dx), the scale of the kick polynomials
+    double       *partials;  // velocity-major: partials[ix*wpx + j]; x-major: partials[ix*ppx + j];
+                             // MODE_METRICS: partials[4*g + k]
     uint64_t n;
     uint64_t q_begin, q_end;
     uint32_t Nx, Nu, stride_d, wpx;   // Nu / wpx: of the node grid
     uint32_t ix_first, n_tasks;
+    uint32_t ix_count, xw, vpw, n_xt, ppx;   // x-major: x range, tile shape, partials per x
     uint32_t chunk;          // slices per LDS chunk
     uint32_t order;
     int      mode;           // trace_mode
@@ -55,7 +76,19 @@ struct trace1_params
 
 // ---- 3D trace ----------------------------------------------------------------
 // Node q = l*Nvel + iv with l = (iz*Ny + iy)*Nx + ix and iv = (iw*Nv + iv')*Nu
-// + iu (cuda_kernel.cu:399-412).  Same wave-task cut with Nu -> Nvel.
+// + iu (cuda_kernel.cu:399-412).  The strict and metrics kernels use the
+// wave-task cut of the 1D case with Nu -> Nvel; the fast and FP32C rho
+// kernels the x-major layout of trace3d.hip (64 neighbouring spatial points
+// per wave, partials[l*ppx + iv/vgroup]).
+// Velocities per x-major partial: 256, halved (down to 16) until the launch
+// has at least 8192 waves; fixed per grid, so results do not depend on the shape.
+inline uint32_t xm3_vgroup( uint64_t n_space, uint64_t n_vel )
+{
+    uint32_t g = 256;
+    const uint64_t tiles = ( n_space + 63 ) / 64;
+    while ( g > 16 && tiles * ( ( n_vel + g - 1 ) / g ) < 8192 ) g /= 2;
+    return g;
+}
 struct trace3_params
 {
     const double *hist;      // coefficient slices (FP64), stride st
@@ -69,6 +102,8 @@ struct trace3_params
     uint32_t sy, sz, st;     // strides of a slice: sy = Nx+order-1, sz = sy*(Ny+order-1)
     uint32_t wpx;            // waves per spatial point: ceil(Nvel/64)
     uint32_t l_first, n_tasks;
+    uint32_t n_count, n_st;  // x-major: spatial points of the range, 64-point tiles
+    uint32_t vgroup, ppx;    // x-major: velocities per partial sum, partials per point
     uint32_t chunk;
     uint32_t order;
     double x_min, y_min, z_min, dx, dy, dz, dx_inv, dy_inv, dz_inv;
@@ -96,10 +131,18 @@ struct launch_shape
 hipError_t launch_trace1d( const trace1_params &p, int variant, const launch_shape &s, hipStream_t st );
 hipError_t launch_trace3d( const trace3_params &p, int variant, const launch_shape &s, hipStream_t st );
 size_t trace1d_lds_bytes( const trace1_params &p, int variant, const launch_shape &s );
+size_t trace1d_slice_elems( const trace1_params &p, int variant );
 hipError_t launch_trace1d_batch( const batch_run *runs, const uint32_t *wg_run, uint32_t n_wg, uint64_t n,
                                  int variant, const launch_shape &s, size_t lds, hipStream_t st );
 size_t trace3d_lds_bytes( const trace3_params &p, int variant, const launch_shape &s );
 bool trace1d_shape_ok( int variant, int order, const launch_shape &s );
+// x-major fields (xw, vpw, n_xt, ppx) from Nx, Nu and ix_count, the grid of
+// a launch (x-major or wave-task decomposition, by variant and mode) and the
+// slices per LDS chunk it uses (bounded by the step and, x-major, by the
+// register prefetch of the table: block * poly_maxc(block) entries)
+void trace1d_xm_setup( trace1_params &p );
+uint32_t trace1d_grid( const trace1_params &p, int variant, const launch_shape &s );
+uint32_t trace1d_chunk( const trace1_params &p, int variant, const launch_shape &s );
 bool trace3d_shape_ok( int variant, int order, const launch_shape &s );
 
 // ---- adjoint (adjoint1d.hip) ---------------------------------------------------
@@ -110,7 +153,7 @@ struct adjoint1_params
     double       *lambda;     // folded slice adjoints, slice m at lambda + m*Nx
     double       *grad_part;  // per wave-task parameter-gradient accumulators [g][NUFI_MAX_MODES]
     uint64_t n;
-    uint32_t Nx, Nu, stride_d, wpx, n_tasks, chunk, npar;
+    uint32_t Nx, Nu, stride_d, stride_q, q2_off, wpx, n_tasks, chunk, npar;
     double x_min, dx, dx_inv, dt, u_min, du;
     nufi_f0 f0;
 };
@@ -157,7 +200,6 @@ struct field_params
     double  *slice;              // history slice n (coefficients)
     double  *dslice;             // 1D differences of slice n (stride_d)
     float   *slice32;            // FP32 coefficients (3D FP32C) or null
-    float   *dslice32;           // FP32 differences (1D FP32C) or null
     double  *energy;             // energies[n]
     uint32_t sy, sz, stride_d;
 };
@@ -165,7 +207,7 @@ struct field_params
 hipError_t launch_field( const field_params &p, hipStream_t st );
 
 // One run of a batched sweep for the field kernel: f holds the bases of the
-// run's history (slice, dslice, dslice32, slice32 at slice 0; energy at step 0).
+// run's history (slice, dslice at slice 0; energy at step 0).
 struct batch_field
 {
     field_params f;
@@ -185,9 +227,22 @@ hipError_t launch_reduce_metrics( const double *partials, uint32_t n_tasks, doub
 hipError_t launch_reduce_partials( const double *partials, uint32_t wpx, double weight,
                                    uint32_t first, uint32_t count, uint32_t n_space,
                                    double *rho, hipStream_t st );
-// Differences (and FP32 copies) of an uploaded 1D slice.
+// Differences (1D) and the FP32 copy (3D FP32C) of an uploaded slice.
 hipError_t launch_slice_tables( int dim, const double *slice, uint32_t len, uint32_t Nx, double *dslice,
-                                float *dslice32, float *slice32, hipStream_t st );
+                                float *slice32, hipStream_t st );
+
+// The 1D kick-polynomial table of one slice, built in LDS by the fast, FP32C
+// and adjoint kernels from the difference history: for cell i (i = -1 .. Nx,
+// periodic; entry j = i+1) the coefficients of kick * 2 dphi/dt =
+// Q0 + Q1 t + Q2 t^2 on [0,1), with Q0 = kick (D0 + D1), Q1 = kick 2 (D1 - D0),
+// Q2 = kick ((D0 - D1) + (D2 - D1)) from the differences D_l = c[i+l+1] -
+// c[i+l] of the cell's window.  Layout: the pairs (Q0, Q1) at 2j, 2j+1, then
+// the Q2 row at poly_q2 + j; the stride is a multiple of 4 elements so that
+// every slice starts 16-byte aligned in FP64 and FP32.
+// Table entries of one chunk held in registers per thread while it is loaded.
+__host__ __device__ constexpr int poly_maxc( int block ) { return block >= 1024 ? 4 : 8; }
+__host__ __device__ inline uint32_t poly_q2( uint32_t Nx ) { return 2u * ( Nx + 2u ); }
+__host__ __device__ inline uint32_t poly_stride( uint32_t Nx ) { return ( 3u * ( Nx + 2u ) + 3u ) & ~3u; }
 
 } // namespace nufi_impl
 

@@ -30,6 +30,8 @@
 //   fp32c  : as fast, in FP32, with the position and velocity carried as
 //            unevaluated sums hi + lo updated by TwoSum / TwoProd; f0 and the
 //            rho sums stay FP64 (order 4).
+#include <algorithm>
+
 #include "device_common.hpp"
 #include "internal.hpp"
 #include "step1d.hpp"
@@ -90,6 +92,71 @@ __device__ __forceinline__ void write_partial( const trace1_params &p, const nod
     const csum v = wave_csum( csum { f, 0.0 } );
     if ( id.task && ( threadIdx.x % WAVE ) == 0 )
         p.partials[ size_t(id.ix) * p.wpx + id.j ] = v.s + v.c;
+}
+
+// ---- x-major layout (fast and FP32C rho kernels) ---------------------------
+struct xm_tile
+{
+    uint32_t xt, vb, xo, vr;   // x-tile, velocity block, lane's x offset and velocity row
+    bool lane_ok;              // the lane maps to an x node of the range
+};
+
+__device__ __forceinline__ xm_tile make_tile( const trace1_params &p, uint32_t wg )
+{
+    xm_tile t;
+    const uint32_t lane = threadIdx.x % WAVE;
+    t.xt = wg % p.n_xt; t.vb = wg / p.n_xt;
+    t.xo = lane % p.xw; t.vr = lane / p.xw;
+    t.lane_ok = t.vr < p.vpw && t.xt * p.xw + t.xo < p.ix_count;
+    return t;
+}
+
+struct xm_node
+{
+    uint32_t ix, iu, vi;       // vi: velocity row within the workgroup
+    bool active;
+};
+
+template <int WPB, int NPT>
+__device__ __forceinline__ xm_node make_xm_node( const trace1_params &p, const xm_tile &t, int r )
+{
+    xm_node id;
+    const uint32_t wave = threadIdx.x / WAVE;
+    id.vi = ( uint32_t( r ) * WPB + wave ) * p.vpw + t.vr;
+    id.ix = t.lane_ok ? p.ix_first + t.xt * p.xw + t.xo : p.ix_first;
+    id.iu = t.vb * ( uint32_t( NPT * WPB ) * p.vpw ) + id.vi;
+    const uint64_t q = uint64_t( id.ix ) * p.Nu + id.iu;
+    id.active = t.lane_ok && id.iu < p.Nu && q >= p.q_begin && q < p.q_end;
+    return id;
+}
+
+// f of the workgroup's nodes -> LDS -> one compensated sum per (x, group of
+// XM_GROUP consecutive velocities), summed in iu order.  `fbuf` may alias the
+// history chunk: the barrier below ends its use.
+template <int BLOCK, int NPT>
+__device__ __forceinline__ void xm_reduce( const trace1_params &p, const xm_tile &t, const xm_node id[NPT],
+                                           const double f[NPT], double *fbuf )
+{
+    constexpr uint32_t WPB = BLOCK / WAVE;
+    const uint32_t groups = ( uint32_t( NPT ) * WPB * p.vpw ) / XM_GROUP;
+    __syncthreads();
+    if ( t.vr < p.vpw )
+    {
+        #pragma unroll
+        for ( int r = 0; r < NPT; ++r ) fbuf[ id[r].vi * p.xw + t.xo ] = id[r].active ? f[r] : 0.0;
+    }
+    __syncthreads();
+    for ( uint32_t k = threadIdx.x; k < groups * p.xw; k += BLOCK )
+    {
+        const uint32_t xo = k % p.xw, gi = k / p.xw;
+        const uint32_t xl = t.xt * p.xw + xo, gg = t.vb * groups + gi;
+        if ( xl >= p.ix_count || gg >= p.ppx ) continue;
+        const double *col = fbuf + gi * XM_GROUP * p.xw + xo;
+        csum acc { 0.0, 0.0 };
+        #pragma unroll
+        for ( uint32_t v = 0; v < XM_GROUP; ++v ) acc = csum_add( acc, csum { col[ v * p.xw ], 0.0 } );
+        p.partials[ size_t( p.ix_first + xl ) * p.ppx + gg ] = acc.s + acc.c;
+    }
 }
 
 // The four metric integrands of one node (cuda_kernel.cu:73-78), wave-summed.
@@ -253,18 +320,76 @@ void trace1d_strict_kernel( trace1_params p )
     else write_metrics( p.partials, id.g, id.task, id.active, p.weight, f, u0*u0 );
 }
 
+// ---- fast FP64 with the kick-polynomial table (order 4, rho) ----------------
+// Per char-step: drift (FMA), floor, index, one 16-byte and one 8-byte LDS read,
+// two FMAs for Q0 + t (Q1 + t Q2), one FMA for the kick.
+template <int BLOCK, int NPT, bool POW2>
+__device__ __forceinline__ void trace1d_poly_body( const trace1_params &p, uint32_t wg, double *lds )
+{
+    constexpr int WPB = BLOCK / WAVE;
+    const xm_tile tile = make_tile( p, wg );
+    xm_node id[NPT];
+    double s[NPT], u[NPT];
+    #pragma unroll
+    for ( int r = 0; r < NPT; ++r )
+    {
+        id[r] = make_xm_node<WPB,NPT>( p, tile, r );
+        s[r] = double( id[r].ix );
+        u[r] = p.nu_min + id[r].iu*p.ndu + p.ndu/2;          // cuda_kernel.cu:44
+    }
+    const double Nxd = double( p.Nx ), invNx = 1.0 / Nxd;
+    const int mask = int( p.Nx ) - 1;
+    const double cd = p.dt * p.dx_inv;
+    const uint32_t sq = p.stride_q, q2 = p.q2_off, cells = p.Nx + 2;
+    poly_fetch<BLOCK,poly_maxc( BLOCK )> pf;
+    poly_prefetch_first( pf, p.dhist, p.stride_d, cells, int64_t( p.n ) - 1, p.chunk );
+
+    for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
+    {
+        const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
+        const int cnt = int( mhi - mlo + 1 );
+        poly_chunk_begin( pf, lds, p.dhist, p.stride_d, sq, q2, cells, p.kick, mhi, p.chunk );
+        for ( int jj = cnt - 1; jj >= 0; --jj )
+        {
+            const double *Q = lds + jj * sq;
+            const double h = ( mlo + jj == 0 ) ? 0.5 : 1.0;
+            if ( POW2 && poly_wrap_due( mlo + jj ) )
+            {
+                #pragma unroll
+                for ( int r = 0; r < NPT; ++r ) rewrap( s[r], Nxd, invNx );
+            }
+            #pragma unroll
+            for ( int r = 0; r < NPT; ++r ) poly_step<POW2>( s[r], u[r], Q, q2, cd, h, mask, Nxd, invNx );
+        }
+    }
+    double f[NPT];
+    #pragma unroll
+    for ( int r = 0; r < NPT; ++r ) f[r] = id[r].active ? f0_1d( p.f0, p.x_min + s[r]*p.dx, u[r] ) : 0.0;
+    xm_reduce<BLOCK,NPT>( p, tile, id, f, lds );
+}
+
+template <int BLOCK, int NPT, bool POW2>
+__global__ __launch_bounds__( BLOCK )
+void trace1d_poly_kernel( trace1_params p )
+{
+    extern __shared__ double lds[];
+    trace1d_poly_body<BLOCK,NPT,POW2>( p, blockIdx.x, lds );
+}
+
 // ---- FP32 compensated (order 4) --------------------------------------------
+// The FP32 kick-polynomial table: an 8-byte (Q0, Q1) and a 4-byte Q2 LDS
+// read per char-step.
 template <int BLOCK, int NPT>
 __device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint32_t wg, float *ldsf )
 {
     constexpr int WPB = BLOCK / WAVE;
-
-    node_id id[NPT];
+    const xm_tile tile = make_tile( p, wg );
+    xm_node id[NPT];
     float sh[NPT], sl[NPT], uh[NPT], ul[NPT];
     #pragma unroll
     for ( int r = 0; r < NPT; ++r )
     {
-        id[r] = make_node<WPB,NPT>( p, r, wg );
+        id[r] = make_xm_node<WPB,NPT>( p, tile, r );
         const double u0 = p.nu_min + id[r].iu*p.ndu + p.ndu/2;
         sh[r] = float( id[r].ix ); sl[r] = 0.0f;
         uh[r] = float( u0 );       ul[r] = float( u0 - double( uh[r] ) );
@@ -274,20 +399,19 @@ __device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint
     const float invNx = 1.0f / Nxf;
     const double cd   = p.dt * p.dx_inv;
     const float cd_hi = float( cd ), cd_lo = float( cd - double( cd_hi ) );
-    const float ck    = float( -0.5 * p.dt * p.dx_inv );
-    const uint32_t sd = p.stride_d;
+    const uint32_t sq = p.stride_q, q2 = p.q2_off, cells = p.Nx + 2;
+    poly_fetch<BLOCK,poly_maxc( BLOCK )> pf;
+    poly_prefetch_first( pf, p.dhist, p.stride_d, cells, int64_t( p.n ) - 1, p.chunk );
 
     for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
     {
         const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
         const int cnt = int( mhi - mlo + 1 );
-        __syncthreads();
-        stage<BLOCK>( ldsf, p.dhist32 + size_t( mlo ) * sd, uint32_t( cnt ) * sd );
-        __syncthreads();
+        poly_chunk_begin( pf, ldsf, p.dhist, p.stride_d, sq, q2, cells, p.kick, mhi, p.chunk );
         for ( int jj = cnt - 1; jj >= 0; --jj )
         {
-            const float *D = ldsf + jj * sd + 1;
-            const float kf = ( mlo + jj == 0 ) ? 0.5f * ck : ck;
+            const float *Q = ldsf + jj * sq;
+            const float hf = ( mlo + jj == 0 ) ? 0.5f : 1.0f;
             #pragma unroll
             for ( int r = 0; r < NPT; ++r )
             {
@@ -308,25 +432,25 @@ __device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint
                 const float k = floorf( sh[r] );
                 const float t = ( sh[r] - k ) + sl[r];
                 const int i = int( k );                   // in [-1, Nx]
-                const float D0 = D[i], D1 = D[i+1], D2 = D[i+2];
-                const float a = 1.0f - t;
-                const float r2 = fmaf( a*a, D0 - D1, fmaf( t*t, D2 - D1, D1 + D1 ) );
-                // kick: u += kf*r2 with TwoSum
+                const float2 q01 = reinterpret_cast<const float2*>( Q )[ i + 1 ];
+                const float kq = fmaf( fmaf( Q[ q2 + ( i + 1 ) ], t, q01.y ), t, q01.x );
+                // kick: u += h*kq with TwoSum
                 float y3, e3;
-                two_sum( uh[r], kf * r2, y3, e3 );
+                two_sum( uh[r], hf * kq, y3, e3 );
                 uh[r] = y3; ul[r] = ul[r] + e3;
             }
         }
     }
 
+    double f[NPT];
     #pragma unroll
     for ( int r = 0; r < NPT; ++r )
     {
         const double s = double( sh[r] ) + double( sl[r] );
         const double u = double( uh[r] ) + double( ul[r] );
-        const double f = id[r].active ? f0_1d( p.f0, p.x_min + s*p.dx, u ) : 0.0;
-        write_partial( p, id[r], f );
+        f[r] = id[r].active ? f0_1d( p.f0, p.x_min + s*p.dx, u ) : 0.0;
     }
+    xm_reduce<BLOCK,NPT>( p, tile, id, f, reinterpret_cast<double*>( ldsf ) );
 }
 
 template <int BLOCK, int NPT>
@@ -352,7 +476,8 @@ void trace1d_batch_kernel( const batch_run *__restrict__ runs, const uint32_t *_
     p.n = n;
     const uint32_t wg = blockIdx.x - runs[r].wg_begin;
     if ( VARIANT == TRACE_FP32C ) trace1d_fp32c_body<BLOCK,NPT>( p, wg, reinterpret_cast<float*>( lds_b ) );
-    else                          trace1d_fast_body<BLOCK,NPT,MODE_RHO>( p, wg, lds_b );
+    else if ( ( p.Nx & ( p.Nx - 1 ) ) == 0 ) trace1d_poly_body<BLOCK,NPT,true>( p, wg, lds_b );
+    else                          trace1d_poly_body<BLOCK,NPT,false>( p, wg, lds_b );
 }
 
 template <typename K>
@@ -376,7 +501,9 @@ hipError_t launch_fast_or_32( const trace1_params &p, int variant, uint32_t grid
         return launch_kernel( trace1d_fast_kernel<BLOCK,NPT,MODE_METRICS>, grid, BLOCK, lds, p, st );
     if ( variant == TRACE_FP32C )
         return launch_kernel( trace1d_fp32c_kernel<BLOCK,NPT>, grid, BLOCK, lds, p, st );
-    return launch_kernel( trace1d_fast_kernel<BLOCK,NPT,MODE_RHO>, grid, BLOCK, lds, p, st );
+    if ( ( p.Nx & ( p.Nx - 1 ) ) == 0 )
+        return launch_kernel( trace1d_poly_kernel<BLOCK,NPT,true>, grid, BLOCK, lds, p, st );
+    return launch_kernel( trace1d_poly_kernel<BLOCK,NPT,false>, grid, BLOCK, lds, p, st );
 }
 
 template <int BLOCK>
@@ -454,7 +581,8 @@ hipError_t launch_trace1d_batch( const batch_run *runs, const uint32_t *wg_run, 
     }
 }
 
-bool trace1d_shape_ok( int variant, int order, const launch_shape &s )
+// shapes with an instantiated kernel (velocity-major launches)
+static bool shape_built( int variant, int order, const launch_shape &s )
 {
     if ( variant == TRACE_STRICT )
         return order >= 3 && order <= 8 && s.npt == 1 &&
@@ -465,30 +593,86 @@ bool trace1d_shape_ok( int variant, int order, const launch_shape &s )
     return bok && nok && s.chunk >= 1;
 }
 
-// slices the backward loop visits: n (rho: n-1 .. 0) or n+1 (metrics: n .. 0)
-static uint64_t visited( const trace1_params &p )
+// x-major: the velocity rows of a workgroup are whole summation groups
+static bool shape_groups( const launch_shape &s )
 {
-    return p.mode == MODE_METRICS ? ( p.n > 0 ? p.n + 1 : 0 ) : p.n;
+    return ( uint32_t( s.block / WAVE ) * uint32_t( s.npt ) ) % XM_GROUP == 0;
+}
+
+// a shape for the rho kernel of the variant
+bool trace1d_shape_ok( int variant, int order, const launch_shape &s )
+{
+    return shape_built( variant, order, s ) && ( variant == TRACE_STRICT || shape_groups( s ) );
+}
+
+static bool x_major( const trace1_params &p, int variant )
+{
+    return p.mode == MODE_RHO && variant != TRACE_STRICT;
+}
+
+void trace1d_xm_setup( trace1_params &p )
+{
+    p.xw   = xm_xw( p.Nx );
+    p.vpw  = WAVE / p.xw;
+    p.n_xt = ( p.ix_count + p.xw - 1 ) / p.xw;
+    p.ppx  = xm_ppx( p.Nu );
+}
+
+uint32_t trace1d_chunk( const trace1_params &p, int variant, const launch_shape &s )
+{
+    uint64_t c = uint64_t( s.chunk );
+    if ( x_major( p, variant ) )
+    {
+        // a chunk's table entries fit the register prefetch of the workgroup
+        const uint64_t cap = uint64_t( s.block ) * uint64_t( poly_maxc( s.block ) ) / ( p.Nx + 2u );
+        c = std::min<uint64_t>( c, std::max<uint64_t>( cap, 1 ) );
+    }
+    const uint64_t v = p.mode == MODE_METRICS ? ( p.n > 0 ? p.n + 1 : 0 ) : p.n;
+    if ( v < c ) c = v > 0 ? v : 1;
+    return uint32_t( c );
+}
+
+uint32_t trace1d_grid( const trace1_params &p, int variant, const launch_shape &s )
+{
+    if ( x_major( p, variant ) )
+    {
+        const uint32_t vbw = uint32_t( s.block / WAVE ) * uint32_t( s.npt ) * p.vpw;
+        return p.n_xt * ( ( p.Nu + vbw - 1 ) / vbw );
+    }
+    const uint32_t tasks_per_block = uint32_t( s.block / WAVE ) * uint32_t( s.npt );
+    return ( p.n_tasks + tasks_per_block - 1 ) / tasks_per_block;
+}
+
+
+// LDS row of one slice: the kick-polynomial table for the fast and FP32C rho
+// kernels, the extended differences for the strict and metrics kernels
+size_t trace1d_slice_elems( const trace1_params &p, int variant )
+{
+    return ( p.mode == MODE_RHO && variant != TRACE_STRICT ) ? p.stride_q : p.stride_d;
 }
 
 size_t trace1d_lds_bytes( const trace1_params &p, int variant, const launch_shape &s )
 {
     const size_t elem = ( variant == TRACE_FP32C && p.mode == MODE_RHO ) ? sizeof(float) : sizeof(double);
-    const uint64_t v = visited( p );
-    const uint64_t c = v < uint64_t( s.chunk ) ? ( v > 0 ? v : 1 ) : uint64_t( s.chunk );
-    return size_t( c ) * p.stride_d * elem;
+    const size_t hist = size_t( trace1d_chunk( p, variant, s ) ) * trace1d_slice_elems( p, variant ) * elem;
+    // x-major: the f values of the workgroup (at most block*npt doubles) reuse the LDS
+    const size_t fbuf = x_major( p, variant ) ? size_t( s.block ) * size_t( s.npt ) * sizeof(double) : 0;
+    return hist > fbuf ? hist : fbuf;
 }
 
 hipError_t launch_trace1d( const trace1_params &p_in, int variant, const launch_shape &s, hipStream_t st )
 {
-    if ( !trace1d_shape_ok( variant, int( p_in.order ), s ) ) return hipErrorInvalidValue;
-    if ( p_in.n_tasks == 0 ) return hipSuccess;
+    if ( !shape_built( variant, int( p_in.order ), s ) ) return hipErrorInvalidValue;
     trace1_params p = p_in;
-    const uint32_t tasks_per_block = uint32_t( s.block / WAVE ) * uint32_t( s.npt );
-    const uint32_t grid = ( p.n_tasks + tasks_per_block - 1 ) / tasks_per_block;
-    p.chunk = uint32_t( s.chunk );
-    const uint64_t v = visited( p );
-    if ( v < p.chunk ) p.chunk = v > 0 ? uint32_t( v ) : 1u;
+    if ( x_major( p, variant ) )
+    {
+        if ( !shape_groups( s ) ) return hipErrorInvalidValue;
+        trace1d_xm_setup( p );
+        if ( p.ix_count == 0 ) return hipSuccess;
+    }
+    else if ( p.n_tasks == 0 ) return hipSuccess;
+    const uint32_t grid = trace1d_grid( p, variant, s );
+    p.chunk = trace1d_chunk( p, variant, s );
     const size_t lds = trace1d_lds_bytes( p, variant, s );
 
     if ( variant == TRACE_STRICT )

@@ -8,9 +8,13 @@
 //
 // The workgroup streams coefficient slices through LDS (one slice is
 // (Nx+3)(Ny+3)(Nz+3) reals: 6859 = 54.9 KB FP64 at 16^3) and every node
-// gathers its 4x4x4 window from LDS.  Rho partial sums: one compensated
-// butterfly per wave of 64 velocity nodes of one spatial point, summed in a
-// fixed order by the field kernel.
+// gathers its 4x4x4 window from LDS.  The fast and FP32C rho kernels lay a
+// wave's lanes over 64 neighbouring spatial nodes at one velocity (x-major,
+// below), so a wave's windows overlap; each lane sums f over a fixed group
+// of consecutive velocities.  The strict and metrics kernels keep the
+// reference's velocity-major order: a wave takes 64 velocity nodes of one
+// spatial point and reduces them with a compensated butterfly.  The partial
+// sums are added in a fixed order by the field kernel.
 //
 // Variants:
 //   strict : the reference's arithmetic (three separate tensor evaluations,
@@ -173,6 +177,45 @@ __device__ __forceinline__ int advance_axis( double &s, double vel, double cd, d
     return i;
 }
 
+// one fast FP64 characteristic step of one node with slice C: drift (unless
+// `still`), exact wrap, E = -grad phi from the shared-basis window, kick by hf*dt
+struct fast3_consts
+{
+    double Nxd, Nyd, Nzd, iNx, iNy, iNz, cdx, cdy, cdz, kx, ky, kz;
+    int Nxi, Nyi, Nzi;
+    uint32_t sy, sz;
+};
+
+__device__ __forceinline__ fast3_consts make_fast3( const trace3_params &p )
+{
+    fast3_consts k;
+    k.Nxd = p.Nx; k.Nyd = p.Ny; k.Nzd = p.Nz;
+    k.iNx = 1.0 / k.Nxd; k.iNy = 1.0 / k.Nyd; k.iNz = 1.0 / k.Nzd;
+    k.Nxi = int( p.Nx ); k.Nyi = int( p.Ny ); k.Nzi = int( p.Nz );
+    k.cdx = p.dt * p.dx_inv; k.cdy = p.dt * p.dy_inv; k.cdz = p.dt * p.dz_inv;
+    k.kx = -p.dt * p.dx_inv; k.ky = -p.dt * p.dy_inv; k.kz = -p.dt * p.dz_inv;
+    k.sy = p.sy; k.sz = p.sz;
+    return k;
+}
+
+__device__ __forceinline__ void step3_fast( const fast3_consts &k, const double *C, double hf, bool still,
+                                            double &sx, double &sy_, double &sz_, double &vu, double &vv, double &vw )
+{
+    double tx, ty, tz;
+    const int i = advance_axis( sx,  vu, k.cdx, k.Nxd, k.iNx, k.Nxi, tx, still );
+    const int j = advance_axis( sy_, vv, k.cdy, k.Nyd, k.iNy, k.Nyi, ty, still );
+    const int l = advance_axis( sz_, vw, k.cdz, k.Nzd, k.iNz, k.Nzi, tz, still );
+    double Bx[4], Dx[4], By[4], Dy[4], Bz[4], Dz[4];
+    cubic_bases( tx, Bx, Dx );
+    cubic_bases( ty, By, Dy );
+    cubic_bases( tz, Bz, Dz );
+    double gx, gy, gz;
+    grad_window( C + l*k.sz + j*k.sy + i, k.sy, k.sz, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
+    vu = fma( hf * k.kx, gx, vu );
+    vv = fma( hf * k.ky, gy, vv );
+    vw = fma( hf * k.kz, gz, vw );
+}
+
 template <int BLOCK, int NPT, int MODE>
 __global__ __launch_bounds__( BLOCK )
 void trace3d_fast_kernel( trace3_params p )
@@ -190,12 +233,8 @@ void trace3d_fast_kernel( trace3_params p )
         node_coords( p, id[r], ix, iy, iz, vu[r], vv[r], vw[r] );
         sx[r] = double( ix ); sy_[r] = double( iy ); sz_[r] = double( iz );
     }
-    const double Nxd = p.Nx, Nyd = p.Ny, Nzd = p.Nz;
-    const double iNx = 1.0 / Nxd, iNy = 1.0 / Nyd, iNz = 1.0 / Nzd;
-    const int Nxi = int( p.Nx ), Nyi = int( p.Ny ), Nzi = int( p.Nz );
-    const double cdx = p.dt * p.dx_inv, cdy = p.dt * p.dy_inv, cdz = p.dt * p.dz_inv;
-    const double kx = -p.dt * p.dx_inv, ky = -p.dt * p.dy_inv, kz = -p.dt * p.dz_inv;
-    const uint32_t st = p.st, sy = p.sy, sz = p.sz;
+    const fast3_consts k3 = make_fast3( p );
+    const uint32_t st = p.st;
 
     for ( int64_t mhi = top_slice3<MODE>( p.n ); mhi >= 0; mhi -= p.chunk )
     {
@@ -212,21 +251,7 @@ void trace3d_fast_kernel( trace3_params p )
             const double hf = ( m == 0 || initial ) ? 0.5 : 1.0;
             #pragma unroll
             for ( int r = 0; r < NPT; ++r )
-            {
-                double tx, ty, tz;
-                const int i = advance_axis( sx[r],  vu[r], cdx, Nxd, iNx, Nxi, tx, initial );
-                const int j = advance_axis( sy_[r], vv[r], cdy, Nyd, iNy, Nyi, ty, initial );
-                const int k = advance_axis( sz_[r], vw[r], cdz, Nzd, iNz, Nzi, tz, initial );
-                double Bx[4], Dx[4], By[4], Dy[4], Bz[4], Dz[4];
-                cubic_bases( tx, Bx, Dx );
-                cubic_bases( ty, By, Dy );
-                cubic_bases( tz, Bz, Dz );
-                double gx, gy, gz;
-                grad_window( C + k*sz + j*sy + i, sy, sz, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
-                vu[r] = fma( hf * kx, gx, vu[r] );
-                vv[r] = fma( hf * ky, gy, vv[r] );
-                vw[r] = fma( hf * kz, gz, vw[r] );
-            }
+                step3_fast( k3, C, hf, initial, sx[r], sy_[r], sz_[r], vu[r], vv[r], vw[r] );
         }
     }
 
@@ -247,9 +272,100 @@ void trace3d_fast_kernel( trace3_params p )
     }
 }
 
-// ---- FP32 compensated --------------------------------------------------------
-struct f2 { float h, l; };
+// ---- x-major layout (fast and FP32C rho kernels) ---------------------------
+// The 64 lanes of a wave take 64 consecutive spatial nodes l (x fastest) at
+// one velocity, so the 4x4x4 windows of a wave overlap and its LDS gathers
+// are near-contiguous.  Workgroup b covers spatial tile b % n_st (l = l_first
+// + 64*(b % n_st) + lane) and the velocity groups (b / n_st)*WPB + w of its
+// waves w; a group is vgroup consecutive velocities iv.  Each lane traces the
+// velocities of its group NPT at a time in iv order and sums f in a
+// compensated accumulator, written to partials[l*ppx + group]: the partial
+// sums do not depend on the launch shape.
+struct xm3_lane
+{
+    uint32_t l, g, ix, iy, iz;
+    bool ok;
+};
 
+template <int WPB>
+__device__ __forceinline__ xm3_lane make_xm3( const trace3_params &p )
+{
+    xm3_lane a;
+    const uint32_t lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    const uint32_t ll = ( blockIdx.x % p.n_st ) * WAVE + lane;
+    a.g  = ( blockIdx.x / p.n_st ) * WPB + wave;
+    a.ok = ll < p.n_count && a.g < p.ppx;
+    a.l  = p.l_first + ( ll < p.n_count ? ll : 0u );
+    a.ix = a.l % p.Nx; a.iy = ( a.l / p.Nx ) % p.Ny; a.iz = a.l / ( p.Nx * p.Ny );
+    return a;
+}
+
+// velocity of node iv of the group (cuda_kernel.cu:402-418) and whether it is traced
+__device__ __forceinline__ bool xm3_velocity( const trace3_params &p, const xm3_lane &a, uint32_t k,
+                                              double &u, double &v, double &w )
+{
+    const uint64_t nvel = uint64_t( p.Nu ) * p.Nv * p.Nw;
+    const uint64_t iv = uint64_t( a.g ) * p.vgroup + k;
+    const uint64_t q = uint64_t( a.l ) * nvel + iv;
+    const uint32_t ivc = iv < nvel ? uint32_t( iv ) : 0u;
+    const uint32_t iu = ivc % p.Nu, ivv = ( ivc / p.Nu ) % p.Nv, iw = ivc / ( p.Nu * p.Nv );
+    u = p.u_min + iu  * p.du + p.du / 2;
+    v = p.v_min + ivv * p.dv + p.dv / 2;
+    w = p.w_min + iw  * p.dw + p.dw / 2;
+    return a.ok && iv < nvel && q >= p.q_begin && q < p.q_end;
+}
+
+template <int BLOCK, int NPT>
+__global__ __launch_bounds__( BLOCK )
+void trace3d_xm_fast_kernel( trace3_params p )
+{
+    extern __shared__ double lds[];
+    constexpr int WPB = BLOCK / WAVE;
+    const xm3_lane a = make_xm3<WPB>( p );
+    const fast3_consts k3 = make_fast3( p );
+    const uint32_t st = p.st;
+    csum acc { 0.0, 0.0 };
+
+    for ( uint32_t pass = 0; pass < p.vgroup; pass += NPT )
+    {
+        double sx[NPT], sy_[NPT], sz_[NPT], vu[NPT], vv[NPT], vw[NPT];
+        bool act[NPT];
+        #pragma unroll
+        for ( int r = 0; r < NPT; ++r )
+        {
+            act[r] = xm3_velocity( p, a, pass + r, vu[r], vv[r], vw[r] );
+            sx[r] = double( a.ix ); sy_[r] = double( a.iy ); sz_[r] = double( a.iz );
+        }
+        for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
+        {
+            const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
+            const int cnt = int( mhi - mlo + 1 );
+            __syncthreads();
+            stage3<BLOCK>( lds, p.hist + size_t( mlo ) * st, uint32_t( cnt ) * st );
+            __syncthreads();
+            for ( int jj = cnt - 1; jj >= 0; --jj )
+            {
+                const double *C = lds + size_t( jj ) * st;
+                const double hf = ( mlo + jj == 0 ) ? 0.5 : 1.0;
+                #pragma unroll
+                for ( int r = 0; r < NPT; ++r )
+                    step3_fast( k3, C, hf, false, sx[r], sy_[r], sz_[r], vu[r], vv[r], vw[r] );
+            }
+        }
+        #pragma unroll
+        for ( int r = 0; r < NPT; ++r )
+        {
+            const double f = act[r]
+                ? f0_3d( p.f0, p.x_min + sx[r]*p.dx, p.y_min + sy_[r]*p.dy, p.z_min + sz_[r]*p.dz,
+                         vu[r], vv[r], vw[r] )
+                : 0.0;
+            acc = csum_add( acc, csum { f, 0.0 } );
+        }
+    }
+    if ( a.ok ) p.partials[ size_t( a.l ) * p.ppx + a.g ] = acc.s + acc.c;
+}
+
+// ---- FP32 compensated --------------------------------------------------------
 __device__ __forceinline__ int advance_axis32( float &sh, float &sl, float uh, float ul,
                                                float cdh, float cdl, float N, float invN, int Ni, float &t )
 {
@@ -281,27 +397,11 @@ __device__ __forceinline__ void kick32( float &uh, float &ul, float q )
 
 template <int BLOCK, int NPT>
 __global__ __launch_bounds__( BLOCK )
-void trace3d_fp32c_kernel( trace3_params p )
+void trace3d_xm_fp32c_kernel( trace3_params p )
 {
     extern __shared__ float ldsf[];
     constexpr int WPB = BLOCK / WAVE;
-
-    node3 id[NPT];
-    float xh[NPT], xl[NPT], yh[NPT], yl[NPT], zh[NPT], zl[NPT];
-    float uh[NPT], ul[NPT], vh[NPT], vl[NPT], wh[NPT], wl[NPT];
-    #pragma unroll
-    for ( int r = 0; r < NPT; ++r )
-    {
-        id[r] = make_node3<WPB,NPT>( p, r );
-        uint32_t ix, iy, iz; double u, v, w;
-        node_coords( p, id[r], ix, iy, iz, u, v, w );
-        xh[r] = float( ix ); xl[r] = 0.0f;
-        yh[r] = float( iy ); yl[r] = 0.0f;
-        zh[r] = float( iz ); zl[r] = 0.0f;
-        uh[r] = float( u ); ul[r] = float( u - double( uh[r] ) );
-        vh[r] = float( v ); vl[r] = float( v - double( vh[r] ) );
-        wh[r] = float( w ); wl[r] = float( w - double( wh[r] ) );
-    }
+    const xm3_lane a = make_xm3<WPB>( p );
     const float Nxf = p.Nx, Nyf = p.Ny, Nzf = p.Nz;
     const float iNx = 1.0f / Nxf, iNy = 1.0f / Nyf, iNz = 1.0f / Nzf;
     const int Nxi = int( p.Nx ), Nyi = int( p.Ny ), Nzi = int( p.Nz );
@@ -311,52 +411,71 @@ void trace3d_fp32c_kernel( trace3_params p )
     const float czh = float( cdz ), czl = float( cdz - double( czh ) );
     const float kx = float( -p.dt * p.dx_inv ), ky = float( -p.dt * p.dy_inv ), kz = float( -p.dt * p.dz_inv );
     const uint32_t st = p.st, sy = p.sy, sz = p.sz;
+    csum acc { 0.0, 0.0 };
 
-    for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
+    for ( uint32_t pass = 0; pass < p.vgroup; pass += NPT )
     {
-        const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
-        const int cnt = int( mhi - mlo + 1 );
-        __syncthreads();
-        stage3<BLOCK>( ldsf, p.hist32 + size_t( mlo ) * st, uint32_t( cnt ) * st );
-        __syncthreads();
-        for ( int jj = cnt - 1; jj >= 0; --jj )
+        float xh[NPT], xl[NPT], yh[NPT], yl[NPT], zh[NPT], zl[NPT];
+        float uh[NPT], ul[NPT], vh[NPT], vl[NPT], wh[NPT], wl[NPT];
+        bool act[NPT];
+        #pragma unroll
+        for ( int r = 0; r < NPT; ++r )
         {
-            const float *C = ldsf + size_t( jj ) * st;
-            const float hf = ( mlo + jj == 0 ) ? 0.5f : 1.0f;
-            #pragma unroll
-            for ( int r = 0; r < NPT; ++r )
+            double u, v, w;
+            act[r] = xm3_velocity( p, a, pass + r, u, v, w );
+            xh[r] = float( a.ix ); xl[r] = 0.0f;
+            yh[r] = float( a.iy ); yl[r] = 0.0f;
+            zh[r] = float( a.iz ); zl[r] = 0.0f;
+            uh[r] = float( u ); ul[r] = float( u - double( uh[r] ) );
+            vh[r] = float( v ); vl[r] = float( v - double( vh[r] ) );
+            wh[r] = float( w ); wl[r] = float( w - double( wh[r] ) );
+        }
+        for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
+        {
+            const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
+            const int cnt = int( mhi - mlo + 1 );
+            __syncthreads();
+            stage3<BLOCK>( ldsf, p.hist32 + size_t( mlo ) * st, uint32_t( cnt ) * st );
+            __syncthreads();
+            for ( int jj = cnt - 1; jj >= 0; --jj )
             {
-                float tx, ty, tz;
-                const int i = advance_axis32( xh[r], xl[r], uh[r], ul[r], cxh, cxl, Nxf, iNx, Nxi, tx );
-                const int j = advance_axis32( yh[r], yl[r], vh[r], vl[r], cyh, cyl, Nyf, iNy, Nyi, ty );
-                const int k = advance_axis32( zh[r], zl[r], wh[r], wl[r], czh, czl, Nzf, iNz, Nzi, tz );
-                float Bx[4], Dx[4], By[4], Dy[4], Bz[4], Dz[4];
-                cubic_bases( tx, Bx, Dx );
-                cubic_bases( ty, By, Dy );
-                cubic_bases( tz, Bz, Dz );
-                float gx, gy, gz;
-                grad_window( C + k*sz + j*sy + i, sy, sz, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
-                kick32( uh[r], ul[r], ( hf * kx ) * gx );
-                kick32( vh[r], vl[r], ( hf * ky ) * gy );
-                kick32( wh[r], wl[r], ( hf * kz ) * gz );
+                const float *C = ldsf + size_t( jj ) * st;
+                const float hf = ( mlo + jj == 0 ) ? 0.5f : 1.0f;
+                #pragma unroll
+                for ( int r = 0; r < NPT; ++r )
+                {
+                    float tx, ty, tz;
+                    const int i = advance_axis32( xh[r], xl[r], uh[r], ul[r], cxh, cxl, Nxf, iNx, Nxi, tx );
+                    const int j = advance_axis32( yh[r], yl[r], vh[r], vl[r], cyh, cyl, Nyf, iNy, Nyi, ty );
+                    const int k = advance_axis32( zh[r], zl[r], wh[r], wl[r], czh, czl, Nzf, iNz, Nzi, tz );
+                    float Bx[4], Dx[4], By[4], Dy[4], Bz[4], Dz[4];
+                    cubic_bases( tx, Bx, Dx );
+                    cubic_bases( ty, By, Dy );
+                    cubic_bases( tz, Bz, Dz );
+                    float gx, gy, gz;
+                    grad_window( C + k*sz + j*sy + i, sy, sz, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
+                    kick32( uh[r], ul[r], ( hf * kx ) * gx );
+                    kick32( vh[r], vl[r], ( hf * ky ) * gy );
+                    kick32( wh[r], wl[r], ( hf * kz ) * gz );
+                }
             }
         }
-    }
-
-    #pragma unroll
-    for ( int r = 0; r < NPT; ++r )
-    {
-        double f = 0.0;
-        if ( id[r].active )
+        #pragma unroll
+        for ( int r = 0; r < NPT; ++r )
         {
-            const double x = p.x_min + ( double( xh[r] ) + double( xl[r] ) ) * p.dx;
-            const double y = p.y_min + ( double( yh[r] ) + double( yl[r] ) ) * p.dy;
-            const double z = p.z_min + ( double( zh[r] ) + double( zl[r] ) ) * p.dz;
-            f = f0_3d( p.f0, x, y, z, double( uh[r] ) + double( ul[r] ), double( vh[r] ) + double( vl[r] ),
-                       double( wh[r] ) + double( wl[r] ) );
+            double f = 0.0;
+            if ( act[r] )
+            {
+                const double x = p.x_min + ( double( xh[r] ) + double( xl[r] ) ) * p.dx;
+                const double y = p.y_min + ( double( yh[r] ) + double( yl[r] ) ) * p.dy;
+                const double z = p.z_min + ( double( zh[r] ) + double( zl[r] ) ) * p.dz;
+                f = f0_3d( p.f0, x, y, z, double( uh[r] ) + double( ul[r] ), double( vh[r] ) + double( vl[r] ),
+                           double( wh[r] ) + double( wl[r] ) );
+            }
+            acc = csum_add( acc, csum { f, 0.0 } );
         }
-        write_partial3( p, id[r], f );
     }
+    if ( a.ok ) p.partials[ size_t( a.l ) * p.ppx + a.g ] = acc.s + acc.c;
 }
 
 // ---- strict FP64 (the reference's arithmetic, order 4) ----------------------
@@ -527,17 +646,17 @@ hipError_t launch3_npt( const trace3_params &p, int variant, int npt, uint32_t g
     {
         switch ( npt )
         {
-        case 1: return launch3( trace3d_fp32c_kernel<BLOCK,1>, grid, BLOCK, lds, p, st );
-        case 2: return launch3( trace3d_fp32c_kernel<BLOCK,2>, grid, BLOCK, lds, p, st );
-        case 4: return launch3( trace3d_fp32c_kernel<BLOCK,4>, grid, BLOCK, lds, p, st );
+        case 1: return launch3( trace3d_xm_fp32c_kernel<BLOCK,1>, grid, BLOCK, lds, p, st );
+        case 2: return launch3( trace3d_xm_fp32c_kernel<BLOCK,2>, grid, BLOCK, lds, p, st );
+        case 4: return launch3( trace3d_xm_fp32c_kernel<BLOCK,4>, grid, BLOCK, lds, p, st );
         default: return hipErrorInvalidValue;
         }
     }
     switch ( npt )
     {
-    case 1: return launch3( trace3d_fast_kernel<BLOCK,1,MODE_RHO>, grid, BLOCK, lds, p, st );
-    case 2: return launch3( trace3d_fast_kernel<BLOCK,2,MODE_RHO>, grid, BLOCK, lds, p, st );
-    case 4: return launch3( trace3d_fast_kernel<BLOCK,4,MODE_RHO>, grid, BLOCK, lds, p, st );
+    case 1: return launch3( trace3d_xm_fast_kernel<BLOCK,1>, grid, BLOCK, lds, p, st );
+    case 2: return launch3( trace3d_xm_fast_kernel<BLOCK,2>, grid, BLOCK, lds, p, st );
+    case 4: return launch3( trace3d_xm_fast_kernel<BLOCK,4>, grid, BLOCK, lds, p, st );
     default: return hipErrorInvalidValue;
     }
 }
@@ -569,10 +688,23 @@ size_t trace3d_lds_bytes( const trace3_params &p, int variant, const launch_shap
 hipError_t launch_trace3d( const trace3_params &p_in, int variant, const launch_shape &s, hipStream_t st )
 {
     if ( !trace3d_shape_ok( variant, int( p_in.order ), s ) ) return hipErrorInvalidValue;
-    if ( p_in.n_tasks == 0 ) return hipSuccess;
     trace3_params p = p_in;
-    const uint32_t tasks_per_block = uint32_t( s.block / WAVE ) * uint32_t( s.npt );
-    const uint32_t grid = ( p.n_tasks + tasks_per_block - 1 ) / tasks_per_block;
+    uint32_t grid;
+    if ( p.mode == MODE_RHO && variant != TRACE_STRICT )
+    {
+        // x-major: spatial tiles x velocity-group blocks; NPT divides vgroup
+        if ( p.n_count == 0 ) return hipSuccess;
+        if ( p.vgroup == 0 || p.vgroup % uint32_t( s.npt ) != 0 ) return hipErrorInvalidValue;
+        const uint32_t wpb = uint32_t( s.block / WAVE );
+        p.n_st = ( p.n_count + WAVE - 1 ) / WAVE;
+        grid = p.n_st * ( ( p.ppx + wpb - 1 ) / wpb );
+    }
+    else
+    {
+        if ( p.n_tasks == 0 ) return hipSuccess;
+        const uint32_t tasks_per_block = uint32_t( s.block / WAVE ) * uint32_t( s.npt );
+        grid = ( p.n_tasks + tasks_per_block - 1 ) / tasks_per_block;
+    }
     p.chunk = uint32_t( s.chunk );
     const uint64_t v = visited3( p );
     if ( v < p.chunk ) p.chunk = v > 0 ? uint32_t( v ) : 1u;

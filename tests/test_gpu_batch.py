@@ -50,7 +50,7 @@ def test_batch_shapes_and_graph_replay_are_bit_stable():
         ref = [b.history(r) for r in range(len(runs))]
         shape, ms = b.autotune(n_probe=20, repeats=1)
         print(f"[batch] autotuned shape {shape} ({ms:.3f} ms)", flush=True)
-        for sh in [(256, 2, 8), shape]:
+        for sh in [(256, 4, 8), shape]:
             b.set_launch_shape(*sh)
             b.run(graph=True)         # recaptured for the new shape
             b.run(graph=True)         # replayed

@@ -178,3 +178,28 @@ def test_strict_other_orders(order):
     st = conf.stride_t(order)
     assert rel(h, hist[: (conf.Nt + 1) * st]) < 1e-11
     assert np.max(np.abs(e - e_ref) / e_ref) < 1e-11
+
+
+@pytest.mark.parametrize("conf_args", [dict(Nx=64, Nu=256, T=1.0), dict(Nx=48, Nu=200, T=1.0)])
+@pytest.mark.parametrize("prec,variant", [(nufi.FP64, _capi.TRACE_FAST), (nufi.FP32C, _capi.TRACE_FP32C)])
+def test_rho_bit_identical_across_launch_shapes(conf_args, prec, variant):
+    """The fast and FP32C kernels sum f over fixed groups of 16 consecutive
+    velocities and re-wrap positions on a schedule fixed by the slice index,
+    so rho does not depend on the workgroup size, nodes per thread or chunk."""
+    conf = nufi.twostream_1d(**conf_args)
+    hist, _ = oracle_history(conf, 14)
+    st = conf.stride_t()
+    out = []
+    with nufi.GpuScheduler(conf, precision=prec) as s:
+        s.set_trace_variant(variant)
+        s.upload_history(hist[: 14 * st], 0, 14)
+        for shape in [(1024, 1, 4), (1024, 4, 16), (512, 2, 3), (256, 4, 64), (1024, 2, 1)]:
+            s.set_launch_shape(*shape)
+            rho = np.zeros(conf.Nx)
+            s.compute_rho(13)
+            s.download_rho(rho)
+            out.append(rho)
+        with pytest.raises(nufi.NufiError):
+            s.set_launch_shape(256, 1, 8)          # 4 velocity rows: not whole groups of 16
+    for r in out[1:]:
+        assert np.array_equal(r, out[0])

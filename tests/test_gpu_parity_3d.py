@@ -114,3 +114,23 @@ def test_q_subranges_3d():
             s.compute_rho(4, a, b)
             s.download_rho(acc)
     assert rel(acc, full) < 1e-13
+
+
+@pytest.mark.parametrize("prec", [nufi.FP64, nufi.FP32C])
+def test_rho_bit_identical_across_launch_shapes_3d(prec):
+    """Each lane of the x-major 3D kernels sums its fixed velocity group in iv
+    order, so rho does not depend on the launch shape."""
+    conf = small_3d()
+    hist, _ = oracle_history(conf, 6)
+    st = conf.stride_t()
+    out = []
+    with nufi.GpuScheduler(conf, precision=prec) as s:
+        s.upload_history(hist[: 5 * st], 0, 5)
+        for shape in [(256, 1, 1), (512, 4, 2), (1024, 2, 1)]:
+            s.set_launch_shape(*shape)
+            rho = np.zeros(conf.n_space())
+            s.compute_rho(4)
+            s.download_rho(rho)
+            out.append(rho)
+    for r in out[1:]:
+        assert np.array_equal(r, out[0])
