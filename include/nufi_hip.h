@@ -46,7 +46,8 @@ enum nufi_status {
                                  (std::range_error in the reference)       */
     NUFI_ERR_DEVICE      = 3, /* HIP runtime failure (cuda::exception)     */
     NUFI_ERR_ALLOC       = 4, /* device allocation failure                 */
-    NUFI_ERR_UNSUPPORTED = 5  /* order / dim / precision not built         */
+    NUFI_ERR_UNSUPPORTED = 5, /* order / dim / precision not built         */
+    NUFI_ERR_IO          = 6  /* history file: open, read, write or format */
 };
 
 /* ---- precision of the characteristic trace ----------------------------- */
@@ -186,7 +187,11 @@ enum nufi_trace_variant {
 int nufi_set_trace_variant(nufi_ctx *ctx, int variant);
 /* Workgroup size, nodes per thread and history slices per LDS chunk of the
  * trace kernel (what the autotuner sets); NUFI_ERR_UNSUPPORTED if the shape
- * is not built or does not fit the LDS. */
+ * is not built or does not fit the LDS.  1D FAST and FP32C kernels take
+ * (block/64)*nodes_per_thread a multiple of 16 (whole velocity groups per
+ * workgroup) and cap the chunk at what the workgroup's register prefetch of
+ * the table holds.  The FAST and FP32C results do not depend on the shape
+ * (fixed summation groups), so tuning never changes a result. */
 int nufi_set_launch_shape(nufi_ctx *ctx, int block, int nodes_per_thread, int chunk);
 int nufi_get_launch_shape(nufi_ctx *ctx, int *block, int *nodes_per_thread, int *chunk);
 /* Autotuner (no counterpart in the reference, whose launch shape is fixed at
@@ -245,6 +250,78 @@ int nufi_batch_get_launch_shape(nufi_batch *b, int *block, int *nodes_per_thread
 /* Autotuner over the batch trace launch at step n_probe (all runs). */
 int nufi_batch_autotune(nufi_batch *b, uint64_t n_probe, int repeats, int *block, int *nodes_per_thread,
                         int *chunk, double *best_ms);
+
+/* ---- diagnostics samplers (1D) ---------------------------------------- */
+/*
+ * The plotting block of bin/test_nufi_gpu_1d.cpp:290-361 on the device, at
+ * `count` arbitrary points (host arrays in and out), from the device history:
+ *   nufi_sample_field  E = -dphi/dx and rho = -d^2phi/dx^2 of slice n
+ *                      (dim1::eval<order,1|2>, fields.hpp:36-61)
+ *   nufi_sample_f      f = eval_f(n, x, v) (rho.hpp:63-96: the trace with the
+ *                      initial half-kick from slice n; slices 0..n are read)
+ *   nufi_phase_flow    eval_phase_flow(n, x, v) (rho.hpp:98-131): the foot
+ *                      point of that trace for n > 1 — for n <= 1 the point
+ *                      is not moved, as in the reference — with x wrapped by
+ *                      x - Lx floor(x / Lx)
+ * The reference's operation order (no FMA contraction), any built order.
+ * NUFI_ERR_RANGE for n > Nt, NUFI_ERR_UNSUPPORTED for 3D contexts.
+ */
+int nufi_sample_field(nufi_ctx *ctx, uint64_t n, const double *x, uint64_t count, double *E, double *rho);
+int nufi_sample_f(nufi_ctx *ctx, uint64_t n, const double *x, const double *v, uint64_t count, double *f);
+int nufi_phase_flow(nufi_ctx *ctx, uint64_t n, const double *x, const double *v, uint64_t count,
+                    double *x_out, double *v_out);
+
+/* ---- history files: checkpoint / resume and isolated steps ------------- */
+/*
+ * The coefficient history (slices 0..n_slices-1, each stride_t reals, the
+ * layout of the device history) on disk.  Two formats:
+ *   NUFI_HISTORY_BINARY  lossless: the 8 bytes "NUFIHST1", a fixed header
+ *                        (little-endian: u32 dim, u32 order, u64 Nx, Ny, Nz,
+ *                        Nt, n_slices, stride_t, f64 dt, u64 n_energies), the
+ *                        slices as f64, then n_energies f64 energies.
+ *   NUFI_HISTORY_TEXT    the layout of the reference's potential files
+ *                        (bin/test_nufi_cpu_3d_isolated.cpp:128-135,
+ *                        158-162, 190-211): the lines "Nt = ", "dt = ",
+ *                        "Nx = ", "Ny = ", "Nz = ", "order = ", a blank line,
+ *                        then one value per line — written here with 17
+ *                        significant digits, so it reads back exactly; files
+ *                        the reference wrote (16 digits) read to within one
+ *                        unit in the 16th digit.  No energies.
+ * The file functions are host-only (no device is touched).
+ */
+enum nufi_history_format {
+    NUFI_HISTORY_BINARY = 0,
+    NUFI_HISTORY_TEXT   = 1
+};
+typedef struct nufi_history_info {
+    int      dim, order;
+    uint64_t Nx, Ny, Nz;       /* Ny = Nz = 1 in 1D                         */
+    uint64_t Nt;               /* the run's last step                        */
+    uint64_t n_slices;         /* slices stored: 0..n_slices-1               */
+    uint64_t stride_t;         /* reals per slice: prod (N_d + order - 1)    */
+    double   dt;
+    uint64_t n_energies;       /* energies stored (binary only)             */
+} nufi_history_info;
+/* Write n_slices slices (and info->n_energies energies, binary only; may be
+ * NULL when n_energies is 0). */
+int nufi_history_write(const char *path, int format, const nufi_history_info *info,
+                       const double *slices, const double *energies);
+/* Read the header (format detected from the first bytes). */
+int nufi_history_read_info(const char *path, nufi_history_info *info, int *format);
+/* Read the header and up to slice_capacity slices / energy_capacity energies. */
+int nufi_history_read(const char *path, nufi_history_info *info, double *slices, uint64_t slice_capacity,
+                      double *energies, uint64_t energy_capacity);
+/* Checkpoint: slices 0..n_slices-1 of the context's device history (and the
+ * energies of those steps, binary) to a file. */
+int nufi_save_history(nufi_ctx *ctx, const char *path, int format, uint64_t n_slices);
+/* Resume / isolated steps: load a file written for the same grid (dim,
+ * order, Nx, Ny, Nz, dt; file Nt <= context Nt) into the device history (the
+ * tables of every slice are rebuilt, energies restored when stored).
+ * *n_slices receives the slice count: nufi_run(ctx, *n_slices, Nt + 1, ...)
+ * continues the run, nufi_step(ctx, n, ...) with n < *n_slices recomputes an
+ * isolated step from the stored slices 0..n-1
+ * (bin/test_nufi_cpu_3d_isolated.cpp:169-228). */
+int nufi_load_history(nufi_ctx *ctx, const char *path, uint64_t *n_slices);
 
 #ifdef __cplusplus
 }

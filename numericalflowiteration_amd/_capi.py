@@ -20,13 +20,26 @@ ERR_RANGE = 2
 ERR_DEVICE = 3
 ERR_ALLOC = 4
 ERR_UNSUPPORTED = 5
+ERR_IO = 6
+
+HISTORY_BINARY = 0
+HISTORY_TEXT = 1
 
 TRACE_FAST = 0
 TRACE_STRICT = 1
 TRACE_FP32C = 2
 
+class HistoryInfo(ctypes.Structure):
+    """nufi_history_info of include/nufi_hip.h."""
+    _fields_ = [("dim", ctypes.c_int), ("order", ctypes.c_int),
+                ("Nx", ctypes.c_uint64), ("Ny", ctypes.c_uint64), ("Nz", ctypes.c_uint64),
+                ("Nt", ctypes.c_uint64), ("n_slices", ctypes.c_uint64), ("stride_t", ctypes.c_uint64),
+                ("dt", ctypes.c_double), ("n_energies", ctypes.c_uint64)]
+
+
 _dp = ctypes.POINTER(ctypes.c_double)
 _cp = ctypes.POINTER(Config)
+_hp = ctypes.POINTER(HistoryInfo)
 _vp = ctypes.c_void_p
 _u64 = ctypes.c_uint64
 _int = ctypes.c_int
@@ -72,6 +85,14 @@ SIGNATURES = {
     "nufi_batch_get_launch_shape": (_int, [_vp, ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_int)]),
     "nufi_batch_autotune": (_int, [_vp, _u64, _int, ctypes.POINTER(_int), ctypes.POINTER(_int),
                                    ctypes.POINTER(_int), _dp]),
+    "nufi_sample_field": (_int, [_vp, _u64, _dp, _u64, _dp, _dp]),
+    "nufi_sample_f": (_int, [_vp, _u64, _dp, _dp, _u64, _dp]),
+    "nufi_phase_flow": (_int, [_vp, _u64, _dp, _dp, _u64, _dp, _dp]),
+    "nufi_history_write": (_int, [ctypes.c_char_p, _int, _hp, _dp, _dp]),
+    "nufi_history_read_info": (_int, [ctypes.c_char_p, _hp, ctypes.POINTER(_int)]),
+    "nufi_history_read": (_int, [ctypes.c_char_p, _hp, _dp, _u64, _dp, _u64]),
+    "nufi_save_history": (_int, [_vp, ctypes.c_char_p, _int, _u64]),
+    "nufi_load_history": (_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_u64)]),
 }
 
 
@@ -85,6 +106,10 @@ class NufiError(RuntimeError):
 
 class NufiRangeError(NufiError, IndexError):
     """n > Nt or an out-of-range node range (std::range_error in the reference)."""
+
+
+class NufiIOError(NufiError, OSError):
+    """A history file that cannot be opened, read, written or parsed."""
 
 
 _lib = None
@@ -119,6 +144,8 @@ def check(rc: int, ctx=None, batch: bool = False):
     msg = (lib.nufi_batch_last_error(ctx) if batch else lib.nufi_last_error(ctx)).decode(errors="replace")
     if rc == ERR_RANGE:
         raise NufiRangeError(rc, msg)
+    if rc == ERR_IO:
+        raise NufiIOError(rc, msg)
     raise NufiError(rc, msg)
 
 

@@ -55,6 +55,8 @@ struct nufi_ctx
     double2 *tw_x = nullptr, *tw_y = nullptr, *tw_z = nullptr;
     double2 *mult_step = nullptr, *mult_pois = nullptr, *mult_fit = nullptr;
     double *ewt = nullptr;
+    double *samp = nullptr;    // sampler points and outputs: 4 x samp_cap reals (grown on demand)
+    uint64_t samp_cap = 0;
 
     std::vector<double> host_rho;
     std::string err;
@@ -109,7 +111,7 @@ void free_all( nufi_ctx *c )
     void *ptrs[] = { c->hist, c->dhist, c->hist32, c->partials, c->rho, c->energies,
                      c->values, c->scratch_slice, c->metrics, c->mpartials, c->energy_tmp,
                      c->rho_hist, c->lambda, c->gvec, c->grad_part, c->grad, c->tw_x, c->tw_y, c->tw_z,
-                     c->mult_step, c->mult_pois, c->mult_fit, c->ewt };
+                     c->mult_step, c->mult_pois, c->mult_fit, c->ewt, c->samp };
     for ( void *p : ptrs ) if ( p ) (void)hipFree( p );
     for ( auto &pr : c->ev ) { (void)hipEventDestroy( pr.first ); (void)hipEventDestroy( pr.second ); }
     c->ev.clear();
@@ -743,6 +745,115 @@ int nufi_download_energies( nufi_ctx *c, uint64_t n_begin, uint64_t n_end, doubl
     HIP_TRY( c, hipMemcpyAsync( out, c->energies + n_begin, ( n_end - n_begin ) * sizeof(double),
                                 hipMemcpyDeviceToHost, c->stream ) );
     HIP_TRY( c, hipStreamSynchronize( c->stream ) );
+    return NUFI_OK;
+}
+
+// ---- diagnostics samplers (bin/test_nufi_gpu_1d.cpp:290-361) ---------------
+namespace
+{
+
+int sample_1d( nufi_ctx *c, int what, uint64_t n, const double *x, const double *v, uint64_t count,
+               double *o0, double *o1 )
+{
+    if ( c == nullptr ) return NUFI_ERR_INVALID;
+    if ( c->dim != 1 ) return fail( c, NUFI_ERR_UNSUPPORTED, "samplers: 1D contexts only" );
+    if ( n > c->conf.Nt ) return fail( c, NUFI_ERR_RANGE, "samplers: time step out of range" );
+    if ( count == 0 ) return NUFI_OK;
+    if ( x == nullptr || o0 == nullptr || ( what != SAMPLE_FIELD && v == nullptr ) ||
+         ( what != SAMPLE_F && o1 == nullptr ) )
+        return NUFI_ERR_INVALID;
+    HIP_TRY( c, hipSetDevice( c->device ) );
+    if ( c->samp_cap < count )
+    {
+        HIP_TRY( c, hipStreamSynchronize( c->stream ) );
+        if ( c->samp ) HIP_TRY( c, hipFree( c->samp ) );
+        c->samp = nullptr; c->samp_cap = 0;
+        if ( int rc = dev_alloc( c, &c->samp, 4 * count, "sampler buffers" ) ) return rc;
+        c->samp_cap = count;
+    }
+    double *dx = c->samp, *dv = c->samp + count, *d0 = c->samp + 2 * count, *d1 = c->samp + 3 * count;
+    const size_t bytes = count * sizeof(double);
+    HIP_TRY( c, hipMemcpyAsync( dx, x, bytes, hipMemcpyHostToDevice, c->stream ) );
+    if ( what != SAMPLE_FIELD ) HIP_TRY( c, hipMemcpyAsync( dv, v, bytes, hipMemcpyHostToDevice, c->stream ) );
+    sample1_params p {};
+    p.dhist = c->dhist; p.x = dx; p.v = dv; p.out0 = d0; p.out1 = d1;
+    p.n = n; p.count = count; p.stride_d = c->sd;
+    p.x_min = c->conf.x_min; p.Lx = c->conf.Lx; p.Lx_inv = c->conf.Lx_inv; p.dx_inv = c->conf.dx_inv;
+    p.dt = c->conf.dt; p.f0 = c->conf.f0;
+    hipError_t e = launch_sample1d( p, what, c->order, c->stream );
+    if ( e != hipSuccess ) return hip_fail( c, e, "sampler launch" );
+    HIP_TRY( c, hipMemcpyAsync( o0, d0, bytes, hipMemcpyDeviceToHost, c->stream ) );
+    if ( what != SAMPLE_F ) HIP_TRY( c, hipMemcpyAsync( o1, d1, bytes, hipMemcpyDeviceToHost, c->stream ) );
+    HIP_TRY( c, hipStreamSynchronize( c->stream ) );
+    return NUFI_OK;
+}
+
+} // namespace
+
+int nufi_sample_field( nufi_ctx *c, uint64_t n, const double *x, uint64_t count, double *E, double *rho )
+{
+    return sample_1d( c, SAMPLE_FIELD, n, x, nullptr, count, E, rho );
+}
+
+int nufi_sample_f( nufi_ctx *c, uint64_t n, const double *x, const double *v, uint64_t count, double *f )
+{
+    return sample_1d( c, SAMPLE_F, n, x, v, count, f, nullptr );
+}
+
+int nufi_phase_flow( nufi_ctx *c, uint64_t n, const double *x, const double *v, uint64_t count,
+                     double *x_out, double *v_out )
+{
+    return sample_1d( c, SAMPLE_FLOW, n, x, v, count, x_out, v_out );
+}
+
+// ---- history files ---------------------------------------------------------
+int nufi_save_history( nufi_ctx *c, const char *path, int format, uint64_t n_slices )
+{
+    if ( c == nullptr || path == nullptr ) return NUFI_ERR_INVALID;
+    if ( n_slices > c->conf.Nt + 1 ) return fail( c, NUFI_ERR_RANGE, "save_history: more slices than the history holds" );
+    std::vector<double> h( n_slices * c->st ), e( n_slices );
+    int rc = nufi_download_history( c, 0, n_slices, h.data() );
+    if ( rc == NUFI_OK ) rc = nufi_download_energies( c, 0, n_slices, e.data() );
+    if ( rc ) return rc;
+    nufi_history_info info {};
+    info.dim = c->dim; info.order = c->order;
+    info.Nx = c->conf.Nx;
+    info.Ny = c->dim == 3 ? c->conf.Ny : 1;
+    info.Nz = c->dim == 3 ? c->conf.Nz : 1;
+    info.Nt = c->conf.Nt; info.n_slices = n_slices; info.stride_t = c->st; info.dt = c->conf.dt;
+    info.n_energies = format == NUFI_HISTORY_BINARY ? n_slices : 0;
+    rc = nufi_history_write( path, format, &info, h.data(), e.data() );
+    if ( rc ) return fail( c, rc, std::string( "save_history: cannot write " ) + path );
+    return NUFI_OK;
+}
+
+int nufi_load_history( nufi_ctx *c, const char *path, uint64_t *n_slices )
+{
+    if ( c == nullptr || path == nullptr ) return NUFI_ERR_INVALID;
+    nufi_history_info info {};
+    int fmt = 0;
+    int rc = nufi_history_read_info( path, &info, &fmt );
+    if ( rc ) return fail( c, rc, std::string( "load_history: cannot read " ) + path );
+    const bool same = info.dim == c->dim && info.order == c->order && info.Nx == c->conf.Nx &&
+                      info.Ny == ( c->dim == 3 ? c->conf.Ny : 1 ) && info.Nz == ( c->dim == 3 ? c->conf.Nz : 1 ) &&
+                      info.stride_t == c->st;
+    // the text header carries dt with six decimals (std::to_string)
+    const bool dt_ok = fmt == NUFI_HISTORY_BINARY ? info.dt == c->conf.dt
+                                                  : std::fabs( info.dt - c->conf.dt ) <= 5e-7;
+    if ( !same || !dt_ok ) return fail( c, NUFI_ERR_INVALID, "load_history: the file is for another grid" );
+    if ( info.Nt > c->conf.Nt ) return fail( c, NUFI_ERR_RANGE, "load_history: the file's run is longer than Nt" );
+    std::vector<double> h( info.n_slices * c->st ), e( info.n_energies );
+    rc = nufi_history_read( path, &info, h.data(), info.n_slices, e.data(), info.n_energies );
+    if ( rc ) return fail( c, rc, std::string( "load_history: cannot read " ) + path );
+    rc = nufi_upload_history( c, 0, info.n_slices, h.data() );
+    if ( rc ) return rc;
+    if ( info.n_energies )
+    {
+        HIP_TRY( c, hipMemcpyAsync( c->energies, e.data(), std::min( info.n_energies, info.n_slices ) * sizeof(double),
+                                    hipMemcpyHostToDevice, c->stream ) );
+        HIP_TRY( c, hipStreamSynchronize( c->stream ) );
+    }
+    if ( n_slices ) *n_slices = info.n_slices;
     return NUFI_OK;
 }
 

@@ -173,6 +173,26 @@ size_t adjoint1d_lds_bytes( const adjoint1_params &p );
 hipError_t launch_adjoint_field( const adjoint_field_params &p, hipStream_t st );
 hipError_t launch_reduce_grad( const double *part, uint32_t n_tasks, uint32_t npar, double *grad, hipStream_t st );
 
+// ---- diagnostics samplers (sample1d.hip) -------------------------------------
+enum sample_what : int
+{
+    SAMPLE_FIELD = 0, // out0 = E = -phi'(x), out1 = -phi''(x) of slice n
+    SAMPLE_F     = 1, // out0 = eval_f(n, x, v) (rho.hpp:63-96)
+    SAMPLE_FLOW  = 2  // (out0, out1) = eval_phase_flow(n, x, v) (rho.hpp:98-131)
+};
+
+struct sample1_params
+{
+    const double *dhist;     // extended differences, slice m at dhist + m*stride_d
+    const double *x, *v;     // the points (device)
+    double *out0, *out1;
+    uint64_t n, count;
+    uint32_t stride_d;
+    double x_min, Lx, Lx_inv, dx_inv, dt;
+    nufi_f0 f0;
+};
+hipError_t launch_sample1d( const sample1_params &p, int what, int order, hipStream_t st );
+
 // ---- field solve (field.hip) -------------------------------------------------
 // Input rho either from per-wave partials (sum_j partials[i*wpx + j], times
 // -dv), or from a plain array.  Output: the padded coefficient slice, its

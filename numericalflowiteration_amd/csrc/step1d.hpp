@@ -58,6 +58,15 @@ __device__ __forceinline__ bool poly_wrap_due( int64_t m )
     return ( m & ( POLY_WRAP - 1 ) ) == POLY_WRAP - 1;
 }
 
+// Stepping down from slice m, the last slice before the next re-wrap point is
+// the one above m' = (m & ~(POLY_WRAP-1)) - 1 (the largest m' < m with
+// poly_wrap_due(m')); returns max(m', mlo - 1): step while slice > it.
+__device__ __forceinline__ int64_t poly_segment_end( int64_t m, int64_t mlo )
+{
+    const int64_t next = ( m & ~int64_t( POLY_WRAP - 1 ) ) - 1;
+    return next > mlo - 1 ? next : mlo - 1;
+}
+
 template <bool POW2>
 __device__ __forceinline__ int poly_locate( double &x, int mask, double Nxd, double invNx, double &t )
 {

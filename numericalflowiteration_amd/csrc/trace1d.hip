@@ -35,6 +35,7 @@
 #include "device_common.hpp"
 #include "internal.hpp"
 #include "step1d.hpp"
+#include "strict1d.hpp"
 
 namespace nufi_impl
 {
@@ -43,14 +44,6 @@ using namespace nufi_dev;
 
 namespace
 {
-
-template <int K>
-__host__ __device__ constexpr double inv_factorial()
-{
-    double f = 1.0;
-    for ( int i = 2; i <= K; ++i ) f *= double(i);
-    return 1.0 / f;
-}
 
 // Cooperative copy of `count` elements into LDS (16-byte vectors where the
 // source allows it; chunk bases are 16-byte aligned by construction).
@@ -255,33 +248,7 @@ void trace1d_fast_kernel( trace1_params p )
 template <int ORDER>
 __device__ __forceinline__ double efield_strict( double x, const double *D, const trace1_params &p )
 {
-#pragma clang fp contract(off)
-    x = x - p.x_min;
-    x = x - p.Lx * floor( x * p.Lx_inv );
-    const double knot = floor( x * p.dx_inv );
-    // ii in [-1, Nx]: x rounded onto Lx gives Nx (fields.hpp:49), a wrap that
-    // rounds below 0 gives -1; the extended difference table covers both with
-    // the periodic values, so no read leaves the slice.
-    const int ii = int( knot );
-    x = x * p.dx_inv - knot;
-
-    constexpr int n = ORDER, d = 1;
-    double c[ORDER];
-    #pragma unroll
-    for ( int j = 1; j < n; ++j ) c[j] = D[ ii + j - 1 ];
-    #pragma unroll
-    for ( int j = 1; j < n - d; ++j )
-    {
-        #pragma unroll
-        for ( int i = n - d - 1; i >= j; --i )
-        {
-            const double lo = ( ( ( x + double(n) ) - double(d) ) - 1.0 ) - double(i);
-            const double hi = double( i - j + 1 ) - x;
-            c[d+i] = lo * c[d+i] + hi * c[d+i-1];
-        }
-    }
-    const double val = inv_factorial<n - d - 1>() * c[n-1];
-    return -( p.dx_inv * val );
+    return -phi_deriv_strict<ORDER,1>( x, D, p.x_min, p.Lx, p.Lx_inv, p.dx_inv );
 }
 
 template <int ORDER, int BLOCK, int MODE>
@@ -349,17 +316,23 @@ __device__ __forceinline__ void trace1d_poly_body( const trace1_params &p, uint3
         const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
         const int cnt = int( mhi - mlo + 1 );
         poly_chunk_begin( pf, lds, p.dhist, p.stride_d, sq, q2, cells, p.kick, mhi, p.chunk );
-        for ( int jj = cnt - 1; jj >= 0; --jj )
+        // segments between re-wrap points, so the step loop itself carries no wrap
+        for ( int jj = cnt - 1; jj >= 0; )
         {
-            const double *Q = lds + jj * sq;
-            const double h = ( mlo + jj == 0 ) ? 0.5 : 1.0;
-            if ( POW2 && poly_wrap_due( mlo + jj ) )
+            const int64_t m = mlo + jj;
+            if ( POW2 && poly_wrap_due( m ) )
             {
                 #pragma unroll
                 for ( int r = 0; r < NPT; ++r ) rewrap( s[r], Nxd, invNx );
             }
-            #pragma unroll
-            for ( int r = 0; r < NPT; ++r ) poly_step<POW2>( s[r], u[r], Q, q2, cd, h, mask, Nxd, invNx );
+            const int jstop = POW2 ? int( poly_segment_end( m, mlo ) - mlo ) : -1;
+            for ( ; jj > jstop; --jj )
+            {
+                const double *Q = lds + jj * sq;
+                const double h = ( mlo + jj == 0 ) ? 0.5 : 1.0;
+                #pragma unroll
+                for ( int r = 0; r < NPT; ++r ) poly_step<POW2>( s[r], u[r], Q, q2, cd, h, mask, Nxd, invNx );
+            }
         }
     }
     double f[NPT];

@@ -13,6 +13,7 @@ One context lives on one device; host arrays are numpy float64.
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -131,6 +132,50 @@ class GpuScheduler:
         if out.size:
             self._check(self._lib.nufi_download_energies(self._ctx, n_begin, n_end, _ptr(out)))
         return out
+
+    # ---- diagnostics samplers (bin/test_nufi_gpu_1d.cpp:290-361) -------------
+    def sample_field(self, n: int, x) -> tuple[np.ndarray, np.ndarray]:
+        """(E, rho) = (-phi', -phi'') of slice n at the points x (1D)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        E, rho = np.zeros_like(x), np.zeros_like(x)
+        if x.size:
+            self._check(self._lib.nufi_sample_field(self._ctx, n, _ptr(x), x.size, _ptr(E), _ptr(rho)))
+        return E, rho
+
+    def sample_f(self, n: int, x, v) -> np.ndarray:
+        """f(n, x, v) with the initial half-kick (rho.hpp:63-96) at the points (x, v)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        if x.shape != v.shape:
+            raise ValueError("x and v differ in shape")
+        f = np.zeros_like(x)
+        if x.size:
+            self._check(self._lib.nufi_sample_f(self._ctx, n, _ptr(x), _ptr(v), x.size, _ptr(f)))
+        return f
+
+    def phase_flow(self, n: int, x, v) -> tuple[np.ndarray, np.ndarray]:
+        """The phase flow of rho.hpp:98-131 at the points (x, v)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        if x.shape != v.shape:
+            raise ValueError("x and v differ in shape")
+        xo, vo = np.zeros_like(x), np.zeros_like(x)
+        if x.size:
+            self._check(self._lib.nufi_phase_flow(self._ctx, n, _ptr(x), _ptr(v), x.size, _ptr(xo), _ptr(vo)))
+        return xo, vo
+
+    # ---- history files (checkpoint / resume, isolated steps) ----------------
+    def save_history(self, path: str, n_slices: int | None = None, fmt="binary"):
+        """Slices 0..n_slices-1 of the device history (and, binary, their energies)."""
+        from .history import _fmt
+        n_slices = self.conf.Nt + 1 if n_slices is None else n_slices
+        self._check(self._lib.nufi_save_history(self._ctx, os.fsencode(path), _fmt(fmt), n_slices))
+
+    def load_history(self, path: str) -> int:
+        """Load a history file of the same grid; returns the number of slices loaded."""
+        n = ctypes.c_uint64(0)
+        self._check(self._lib.nufi_load_history(self._ctx, os.fsencode(path), ctypes.byref(n)))
+        return int(n.value)
 
     def synchronize(self):
         self._check(self._lib.nufi_synchronize(self._ctx))

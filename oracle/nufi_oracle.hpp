@@ -331,6 +331,30 @@ real f1d( size_t n, real x, real u, const real *hist, const grid1<real> &g )
     return f0_1d<real>( g, x, u );
 }
 
+/* The phase flow of rho.hpp:98-131: the foot point (x, u) of f1d's trace
+ * for n > 1 (for n <= 1 the point is not moved), then x wrapped by
+ * x - Lx floor(x / Lx) (into [0, Lx), whatever x_min is). */
+template <typename real, int order>
+void phase_flow1d( size_t n, real &x, real &u, const real *hist, const grid1<real> &g )
+{
+    if ( n > 1 )
+    {
+        const size_t stride_t = g.Nx + order - 1;
+        real E = -field1d<real,order,1>( x, hist + n*stride_t, g );
+        u = u + real_of<real>(0.5)*g.dt*E;
+        while ( --n )
+        {
+            x = x - g.dt*u;
+            E = -field1d<real,order,1>( x, hist + n*stride_t, g );
+            u = u + g.dt*E;
+        }
+        x = x - g.dt*u;
+        E = -field1d<real,order,1>( x, hist, g );
+        u = u + real_of<real>(0.5)*g.dt*E;
+    }
+    x = x - g.Lx * floor( x*g.Lx_inv );
+}
+
 template <typename real, int order>
 void efield3d( real x, real y, real z, const real *slice, const grid3<real> &g,
                real &Ex, real &Ey, real &Ez )

@@ -119,6 +119,18 @@ struct trace1d_op
     }
 };
 template <int order>
+struct phase_flow1d_op
+{
+    static int run( const nufi_config *c, uint64_t n, double x, double u, const double *hist,
+                    double *x_out, double *u_out )
+    {
+        grid1<double> g( *c );
+        phase_flow1d<double,order>( n, x, u, hist, g );
+        *x_out = x; *u_out = u;
+        return 0;
+    }
+};
+template <int order>
 struct trace3d_op
 {
     static int run( const nufi_config *c, int with_half_kick, uint64_t n, const double *xv,
@@ -382,6 +394,10 @@ int oracle_field3d( const nufi_config *c, int order, int dx, int dy, int dz,
 int oracle_trace1d( const nufi_config *c, int order, int with_half_kick, uint64_t n, double x, double u,
                     const double *hist, double *out )
 { if ( bad( c, 1 ) ) return NUFI_ERR_INVALID; return dispatch_order<trace1d_op>( order, c, with_half_kick, n, x, u, hist, out ); }
+
+int oracle_phase_flow1d( const nufi_config *c, int order, uint64_t n, double x, double u, const double *hist,
+                         double *x_out, double *u_out )
+{ if ( bad( c, 1 ) ) return NUFI_ERR_INVALID; return dispatch_order<phase_flow1d_op>( order, c, n, x, u, hist, x_out, u_out ); }
 
 int oracle_trace3d( const nufi_config *c, int order, int with_half_kick, uint64_t n, const double *xv,
                     const double *hist, double *out )

@@ -43,6 +43,7 @@ def lib():
             "oracle_field1d": [_cp, _int, _int, _d, _dp, _dp],
             "oracle_field3d": [_cp, _int, _int, _int, _int, _d, _d, _d, _dp, _dp],
             "oracle_trace1d": [_cp, _int, _int, _u64, _d, _d, _dp, _dp],
+            "oracle_phase_flow1d": [_cp, _int, _u64, _d, _d, _dp, _dp, _dp],
             "oracle_trace3d": [_cp, _int, _int, _u64, _dp, _dp, _dp],
             "oracle_rho1d": [_cp, _int, _u64, _dp, _dp, _int],
             "oracle_rho3d": [_cp, _int, _u64, _dp, _dp, _int],
@@ -122,6 +123,13 @@ def trace1d(conf: Config, n: int, x: float, u: float, hist, half_kick=False, ord
     _check(lib().oracle_trace1d(ctypes.byref(conf), order, int(half_kick), n, x, u, _p(hist), _p(out)),
            "trace1d")
     return float(out[0])
+
+
+def phase_flow1d(conf: Config, n: int, x: float, u: float, hist, order: int = 4):
+    xo, uo = np.zeros(1), np.zeros(1)
+    _check(lib().oracle_phase_flow1d(ctypes.byref(conf), order, n, x, u, _p(hist), _p(xo), _p(uo)),
+           "phase_flow1d")
+    return float(xo[0]), float(uo[0])
 
 
 def trace3d(conf: Config, n: int, xv, hist, half_kick=False, order: int = 4) -> float:

@@ -1,0 +1,92 @@
+"""Diagnostics samplers on the GPU (nufi_sample_field / nufi_sample_f /
+nufi_phase_flow; bin/test_nufi_gpu_1d.cpp:290-361) against the oracle on the
+same history: E and -phi'' of slice n (fields.hpp:36-61), eval_f
+(rho.hpp:63-96) and eval_phase_flow (rho.hpp:98-131) at arbitrary points,
+including the edge x = x_min + Lx, points outside the box and the n <= 1
+phase-flow quirk.  The device evaluates in the reference's operation order;
+tolerances: field 1e-14 (relative to the field's scale), f and phase flow
+1e-12 (device cos/exp differ from the host's in the last bits)."""
+import numpy as np
+import pytest
+
+import numericalflowiteration_amd as nufi
+from tests import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(order=4):
+    conf = nufi.landau_1d(Nx=32, Nu=64, T=2.0)                 # Nt = 32
+    hist = O.new_history(conf, conf.Nt + 1, order=order)
+    O.run(conf, 0, conf.Nt + 1, hist, order=order)
+    return conf, hist
+
+
+def points(conf, count=300, seed=5):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(conf.x_min - conf.Lx, conf.x_max + conf.Lx, count)
+    x[:4] = [conf.x_min, conf.x_min + conf.Lx, conf.x_min + 0.5 * conf.dx, -1e-17]
+    v = rng.uniform(conf.u_min, conf.u_max, count)
+    return x, v
+
+
+@pytest.mark.parametrize("order", [4, 3, 5])
+def test_sample_field_against_oracle(order):
+    conf, hist = setup(order)
+    st = conf.stride_t(order)
+    x, _ = points(conf)
+    with nufi.GpuScheduler(conf, order=order) as s:
+        s.upload_history(hist, 0, conf.Nt + 1)
+        for n in [0, 10, conf.Nt]:
+            E, rho = s.sample_field(n, x)
+            sl = hist[n * st:(n + 1) * st]
+            E_ref = np.array([-O.field1d(conf, sl, xi, 1, order=order) for xi in x])
+            r_ref = np.array([-O.field1d(conf, sl, xi, 2, order=order) for xi in x])
+            eE = np.max(np.abs(E - E_ref)) / np.max(np.abs(E_ref))
+            eR = np.max(np.abs(rho - r_ref)) / np.max(np.abs(r_ref))
+            print(f"[samplers] order={order} n={n}: E err {eE:.2e}, -phi'' err {eR:.2e}", flush=True)
+            assert eE <= 1e-14 and eR <= 1e-14
+
+
+def test_sample_f_and_phase_flow_against_oracle():
+    conf, hist = setup()
+    x, v = points(conf, 200)
+    with nufi.GpuScheduler(conf) as s:
+        s.upload_history(hist, 0, conf.Nt + 1)
+        for n in [0, 1, 2, 5, conf.Nt]:
+            f = s.sample_f(n, x, v)
+            f_ref = np.array([O.trace1d(conf, n, xi, vi, hist, half_kick=True) for xi, vi in zip(x, v)])
+            ef = np.max(np.abs(f - f_ref) / np.maximum(np.abs(f_ref), 1e-300))
+            xo, vo = s.phase_flow(n, x, v)
+            ref = np.array([O.phase_flow1d(conf, n, xi, vi, hist) for xi, vi in zip(x, v)])
+            ex = np.max(np.abs(xo - ref[:, 0])) / conf.Lx
+            ev = np.max(np.abs(vo - ref[:, 1])) / np.max(np.abs(ref[:, 1]))
+            print(f"[samplers] n={n}: f err {ef:.2e}, flow x err {ex:.2e}, v err {ev:.2e}", flush=True)
+            assert ef <= 1e-12 and ex <= 1e-12 and ev <= 1e-12
+            assert np.all((xo >= 0) & (xo < conf.Lx))
+            if n <= 1:                               # not moved (rho.hpp:102)
+                assert np.array_equal(vo, v)
+
+
+def test_phase_flow_is_the_foot_of_eval_f():
+    """f0 at the phase-flow point equals the sampled f (n > 1)."""
+    conf, hist = setup()
+    x, v = points(conf, 100, seed=9)
+    a, k = conf.f0.alpha, conf.f0.k
+    with nufi.GpuScheduler(conf) as s:
+        s.upload_history(hist, 0, conf.Nt + 1)
+        f = s.sample_f(20, x, v)
+        xo, vo = s.phase_flow(20, x, v)
+    f0 = (1 + a * np.cos(k * xo)) * np.exp(-vo * vo / 2) / np.sqrt(2 * np.pi)
+    assert np.max(np.abs(f0 - f) / f) < 1e-12
+
+
+def test_sampler_errors():
+    conf, hist = setup()
+    with nufi.GpuScheduler(conf) as s:
+        with pytest.raises(nufi.NufiRangeError):
+            s.sample_field(conf.Nt + 1, [0.0])
+        assert s.sample_f(3, [], []).size == 0
+    with nufi.GpuScheduler(nufi.landau_3d(N=8, V=4, T=1.0)) as s3:
+        with pytest.raises(nufi.NufiError):
+            s3.sample_field(0, [0.0])

@@ -192,3 +192,22 @@ def test_metrics_zero_field_closed_forms():
     # after free streaming of a spatially uniform f0 nothing changes
     m5 = O.metrics(conf, 5, h)
     assert np.allclose(m5, m, rtol=1e-13, atol=0)
+
+
+def test_phase_flow_pins():
+    """rho.hpp:98-131: for n > 1 the phase flow is the foot point of the
+    traced f (rho.hpp:63-96), so f0 at it equals f1d; for n <= 1 the point is
+    only wrapped (the reference does not trace); x lands in [0, Lx)."""
+    conf = nufi.landau_1d(Nx=32, Nu=64, T=2.0)
+    hist = O.new_history(conf, conf.Nt + 1)
+    O.run(conf, 0, conf.Nt + 1, hist)
+    a, k = conf.f0.alpha, conf.f0.k
+    for n in [2, 7, conf.Nt]:
+        for x, u in [(0.3, 1.7), (11.0, -2.2), (6.1, 0.01)]:
+            xf, uf = O.phase_flow1d(conf, n, x, u, hist)
+            assert 0.0 <= xf < conf.Lx
+            ref = O.trace1d(conf, n, x, u, hist, half_kick=True)
+            assert abs(f0_landau(xf, uf, a, k) - ref) <= 1e-13 * ref, (n, x, u)
+    for n in [0, 1]:
+        xf, uf = O.phase_flow1d(conf, n, conf.Lx + 0.25, 0.7, hist)
+        assert uf == 0.7 and abs(xf - 0.25) < 1e-15
