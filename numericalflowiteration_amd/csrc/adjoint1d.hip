@@ -58,8 +58,10 @@ __device__ inline void f0_grad_1d( const nufi_f0 &p, double x, double u, double 
     case NUFI_F0_MULTIMODE_1D:
     {
         double sm = 1.0, dsm = 0.0;
-        for ( int m = 1; m <= p.n_modes; ++m )
+        #pragma unroll
+        for ( int m = 1; m <= NUFI_MAX_MODES; ++m )     // static indices: no private-memory copy
         {
+            if ( m > p.n_modes ) continue;
             const double km = p.k * m;
             const double c = cos( km * x ), s = sin( km * x );
             sm += p.a[m-1] * c;
@@ -72,7 +74,8 @@ __device__ inline void f0_grad_1d( const nufi_f0 &p, double x, double u, double 
     }
     default:
         fx = fu = 0.0;
-        for ( int m = 0; m < npar; ++m ) fa[m] = 0.0;
+        #pragma unroll
+        for ( int m = 0; m < NUFI_MAX_MODES; ++m ) if ( m < npar ) fa[m] = 0.0;
     }
 }
 
@@ -107,10 +110,12 @@ void adjoint1d_kernel( adjoint1_params p )
         __syncthreads();
         build_poly<BLOCK>( Ql, p.dhist, p.stride_d, sq, q2, mlo, cnt, p.Nx + 2, kick );
         __syncthreads();
-        for ( int jj = cnt - 1; jj >= 0; --jj )
+        for ( int jj = cnt - 1; jj >= 0; )          // segments between re-wrap points
         {
             if ( POW2 && poly_wrap_due( mlo + jj ) ) rewrap( s, Nxd, invNx );
-            poly_step<POW2>( s, u, Ql + jj * sq, q2, cd, ( mlo + jj == 0 ) ? 0.5 : 1.0, mask, Nxd, invNx );
+            const int jstop = POW2 ? int( poly_segment_end( mlo + jj, mlo ) - mlo ) : -1;
+            for ( ; jj > jstop; --jj )
+                poly_step<POW2>( s, u, Ql + jj * sq, q2, cd, ( mlo + jj == 0 ) ? 0.5 : 1.0, mask, Nxd, invNx );
         }
     }
 
@@ -120,8 +125,10 @@ void adjoint1d_kernel( adjoint1_params p )
     f0_grad_1d( p.f0, p.x_min + s*p.dx, u, fx, fu, fa, p.npar );
     double sbar = fbar * fx * p.dx;     // d/ds = dx d/dx
     double ubar = fbar * fu;
-    for ( int m = 0; m < int( p.npar ); ++m )
+    #pragma unroll
+    for ( int m = 0; m < NUFI_MAX_MODES; ++m )          // npar is uniform: the branch is too
     {
+        if ( m >= int( p.npar ) ) continue;
         const csum v = wave_csum( csum { fbar * fa[m], 0.0 } );
         if ( task && lane == 0 ) p.grad_part[ size_t( g ) * NUFI_MAX_MODES + m ] += v.s + v.c;
     }

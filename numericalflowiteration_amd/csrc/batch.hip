@@ -202,6 +202,8 @@ int nufi_batch_create( const nufi_config *confs, uint64_t n_runs, int order, int
     if ( ( rc = balloc( b, &b->d_runs, n_runs, "batch run table" ) ) ) return bail( rc );
     if ( ( rc = balloc( b, &b->d_fields, n_runs, "batch field table" ) ) ) return bail( rc );
 
+    int tw_lds = 0;
+    b->field_lds = field_batch_lds_bytes( b->max_N, &tw_lds );
     std::vector<double2> tw( n_tab ), mult( n_tab );
     std::vector<double> ewt( n_tab );
     std::vector<batch_field> fields( n_runs );
@@ -236,7 +238,7 @@ int nufi_batch_create( const nufi_config *confs, uint64_t n_runs, int order, int
         uint32_t l2 = 0; while ( ( uint64_t( 1 ) << l2 ) < c.Nx ) ++l2;
         f.log2x = ( ( c.Nx & ( c.Nx - 1 ) ) == 0 ) ? l2 : 0xffffffffu;
         f.log2y = 0; f.log2z = 0;
-        f.partials = p.partials; f.wpx = b->wpx[r]; f.weight = -c.du;
+        f.partials = p.partials; f.wpx = b->wpx[r]; f.weight = -c.du; f.tw_lds = tw_lds;
         f.tw_x = b->tw + b->off_tab[r]; f.tw_y = f.tw_x; f.tw_z = f.tw_x;
         f.mult = b->mult + b->off_tab[r]; f.ewt = b->ewt + b->off_tab[r];
         f.slice = b->hist + b->off_hist[r];
@@ -250,7 +252,6 @@ int nufi_batch_create( const nufi_config *confs, uint64_t n_runs, int order, int
          ( he = hipMemcpy( b->ewt, ewt.data(), n_tab * sizeof(double), hipMemcpyHostToDevice ) ) != hipSuccess ||
          ( he = hipMemcpy( b->d_fields, fields.data(), n_runs * sizeof(batch_field), hipMemcpyHostToDevice ) ) != hipSuccess )
     { bhip( b, he, "batch table upload" ); return bail( NUFI_ERR_DEVICE ); }
-    b->field_lds = field_batch_lds_bytes( b->max_N );
 
     const size_t elem = b->variant == TRACE_FP32C ? sizeof(float) : sizeof(double);
     const int chunk = int( std::max<size_t>( 1, std::min<size_t>( 64, 65536 / ( size_t( b->max_sq ) * elem ) ) ) );

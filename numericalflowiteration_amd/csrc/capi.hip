@@ -162,9 +162,9 @@ uint32_t rho_ppx( const nufi_ctx *c )
     return c->variant != TRACE_STRICT ? c->ppx : c->wpx;
 }
 
-// Default launch shapes (the autotuner refines them): measured best shapes on
-// MI355X for the two-stream 1D grid (1024 threads, chunk 64) and 3D Landau 16^3
-// (FP64: 1024 threads; FP32C: 256 threads x 4 nodes).
+// Default launch shapes (the autotuner refines them): 1D x-major 1024 threads
+// x 1 node (16 velocity rows per workgroup), chunk by the LDS budget; 3D
+// 512 threads x 2 (FP64) or x 4 (FP32C) nodes.
 void default_shape( nufi_ctx *c )
 {
     const uint64_t waves = c->n_space * c->wpx;
@@ -191,8 +191,8 @@ void default_shape( nufi_ctx *c )
         int chunk = int( std::min<size_t>( c->lds_cap, 64 * 1024 ) / per );
         chunk = std::max( 1, std::min( chunk, 1 ) );
         if ( c->variant == TRACE_STRICT ) c->shape = launch_shape { 256, 1, chunk };
-        else if ( c->variant == TRACE_FP32C ) c->shape = launch_shape { 256, 4, chunk };
-        else c->shape = launch_shape { 1024, 1, chunk };
+        else if ( c->variant == TRACE_FP32C ) c->shape = launch_shape { 512, 4, chunk };
+        else c->shape = launch_shape { 512, 2, chunk };
     }
 }
 
@@ -948,7 +948,7 @@ int nufi_autotune( nufi_ctx *c, uint64_t n_probe, int repeats, int *block, int *
     else
     {
         if ( c->variant == TRACE_STRICT ) cand.push_back( { 256, 1, 1 } );
-        else for ( int b : blocks1 ) for ( int np : npts ) for ( int ch : { 1, 2 } ) cand.push_back( { b, np, ch } );
+        else for ( int b : { 256, 512 } ) for ( int np : npts ) for ( int ch : { 1, 2 } ) cand.push_back( { b, np, ch } );
     }
     const launch_shape saved = c->shape;
     const bool saved_timing = c->timing;

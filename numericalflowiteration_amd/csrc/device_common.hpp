@@ -14,6 +14,10 @@ namespace nufi_dev
 
 constexpr int WAVE = 64;
 
+// Global loads each thread keeps in flight when it copies a history chunk
+// into LDS (the copy is otherwise one L2 round trip per element).
+constexpr unsigned STAGE_BATCH = 8;
+
 // 1/sqrt(2 pi), (2 pi)^(-3/2) and the 3D two-stream constant, as the literals
 // of nufi/config.hpp:82, 233, 238.
 constexpr double kInvSqrt2Pi = 0.39894228040143267793994;
@@ -35,8 +39,9 @@ __device__ inline double f0_1d( const nufi_f0 &p, double x, double u )
     case NUFI_F0_MULTIMODE_1D:
     {
         double s = 1.0;
-        for ( int m = 1; m <= p.n_modes; ++m )
-            s = s + p.a[m-1] * cos( ( p.k*m ) * x );
+        #pragma unroll
+        for ( int m = 1; m <= NUFI_MAX_MODES; ++m )     // static indices: no private-memory copy
+            if ( m <= p.n_modes ) s = s + p.a[m-1] * cos( ( p.k*m ) * x );
         return kInvSqrt2Pi * s * exp( -u*u/2.0 );
     }
     default:

@@ -18,6 +18,14 @@ __device__ __forceinline__ double2 cmul( double2 a, double2 b )
     return make_double2( a.x*b.x - a.y*b.y, a.x*b.y + a.y*b.x );
 }
 
+// twiddles an axis of length L reads: exp(-2 pi i k/L) for k < L/2 (radix 2)
+// or k < L (direct DFT); none for L <= 1
+__host__ __device__ inline uint32_t twiddle_entries( uint32_t L, uint32_t log2L )
+{
+    if ( L <= 1 ) return 0;
+    return log2L == 0xffffffffu ? L : L / 2;
+}
+
 struct axis_desc
 {
     uint32_t L, log2L, S;       // length, log2 (or ~0u), stride

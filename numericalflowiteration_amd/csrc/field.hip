@@ -29,16 +29,17 @@ using namespace nufi_dev;
 namespace
 {
 
-__device__ double2 *fft3( double2 *A, double2 *B, const field_params &p, bool inverse )
+__device__ double2 *fft3( double2 *A, double2 *B, const field_params &p, const double2 *tx, const double2 *ty,
+                          const double2 *tz, bool inverse )
 {
     axis_desc ax { p.Nx, p.log2x, 1u,          p.N / p.Nx, 0u, p.Nx, p.Ny };
     axis_desc ay { p.Ny, p.log2y, p.Nx,        p.N / p.Ny, 1u, p.Nx, p.Ny };
     axis_desc az { p.Nz, p.log2z, p.Nx * p.Ny, p.N / p.Nz, 2u, p.Nx, p.Ny };
-    double2 *r = fft_axis( A, B, ax, p.tw_x, inverse );
+    double2 *r = fft_axis( A, B, ax, tx, inverse );
     double2 *o = ( r == A ) ? B : A;
-    r = fft_axis( r, o, ay, p.tw_y, inverse );
+    r = fft_axis( r, o, ay, ty, inverse );
     o = ( r == A ) ? B : A;
-    r = fft_axis( r, o, az, p.tw_z, inverse );
+    r = fft_axis( r, o, az, tz, inverse );
     return r;
 }
 
@@ -48,6 +49,20 @@ __device__ __forceinline__ void field_body( const field_params &p, double2 *lds2
     double *red = reinterpret_cast<double*>( lds2 + 2*p.N );
     const uint32_t N = p.N;
 
+    // the twiddle tables in LDS (one load per kernel instead of one L2 round
+    // trip per FFT stage), when they fit beside the buffers
+    const double2 *tx = p.tw_x, *ty = p.tw_y, *tz = p.tw_z;
+    if ( p.tw_lds )
+    {
+        double2 *t = lds2 + 2*p.N + ( FIELD_BLOCK / WAVE ) / 2;
+        const uint32_t lx = twiddle_entries( p.Nx, p.log2x ), ly = twiddle_entries( p.Ny, p.log2y );
+        const uint32_t lz = twiddle_entries( p.Nz, p.log2z );
+        for ( uint32_t i = threadIdx.x; i < lx; i += FIELD_BLOCK ) t[i] = p.tw_x[i];
+        for ( uint32_t i = threadIdx.x; i < ly; i += FIELD_BLOCK ) t[lx + i] = p.tw_y[i];
+        for ( uint32_t i = threadIdx.x; i < lz; i += FIELD_BLOCK ) t[lx + ly + i] = p.tw_z[i];
+        tx = t; ty = t + lx; tz = t + lx + ly;
+    }
+
     // 1. rho (or the nodal values) into A
     for ( uint32_t i = threadIdx.x; i < N; i += FIELD_BLOCK )
     {
@@ -55,8 +70,7 @@ __device__ __forceinline__ void field_body( const field_params &p, double2 *lds2
         if ( p.mode == FIELD_STEP && p.partials != nullptr )
         {
             csum acc { 0.0, 0.0 };
-            const double *pp = p.partials + size_t( i ) * p.wpx;
-            for ( uint32_t j = 0; j < p.wpx; ++j ) acc = csum_add( acc, csum { pp[j], 0.0 } );
+            for ( uint32_t j = 0; j < p.wpx; ++j ) acc = csum_add( acc, csum { p.partials[ size_t( j ) * N + i ], 0.0 } );
             v = p.weight * ( acc.s + acc.c );
         }
         else
@@ -69,7 +83,7 @@ __device__ __forceinline__ void field_body( const field_params &p, double2 *lds2
     __syncthreads();
 
     // 2. forward transform
-    double2 *F = fft3( A, B, p, false );
+    double2 *F = fft3( A, B, p, tx, ty, tz, false );
     double2 *G = ( F == A ) ? B : A;
 
     // 3. spectral energy and the multiplier
@@ -93,7 +107,7 @@ __device__ __forceinline__ void field_body( const field_params &p, double2 *lds2
     }
 
     // 4. inverse transform
-    double2 *R = fft3( F, G, p, true );
+    double2 *R = fft3( F, G, p, tx, ty, tz, true );
 
     // 5. outputs
     if ( p.mode == FIELD_POISSON )
@@ -157,8 +171,7 @@ __global__ void reduce_partials_kernel( const double *partials, uint32_t wpx, do
     if ( i >= first && i < first + count )
     {
         csum acc { 0.0, 0.0 };
-        const double *pp = partials + size_t( i ) * wpx;
-        for ( uint32_t j = 0; j < wpx; ++j ) acc = csum_add( acc, csum { pp[j], 0.0 } );
+        for ( uint32_t j = 0; j < wpx; ++j ) acc = csum_add( acc, csum { partials[ size_t( j ) * n_space + i ], 0.0 } );
         v = weight * ( acc.s + acc.c );
     }
     rho[i] = v;
@@ -212,14 +225,29 @@ int field_supported( int, uint32_t Nx, uint32_t Ny, uint32_t Nz )
     return ( N * 2 * sizeof(double2) + 64 * sizeof(double) ) <= 160u * 1024u ? 1 : 0;
 }
 
-size_t field_lds_bytes( const field_params &p )
+// buffers + reduction scratch, and the twiddle tables when they fit
+static size_t field_base_bytes( uint32_t N )
 {
-    return size_t( p.N ) * 2 * sizeof(double2) + ( FIELD_BLOCK / WAVE ) * sizeof(double);
+    return size_t( N ) * 2 * sizeof(double2) + ( FIELD_BLOCK / WAVE ) * sizeof(double);
 }
 
-hipError_t launch_field( const field_params &p, hipStream_t st )
+static size_t field_twiddle_bytes( const field_params &p )
 {
+    return sizeof(double2) * ( size_t( twiddle_entries( p.Nx, p.log2x ) ) + twiddle_entries( p.Ny, p.log2y ) +
+                               twiddle_entries( p.Nz, p.log2z ) );
+}
+
+size_t field_lds_bytes( const field_params &p )
+{
+    const size_t base = field_base_bytes( p.N ), all = base + field_twiddle_bytes( p );
+    return all <= 160u * 1024u ? all : base;
+}
+
+hipError_t launch_field( const field_params &p_in, hipStream_t st )
+{
+    field_params p = p_in;
     const size_t lds = field_lds_bytes( p );
+    p.tw_lds = lds > field_base_bytes( p.N ) ? 1 : 0;
     if ( lds > 65536 )
     {
         hipError_t e = hipFuncSetAttribute( reinterpret_cast<const void*>( field_kernel ),
@@ -233,13 +261,22 @@ hipError_t launch_field( const field_params &p, hipStream_t st )
 hipError_t launch_field_batch( const batch_field *runs, uint32_t n_runs, uint64_t n, size_t lds, hipStream_t st )
 {
     if ( n_runs == 0 ) return hipSuccess;
+    if ( lds > 65536 )
+    {
+        hipError_t e = hipFuncSetAttribute( reinterpret_cast<const void*>( field_batch_kernel ),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, int( lds ) );
+        if ( e != hipSuccess ) return e;
+    }
     hipLaunchKernelGGL( field_batch_kernel, dim3( n_runs ), dim3( FIELD_BLOCK ), lds, st, runs, n );
     return hipGetLastError();
 }
 
-size_t field_batch_lds_bytes( uint32_t max_N )
+size_t field_batch_lds_bytes( uint32_t max_N, int *tw_lds )
 {
-    return size_t( max_N ) * 2 * sizeof(double2) + ( FIELD_BLOCK / WAVE ) * sizeof(double);
+    // 1D runs: at most max_N twiddles (max_N/2 for a power of two)
+    const size_t all = field_base_bytes( max_N ) + size_t( max_N ) * sizeof(double2);
+    *tw_lds = all <= 160u * 1024u ? 1 : 0;
+    return *tw_lds ? all : field_base_bytes( max_N );
 }
 
 hipError_t launch_reduce_partials( const double *partials, uint32_t wpx, double weight,

@@ -28,7 +28,7 @@ enum trace_variant : int
 //
 // velocity-major (strict and metrics kernels): the launch is cut into
 // wave-tasks; task g covers ix = ix_first + g / wpx and iu = (g % wpx)*64 +
-// lane, and writes one partial sum of f to partials[ix*wpx + g % wpx].
+// lane, and writes one partial sum of f to partials[(g % wpx)*Nx + ix].
 //
 // x-major (fast and FP32C rho kernels): the 64 lanes of a wave take xw
 // consecutive x nodes (xw = min(Nx, 64)) at vpw = 64/xw velocities; lanes at
@@ -39,8 +39,10 @@ enum trace_variant : int
 // the velocity rows (r*WPB + w)*vpw + lane/xw, r < NPT.  The f values are
 // staged in LDS and summed over fixed groups of XM_GROUP consecutive
 // velocities (a sequential compensated sum in iu order) into
-// partials[ix*ppx + iu/XM_GROUP], ppx = ceil(Nu/XM_GROUP): the partials do not
-// depend on the launch shape.
+// partials[(iu/XM_GROUP)*Nx + ix], ppx = ceil(Nu/XM_GROUP) groups: the
+// partials do not depend on the launch shape.  All rho partials are stored
+// group-major (partials[j*n_space + i]), so both their writers and the
+// field kernel's fixed-order sum over j read and write coalesced.
 constexpr uint32_t XM_GROUP = 16;
 inline uint32_t xm_xw( uint32_t Nx ) { return Nx < 64 ? Nx : 64; }
 inline uint32_t xm_ppx( uint32_t Nu ) { return ( Nu + XM_GROUP - 1 ) / XM_GROUP; }
@@ -58,7 +60,7 @@ struct trace1_params
     const double *dhist;     // extended differences of slice m at dhist + m*stride_d (FP64)
     uint32_t stride_q, q2_off;        // LDS kick-polynomial table: slice stride, Q2 row (poly_stride, poly_q2)
     double   kick;                    // -dt/(2 dx), the scale of the kick polynomials
-    double       *partials;  // velocity-major: partials[ix*wpx + j]; x-major: partials[ix*ppx + j];
+    double       *partials;  // velocity-major: partials[j*Nx + ix], j < wpx; x-major: j < ppx;
                              // MODE_METRICS: partials[4*g + k]
     uint64_t n;
     uint64_t q_begin, q_end;
@@ -79,7 +81,7 @@ struct trace1_params
 // + iu (cuda_kernel.cu:399-412).  The strict and metrics kernels use the
 // wave-task cut of the 1D case with Nu -> Nvel; the fast and FP32C rho
 // kernels the x-major layout of trace3d.hip (64 neighbouring spatial points
-// per wave, partials[l*ppx + iv/vgroup]).
+// per wave, partials[(iv/vgroup)*n_space + l]).
 // Velocities per x-major partial: 256, halved (down to 16) until the launch
 // has at least 8192 waves; fixed per grid, so results do not depend on the shape.
 inline uint32_t xm3_vgroup( uint64_t n_space, uint64_t n_vel )
@@ -93,7 +95,7 @@ struct trace3_params
 {
     const double *hist;      // coefficient slices (FP64), stride st
     const float  *hist32;    // FP32 copy (TRACE_FP32C)
-    double       *partials;  // MODE_RHO: partials[l*wpx + j]; MODE_METRICS: partials[4*g + k]
+    double       *partials;  // MODE_RHO: partials[j*n_space + l]; MODE_METRICS: partials[4*g + k]
     int      mode;
     double   weight;
     uint64_t n;
@@ -194,8 +196,8 @@ struct sample1_params
 hipError_t launch_sample1d( const sample1_params &p, int what, int order, hipStream_t st );
 
 // ---- field solve (field.hip) -------------------------------------------------
-// Input rho either from per-wave partials (sum_j partials[i*wpx + j], times
-// -dv), or from a plain array.  Output: the padded coefficient slice, its
+// Input rho either from the partials (sum_j partials[j*N + i] in order j = 0..,
+// times -dv), or from a plain array.  Output: the padded coefficient slice, its
 // differences (1D) and FP32 copies, the energy, and optionally the nodal phi.
 enum field_mode : int
 {
@@ -215,6 +217,7 @@ struct field_params
     double  *values;             // rho / phi (FIELD_POISSON in/out, FIELD_FIT in)
     double  *rho_out;            // optional copy of rho
     const double2 *tw_x, *tw_y, *tw_z; // forward twiddles exp(-2 pi i k/N_axis)
+    int      tw_lds;             // copy them into LDS first (set by the launcher)
     const double2 *mult;         // per-mode multiplier (complex), N entries
     const double  *ewt;          // per-mode energy weight, N entries
     double  *slice;              // history slice n (coefficients)
@@ -235,14 +238,15 @@ struct batch_field
     uint32_t st, sd;
 };
 hipError_t launch_field_batch( const batch_field *runs, uint32_t n_runs, uint64_t n, size_t lds, hipStream_t st );
-size_t field_batch_lds_bytes( uint32_t max_N );
+// LDS of the batch field launch; *tw_lds tells whether the runs' twiddles fit it
+size_t field_batch_lds_bytes( uint32_t max_N, int *tw_lds );
 size_t field_lds_bytes( const field_params &p );
 int field_supported( int dim, uint32_t Nx, uint32_t Ny, uint32_t Nz );
 
 // metrics[k] = sum_g partials[4*g + k] in a fixed order (one workgroup).
 hipError_t launch_reduce_metrics( const double *partials, uint32_t n_tasks, double *metrics, hipStream_t st );
 
-// Sum of per-wave partials into rho[ix] = weight * sum_j partials[ix*wpx+j]
+// Sum of the partials into rho[ix] = weight * sum_j partials[j*n_space + ix]
 // for ix in [first, first+count), zero elsewhere (n_space entries).
 hipError_t launch_reduce_partials( const double *partials, uint32_t wpx, double weight,
                                    uint32_t first, uint32_t count, uint32_t n_space,

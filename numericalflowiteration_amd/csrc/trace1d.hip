@@ -50,11 +50,26 @@ namespace
 template <int BLOCK, typename T>
 __device__ __forceinline__ void stage( T *__restrict__ dst, const T *__restrict__ src, uint32_t count )
 {
-    constexpr uint32_t V = 16 / sizeof(T);
+    constexpr uint32_t V = 16 / sizeof(T), B = STAGE_BATCH;
     const uint32_t nv = count / V;
     const uint4 *s4 = reinterpret_cast<const uint4*>( src );
     uint4 *d4 = reinterpret_cast<uint4*>( dst );
-    for ( uint32_t i = threadIdx.x; i < nv; i += BLOCK ) d4[i] = s4[i];
+    for ( uint32_t base = threadIdx.x; base < nv; base += B * BLOCK )   // B loads in flight per thread
+    {
+        uint4 v[B];
+        #pragma unroll
+        for ( uint32_t k = 0; k < B; ++k )             // clamped, unconditional loads stay in registers
+        {
+            const uint32_t i = base + k * BLOCK;
+            v[k] = s4[ i < nv ? i : nv - 1 ];
+        }
+        #pragma unroll
+        for ( uint32_t k = 0; k < B; ++k )
+        {
+            const uint32_t i = base + k * BLOCK;
+            if ( i < nv ) d4[i] = v[k];
+        }
+    }
     for ( uint32_t i = nv*V + threadIdx.x; i < count; i += BLOCK ) dst[i] = src[i];
 }
 
@@ -84,7 +99,7 @@ __device__ __forceinline__ void write_partial( const trace1_params &p, const nod
 {
     const csum v = wave_csum( csum { f, 0.0 } );
     if ( id.task && ( threadIdx.x % WAVE ) == 0 )
-        p.partials[ size_t(id.ix) * p.wpx + id.j ] = v.s + v.c;
+        p.partials[ size_t(id.j) * p.Nx + id.ix ] = v.s + v.c;
 }
 
 // ---- x-major layout (fast and FP32C rho kernels) ---------------------------
@@ -124,7 +139,7 @@ __device__ __forceinline__ xm_node make_xm_node( const trace1_params &p, const x
 }
 
 // f of the workgroup's nodes -> LDS -> one compensated sum per (x, group of
-// XM_GROUP consecutive velocities), summed in iu order.  `fbuf` may alias the
+// XM_GROUP consecutive velocities), summed in iu order, at partials[group*Nx + x].  `fbuf` may alias the
 // history chunk: the barrier below ends its use.
 template <int BLOCK, int NPT>
 __device__ __forceinline__ void xm_reduce( const trace1_params &p, const xm_tile &t, const xm_node id[NPT],
@@ -148,7 +163,7 @@ __device__ __forceinline__ void xm_reduce( const trace1_params &p, const xm_tile
         csum acc { 0.0, 0.0 };
         #pragma unroll
         for ( uint32_t v = 0; v < XM_GROUP; ++v ) acc = csum_add( acc, csum { col[ v * p.xw ], 0.0 } );
-        p.partials[ size_t( p.ix_first + xl ) * p.ppx + gg ] = acc.s + acc.c;
+        p.partials[ size_t( gg ) * p.Nx + p.ix_first + xl ] = acc.s + acc.c;
     }
 }
 
@@ -566,10 +581,11 @@ static bool shape_built( int variant, int order, const launch_shape &s )
     return bok && nok && s.chunk >= 1;
 }
 
-// x-major: the velocity rows of a workgroup are whole summation groups
+// x-major: the velocity rows of a workgroup are whole summation groups; and
+// 1024 threads x 4 nodes exceeds the 128 VGPRs a 1024-thread workgroup allows
 static bool shape_groups( const launch_shape &s )
 {
-    return ( uint32_t( s.block / WAVE ) * uint32_t( s.npt ) ) % XM_GROUP == 0;
+    return ( uint32_t( s.block / WAVE ) * uint32_t( s.npt ) ) % XM_GROUP == 0 && !( s.block == 1024 && s.npt == 4 );
 }
 
 // a shape for the rho kernel of the variant

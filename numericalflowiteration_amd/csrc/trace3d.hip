@@ -44,8 +44,25 @@ namespace
 template <int BLOCK, typename T>
 __device__ __forceinline__ void stage3( T *__restrict__ dst, const T *__restrict__ src, uint32_t count )
 {
-    // slice bases are not 16-byte aligned in general (odd strides): scalar copy
-    for ( uint32_t i = threadIdx.x; i < count; i += BLOCK ) dst[i] = src[i];
+    // slice bases are not 16-byte aligned in general (odd strides): scalar
+    // copies, STAGE_BATCH loads in flight per thread before their LDS stores
+    constexpr uint32_t B = STAGE_BATCH;
+    for ( uint32_t base = threadIdx.x; base < count; base += B * BLOCK )
+    {
+        T v[B];
+        #pragma unroll
+        for ( uint32_t k = 0; k < B; ++k )             // clamped, unconditional loads stay in registers
+        {
+            const uint32_t i = base + k * BLOCK;
+            v[k] = src[ i < count ? i : count - 1 ];
+        }
+        #pragma unroll
+        for ( uint32_t k = 0; k < B; ++k )
+        {
+            const uint32_t i = base + k * BLOCK;
+            if ( i < count ) dst[i] = v[k];
+        }
+    }
 }
 
 struct node3
@@ -75,7 +92,7 @@ __device__ __forceinline__ void write_partial3( const trace3_params &p, const no
 {
     const csum v = wave_csum( csum { f, 0.0 } );
     if ( id.task && ( threadIdx.x % WAVE ) == 0 )
-        p.partials[ size_t( id.l ) * p.wpx + id.j ] = v.s + v.c;
+        p.partials[ size_t( id.j ) * ( size_t( p.Nx ) * p.Ny * p.Nz ) + id.l ] = v.s + v.c;
 }
 
 // the four metric integrands (cuda_kernel.cu:453-465), wave-summed
@@ -279,8 +296,8 @@ void trace3d_fast_kernel( trace3_params p )
 // + 64*(b % n_st) + lane) and the velocity groups (b / n_st)*WPB + w of its
 // waves w; a group is vgroup consecutive velocities iv.  Each lane traces the
 // velocities of its group NPT at a time in iv order and sums f in a
-// compensated accumulator, written to partials[l*ppx + group]: the partial
-// sums do not depend on the launch shape.
+// compensated accumulator, written to partials[group*n_space + l]: the
+// partial sums do not depend on the launch shape.
 struct xm3_lane
 {
     uint32_t l, g, ix, iy, iz;
@@ -362,7 +379,7 @@ void trace3d_xm_fast_kernel( trace3_params p )
             acc = csum_add( acc, csum { f, 0.0 } );
         }
     }
-    if ( a.ok ) p.partials[ size_t( a.l ) * p.ppx + a.g ] = acc.s + acc.c;
+    if ( a.ok ) p.partials[ size_t( a.g ) * ( size_t( p.Nx ) * p.Ny * p.Nz ) + a.l ] = acc.s + acc.c;
 }
 
 // ---- FP32 compensated --------------------------------------------------------
@@ -475,7 +492,7 @@ void trace3d_xm_fp32c_kernel( trace3_params p )
             acc = csum_add( acc, csum { f, 0.0 } );
         }
     }
-    if ( a.ok ) p.partials[ size_t( a.l ) * p.ppx + a.g ] = acc.s + acc.c;
+    if ( a.ok ) p.partials[ size_t( a.g ) * ( size_t( p.Nx ) * p.Ny * p.Nz ) + a.l ] = acc.s + acc.c;
 }
 
 // ---- strict FP64 (the reference's arithmetic, order 4) ----------------------
@@ -668,8 +685,9 @@ bool trace3d_shape_ok( int variant, int order, const launch_shape &s )
     if ( order != 4 || s.chunk < 1 ) return false;
     if ( variant == TRACE_STRICT )
         return s.npt == 1 && ( s.block == 64 || s.block == 256 );
-    return ( s.block == 256 || s.block == 512 || s.block == 1024 ) &&
-           ( s.npt == 1 || s.npt == 2 || s.npt == 4 );
+    // 1024-thread workgroups cap a wave at 128 VGPRs, and these kernels then
+    // spill to scratch: not built
+    return ( s.block == 256 || s.block == 512 ) && ( s.npt == 1 || s.npt == 2 || s.npt == 4 );
 }
 
 static uint64_t visited3( const trace3_params &p )
@@ -725,7 +743,6 @@ hipError_t launch_trace3d( const trace3_params &p_in, int variant, const launch_
     {
     case 256:  return launch3_npt<256> ( p, variant, s.npt, grid, lds, st );
     case 512:  return launch3_npt<512> ( p, variant, s.npt, grid, lds, st );
-    case 1024: return launch3_npt<1024>( p, variant, s.npt, grid, lds, st );
     default:   return hipErrorInvalidValue;
     }
 }

@@ -82,6 +82,7 @@ struct field1d_op
         {
         case 0: *out = field1d<double,order,0>( x, slice, g ); return 0;
         case 1: *out = field1d<double,order,1>( x, slice, g ); return 0;
+        case 2: *out = field1d<double,order,2>( x, slice, g ); return 0;   // rho = -phi'' plots
         default: return NUFI_ERR_UNSUPPORTED;
         }
     }

@@ -193,7 +193,7 @@ def test_rho_bit_identical_across_launch_shapes(conf_args, prec, variant):
     with nufi.GpuScheduler(conf, precision=prec) as s:
         s.set_trace_variant(variant)
         s.upload_history(hist[: 14 * st], 0, 14)
-        for shape in [(1024, 1, 4), (1024, 4, 16), (512, 2, 3), (256, 4, 64), (1024, 2, 1)]:
+        for shape in [(1024, 1, 4), (512, 4, 16), (512, 2, 3), (256, 4, 64), (1024, 2, 1)]:
             s.set_launch_shape(*shape)
             rho = np.zeros(conf.Nx)
             s.compute_rho(13)

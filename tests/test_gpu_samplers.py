@@ -32,16 +32,26 @@ def points(conf, count=300, seed=5):
 
 @pytest.mark.parametrize("order", [4, 3, 5])
 def test_sample_field_against_oracle(order):
+    """A point whose wrapped position rounds to exactly Lx (knot Nx; here
+    x = -1e-17) makes the reference read
+    one coefficient past the slice (fields.hpp:46-52); the device reads the
+    periodic copy.  At t = 0 that coefficient weighs 0 except in the top
+    derivative (order 3, second derivative), so there the oracle is replaced
+    by its value at x_min (periodicity)."""
     conf, hist = setup(order)
     st = conf.stride_t(order)
     x, _ = points(conf)
+    y = x - conf.x_min                               # the reference's wrap, fields.hpp:42-46
+    y = y - conf.Lx * np.floor(y * conf.Lx_inv)
+    x_oracle = np.where(y == conf.Lx, conf.x_min, x)
+    assert np.sum(y == conf.Lx) >= 1                 # -1e-17 lands on the edge
     with nufi.GpuScheduler(conf, order=order) as s:
         s.upload_history(hist, 0, conf.Nt + 1)
         for n in [0, 10, conf.Nt]:
             E, rho = s.sample_field(n, x)
             sl = hist[n * st:(n + 1) * st]
             E_ref = np.array([-O.field1d(conf, sl, xi, 1, order=order) for xi in x])
-            r_ref = np.array([-O.field1d(conf, sl, xi, 2, order=order) for xi in x])
+            r_ref = np.array([-O.field1d(conf, sl, xi, 2, order=order) for xi in x_oracle])
             eE = np.max(np.abs(E - E_ref)) / np.max(np.abs(E_ref))
             eR = np.max(np.abs(rho - r_ref)) / np.max(np.abs(r_ref))
             print(f"[samplers] order={order} n={n}: E err {eE:.2e}, -phi'' err {eR:.2e}", flush=True)
@@ -63,7 +73,9 @@ def test_sample_f_and_phase_flow_against_oracle():
             ev = np.max(np.abs(vo - ref[:, 1])) / np.max(np.abs(ref[:, 1]))
             print(f"[samplers] n={n}: f err {ef:.2e}, flow x err {ex:.2e}, v err {ev:.2e}", flush=True)
             assert ef <= 1e-12 and ex <= 1e-12 and ev <= 1e-12
-            assert np.all((xo >= 0) & (xo < conf.Lx))
+            # x - Lx floor(x / Lx) gives exactly Lx when x / Lx rounds just below 1
+            # (here x = x_min + Lx); the reference's wrap has that edge too
+            assert np.all((xo >= 0) & (xo <= conf.Lx))
             if n <= 1:                               # not moved (rho.hpp:102)
                 assert np.array_equal(vo, v)
 

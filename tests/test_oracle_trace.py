@@ -211,3 +211,19 @@ def test_phase_flow_pins():
     for n in [0, 1]:
         xf, uf = O.phase_flow1d(conf, n, conf.Lx + 0.25, 0.7, hist)
         assert uf == 0.7 and abs(xf - 0.25) < 1e-15
+
+
+def test_field_second_derivative_pins():
+    """field1d with derivative 2 (the -phi'' of the reference's rho plots,
+    bin/test_nufi_gpu_1d.cpp:298) against a central difference of the first
+    derivative and, for a single-mode potential, the closed form -kappa^2 phi."""
+    conf = nufi.landau_1d(Nx=64, Nu=16, T=1.0)
+    N, L = conf.Nx, conf.Lx
+    kap = 2 * math.pi * 3 / L
+    sl = spline_coeffs_of(np.cos(kap * np.arange(N) * conf.dx), N)
+    for x in [0.37, 5.2, 11.9]:
+        d2 = O.field1d(conf, sl, x, 2)
+        h = 1e-5
+        fd = (O.field1d(conf, sl, x + h, 1) - O.field1d(conf, sl, x - h, 1)) / (2 * h)
+        assert abs(d2 - fd) < 1e-6 * kap * kap
+        assert abs(d2 + kap * kap * math.cos(kap * x)) < 5e-3 * kap * kap     # spline error at 21 points per wave

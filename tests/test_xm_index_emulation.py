@@ -24,7 +24,8 @@ def xm_setup(Nx, Nu, ix_count):
 
 
 def shape_ok(block, npt):
-    return block in (64, 128, 256, 512, 1024) and npt in (1, 2, 4) and ((block // WAVE) * npt) % XM_GROUP == 0
+    return (block in (64, 128, 256, 512, 1024) and npt in (1, 2, 4) and ((block // WAVE) * npt) % XM_GROUP == 0
+            and not (block == 1024 and npt == 4))           # would exceed 128 VGPRs
 
 
 def emulate(Nx, Nu, q_begin, q_end, block, npt):
@@ -68,7 +69,7 @@ def emulate(Nx, Nu, q_begin, q_end, block, npt):
                 continue
             col = [fbuf[(gi * XM_GROUP + v) * xw + xo] for v in range(XM_GROUP)]
             ix = ix_first + xl
-            assert ix * ppx + gg < Nx * ppx
+            assert gg * Nx + ix < ppx * Nx             # partials[group*Nx + x]
             written[ix, gg] += 1
             sums[ix, gg] = sum(col)
     # every node of the range traced exactly once, nothing else
@@ -90,7 +91,7 @@ SHAPES = [(b, n) for b, n in itertools.product((64, 128, 256, 512, 1024), (1, 2,
 
 
 def test_shape_rule():
-    assert set(SHAPES) == {(256, 4), (512, 2), (512, 4), (1024, 1), (1024, 2), (1024, 4)}
+    assert set(SHAPES) == {(256, 4), (512, 2), (512, 4), (1024, 1), (1024, 2)}
 
 
 @pytest.mark.parametrize("Nx,Nu", [(256, 1024), (64, 256), (128, 512), (32, 128), (48, 200), (16, 32),
@@ -163,7 +164,7 @@ def emulate3(n_space, n_vel, q_begin, q_end, block, npt):
                     if ok and iv < n_vel and q_begin <= q < q_end:
                         traced[l, iv] += 1
             if ok:
-                assert l * ppx + g < n_space * ppx
+                assert g * n_space + l < ppx * n_space  # partials[group*n_space + l]
                 written[l, g] += 1
     q = np.arange(n_space * n_vel).reshape(n_space, n_vel)
     want = ((q >= q_begin) & (q < q_end)).astype(np.int64)
@@ -181,7 +182,7 @@ def test_xm3_vgroup_rule():
 @pytest.mark.parametrize("n_space,n_vel", [(512, 512), (64, 100), (100, 64), (27, 343)])
 def test_xm3_full_and_sub_ranges(n_space, n_vel):
     N = n_space * n_vel
-    for block, npt in [(256, 1), (256, 4), (512, 2), (1024, 4)]:
+    for block, npt in [(256, 1), (256, 4), (512, 2), (512, 4)]:
         emulate3(n_space, n_vel, 0, N, block, npt)
     for a, b in [(0, 1), (N - 1, N), (n_vel - 3, 3 * n_vel + 5), (N // 3, N // 2)]:
         emulate3(n_space, n_vel, a, b, 256, 2)
