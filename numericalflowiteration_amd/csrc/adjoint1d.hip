@@ -11,10 +11,11 @@
 //   per node: fbar = -du g_n[i]; trace forward to the foot; dJ/da += fbar df0/da;
 //   then walk back through slices 0 .. n-1, undoing each Stormer-Verlet step
 //   as it was taken (the states are recomputed, never stored), with the
-//   kick polynomial p(t) = Q0 + Q1 t + Q2 t^2 of the forward kernel:
-//     forward  s_a = s_b - cd u_b,  u_a = u_b + h p(s_a; S_m)
-//     reverse  pbar = h ubar;  sbar += pbar dp/dt;  lambda_m += pbar dp/dS_m;
-//              u_b = u_a - h p;  ubar += -cd sbar;  s_b = s_a + cd u_b
+//   polynomial p(t) = Q0 + Q1 t + Q2 t^2 of 2 dphi/dt of the forward kernel
+//   (hk = h kick, h = 1, or 1/2 on slice 0):
+//     forward  s_a = s_b - cd u_b,  u_a = u_b + hk p(s_a; S_m)
+//     reverse  pbar = hk ubar;  sbar += pbar dp/dt;  lambda_m += pbar dp/dS_m;
+//              u_b = u_a - hk p;  ubar += -cd sbar;  s_b = s_a + cd u_b
 // lambda_m accumulates in LDS per workgroup (all nodes of a workgroup visit the
 // same slice at the same time) and is flushed with global atomics once per
 // chunk of slices; its summation order therefore varies at the rounding level.
@@ -115,7 +116,8 @@ void adjoint1d_kernel( adjoint1_params p )
             if ( POW2 && poly_wrap_due( mlo + jj ) ) rewrap( s, Nxd, invNx );
             const int jstop = POW2 ? int( poly_segment_end( mlo + jj, mlo ) - mlo ) : -1;
             for ( ; jj > jstop; --jj )
-                poly_step<POW2>( s, u, Ql + jj * sq, q2, cd, ( mlo + jj == 0 ) ? 0.5 : 1.0, mask, Nxd, invNx );
+                poly_step<POW2>( s, u, Ql + jj * sq, Ql + jj * sq + q2, cd, ( mlo + jj == 0 ) ? 0.5 * kick : kick, mask,
+                                 Nxd, invNx );
         }
     }
 
@@ -144,19 +146,19 @@ void adjoint1d_kernel( adjoint1_params p )
         __syncthreads();
         for ( int jj = 0; jj < cnt; ++jj )
         {
-            const double h = ( mlo + jj == 0 ) ? 0.5 : 1.0;
+            const double hk = ( mlo + jj == 0 ) ? 0.5 * kick : kick;
             // the cell and t the forward step used at this position
             const double k = floor( s );
             const double t = s - k;
             const int i = POW2 ? ( int( k ) & mask ) : int( k );
             double dpdt;
             const double pv = poly_eval( Ql + jj * sq, q2, i, t, dpdt );
-            const double pbar = h * ubar;
+            const double pbar = hk * ubar;
             sbar = fma( pbar, dpdt, sbar );
             if ( active )
             {
-                // Q = kick (q0, q1, q2); q0 = D0 + D1, q1 = 2 (D1 - D0), q2 = D0 - 2 D1 + D2
-                const double q0 = kick * pbar, q1 = kick * pbar * t, q2 = kick * pbar * ( t*t );
+                // adjoints of q0 = D0 + D1, q1 = 2 (D1 - D0), q2 = D0 - 2 D1 + D2
+                const double q0 = pbar, q1 = pbar * t, q2 = pbar * ( t*t );
                 const double d0 = q0 - 2.0*q1 + q2, d1 = q0 + 2.0*q1 - 2.0*q2, d2 = q2;
                 double *L = lam + jj * p.Nx;
                 const int i0 = ( i + Nx ) % Nx;
@@ -166,7 +168,7 @@ void adjoint1d_kernel( adjoint1_params p )
                 atomicAdd( L + ( i0 + 3 ) % Nx, d2 );
             }
             // undo the step
-            u = fma( -h, pv, u );
+            u = fma( -hk, pv, u );
             ubar = fma( -cd, sbar, ubar );
             s = fma( cd, u, s );
             if ( !POW2 || poly_wrap_due( mlo + jj ) ) rewrap( s, Nxd, invNx );

@@ -101,8 +101,10 @@ int apply_shape( nufi_batch *b, const launch_shape &s )
         batch_run &h = b->h_runs[r];
         trace1_params q = h.p;
         q.n = h.Nt;                                    // the chunk bound of the longest step
-        h.p.chunk = trace1d_chunk( q, b->variant, s );
-        lds = std::max( lds, size_t( h.p.chunk ) * b->sq[r] * elem );
+        bool pipe = false;
+        h.p.chunk = trace1d_chunk( q, b->variant, s, 65536, &pipe );
+        h.p.pipe = pipe ? 1u : 0u;
+        lds = std::max( lds, ( pipe ? 2 : 1 ) * size_t( h.p.chunk ) * b->sq[r] * elem );
         h.wg_begin = uint32_t( wg_run.size() );
         h.n_wg = trace1d_grid( h.p, b->variant, s );
         wg_run.insert( wg_run.end(), h.n_wg, uint32_t( r ) );

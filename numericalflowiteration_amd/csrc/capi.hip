@@ -162,9 +162,27 @@ uint32_t rho_ppx( const nufi_ctx *c )
     return c->variant != TRACE_STRICT ? c->ppx : c->wpx;
 }
 
+// The chunk a rho launch of shape s uses and its LDS: 1D x-major launches
+// plan the requested chunk within 160 KiB (poly_chunk_plan).
+int effective_chunk( const nufi_ctx *c, const launch_shape &s, size_t *lds = nullptr )
+{
+    const size_t elem = ( c->variant == TRACE_FP32C ) ? sizeof(float) : sizeof(double);
+    const size_t row = c->dim == 1 ? ( c->variant == TRACE_STRICT ? c->sd : c->sq ) : c->st;
+    if ( c->dim != 1 || c->variant == TRACE_STRICT )
+    {
+        if ( lds ) *lds = size_t( s.chunk ) * row * elem;
+        return s.chunk;
+    }
+    bool pipe = false;
+    const uint32_t ch = poly_chunk_plan( uint32_t( c->conf.Nx ), s.block, elem, uint64_t( s.chunk ), 160 * 1024, &pipe );
+    if ( lds ) *lds = ( pipe ? 2 : 1 ) * size_t( ch ) * row * elem;
+    return int( ch );
+}
+
 // Default launch shapes (the autotuner refines them): 1D x-major 1024 threads
-// x 1 node (16 velocity rows per workgroup), chunk by the LDS budget; 3D
-// 512 threads x 2 (FP64) or x 4 (FP32C) nodes.
+// x 1 node (16 velocity rows per workgroup), the largest chunk the register
+// prefetch and 160 KiB of LDS hold; 3D 512 threads x 2 (FP64) or x 4 (FP32C)
+// nodes.
 void default_shape( nufi_ctx *c )
 {
     const uint64_t waves = c->n_space * c->wpx;
@@ -182,8 +200,12 @@ void default_shape( nufi_ctx *c )
             block = block >= 1024 ? 1024 : ( block >= 256 ? 256 : 64 );
             c->shape = launch_shape { block, 1, chunk };
         }
-        else
-            c->shape = launch_shape { 1024, 1, chunk };   // x-major: 16 velocity rows per workgroup
+        else    // x-major: 16 velocity rows per workgroup, the largest chunk planned
+        {
+            bool pipe = false;
+            c->shape = launch_shape { 1024, 1, int( poly_chunk_plan( uint32_t( c->conf.Nx ), 1024, elem, 1u << 20,
+                                                                    std::min<size_t>( c->lds_cap, 160 * 1024 ), &pipe ) ) };
+        }
     }
     else
     {
@@ -912,9 +934,8 @@ int nufi_set_launch_shape( nufi_ctx *c, int block, int npt, int chunk )
     const launch_shape s { block, npt, chunk };
     const bool ok = c->dim == 1 ? trace1d_shape_ok( c->variant, c->order, s )
                                 : trace3d_shape_ok( c->variant, c->order, s );
-    const size_t elem = ( c->variant == TRACE_FP32C ) ? sizeof(float) : sizeof(double);
-    const size_t row = c->dim == 1 ? ( c->variant == TRACE_STRICT ? c->sd : c->sq ) : c->st;
-    const size_t lds = size_t( chunk ) * row * elem;
+    size_t lds = 0;
+    if ( ok ) (void)effective_chunk( c, s, &lds );
     if ( !ok || lds > c->lds_cap || lds > 160u * 1024u )
         return fail( c, NUFI_ERR_UNSUPPORTED, "set_launch_shape: shape not built or LDS too large" );
     c->shape = s;
@@ -943,7 +964,7 @@ int nufi_autotune( nufi_ctx *c, uint64_t n_probe, int repeats, int *block, int *
         if ( c->variant == TRACE_STRICT )
         { for ( int b : { 256, 1024 } ) for ( int ch : { 16, 32, 64 } ) cand.push_back( { b, 1, ch } ); }
         else    // x-major: trace1d_shape_ok keeps the shapes with whole summation groups
-            for ( int b : blocks1 ) for ( int np : npts ) for ( int ch : { 4, 8, 16, 32, 64 } ) cand.push_back( { b, np, ch } );
+            for ( int b : blocks1 ) for ( int np : npts ) for ( int ch : { 4, 8, 16, 32, 1 << 20 } ) cand.push_back( { b, np, ch } );
     }
     else
     {
@@ -958,13 +979,19 @@ int nufi_autotune( nufi_ctx *c, uint64_t n_probe, int repeats, int *block, int *
     HIP_TRY( c, hipEventCreate( &e1 ) );
     double best = 1e300;
     launch_shape best_s = saved;
-    const size_t elem = ( c->variant == TRACE_FP32C ) ? sizeof(float) : sizeof(double);
-    const size_t per = size_t( c->dim == 1 ? ( c->variant == TRACE_STRICT ? c->sd : c->sq ) : c->st ) * elem;
     int rc = NUFI_OK;
+    std::vector<launch_shape> tried;
     for ( const launch_shape &sh : cand )
     {
         const bool ok = c->dim == 1 ? trace1d_shape_ok( c->variant, c->order, sh ) : trace3d_shape_ok( c->variant, c->order, sh );
-        if ( !ok || size_t( sh.chunk ) * per > std::min<size_t>( c->lds_cap, 160 * 1024 ) ) continue;
+        size_t lds = 0;
+        if ( ok ) (void)effective_chunk( c, sh, &lds );
+        if ( !ok || lds > std::min<size_t>( c->lds_cap, 160 * 1024 ) ) continue;
+        bool seen = false;      // x-major chunks above the plan run as the plan: time each launch once
+        for ( const launch_shape &t : tried )
+            seen = seen || ( t.block == sh.block && t.npt == sh.npt && effective_chunk( c, t ) == effective_chunk( c, sh ) );
+        if ( seen ) continue;
+        tried.push_back( sh );
         c->shape = sh;
         uint64_t f, n;
         if ( ( rc = launch_trace( c, n_probe, 0, c->n_nodes, &f, &n ) ) ) break;    // warm-up

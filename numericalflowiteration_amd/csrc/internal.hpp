@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include <string>
 #include <vector>
 
@@ -60,6 +62,7 @@ struct trace1_params
     const double *dhist;     // extended differences of slice m at dhist + m*stride_d (FP64)
     uint32_t stride_q, q2_off;        // LDS kick-polynomial table: slice stride, Q2 row (poly_stride, poly_q2)
     double   kick;                    // -dt/(2 dx), the scale of the kick polynomials
+    uint32_t pipe;                    // x-major: tables double-buffered and prefetched (poly_pipeline)
     double       *partials;  // velocity-major: partials[j*Nx + ix], j < wpx; x-major: j < ppx;
                              // MODE_METRICS: partials[4*g + k]
     uint64_t n;
@@ -140,11 +143,12 @@ size_t trace3d_lds_bytes( const trace3_params &p, int variant, const launch_shap
 bool trace1d_shape_ok( int variant, int order, const launch_shape &s );
 // x-major fields (xw, vpw, n_xt, ppx) from Nx, Nu and ix_count, the grid of
 // a launch (x-major or wave-task decomposition, by variant and mode) and the
-// slices per LDS chunk it uses (bounded by the step and, x-major, by the
-// register prefetch of the table: block * poly_maxc(block) entries)
+// slices per LDS chunk it uses (bounded by the step and, x-major, planned by
+// poly_chunk_plan within lds_max bytes, *pipe telling which loop it takes)
 void trace1d_xm_setup( trace1_params &p );
 uint32_t trace1d_grid( const trace1_params &p, int variant, const launch_shape &s );
-uint32_t trace1d_chunk( const trace1_params &p, int variant, const launch_shape &s );
+uint32_t trace1d_chunk( const trace1_params &p, int variant, const launch_shape &s, size_t lds_max = 160 * 1024,
+                        bool *pipe = nullptr );
 bool trace3d_shape_ok( int variant, int order, const launch_shape &s );
 
 // ---- adjoint (adjoint1d.hip) ---------------------------------------------------
@@ -257,16 +261,33 @@ hipError_t launch_slice_tables( int dim, const double *slice, uint32_t len, uint
 
 // The 1D kick-polynomial table of one slice, built in LDS by the fast, FP32C
 // and adjoint kernels from the difference history: for cell i (i = -1 .. Nx,
-// periodic; entry j = i+1) the coefficients of kick * 2 dphi/dt =
-// Q0 + Q1 t + Q2 t^2 on [0,1), with Q0 = kick (D0 + D1), Q1 = kick 2 (D1 - D0),
-// Q2 = kick ((D0 - D1) + (D2 - D1)) from the differences D_l = c[i+l+1] -
-// c[i+l] of the cell's window.  Layout: the pairs (Q0, Q1) at 2j, 2j+1, then
-// the Q2 row at poly_q2 + j; the stride is a multiple of 4 elements so that
-// every slice starts 16-byte aligned in FP64 and FP32.
-// Table entries of one chunk held in registers per thread while it is loaded.
-__host__ __device__ constexpr int poly_maxc( int block ) { return block >= 1024 ? 4 : 8; }
+// periodic; entry j = i+1) the coefficients of 2 dphi/dt =
+// Q0 + Q1 t + Q2 t^2 on [0,1), with Q0 = D0 + D1, Q1 = 2 (D1 - D0),
+// Q2 = (D0 - D1) + (D2 - D1) from the differences D_l = c[i+l+1] - c[i+l] of
+// the cell's window (the FP32 table scaled by the kick -dt/(2 dx)).  Layout:
+// the pairs (Q0, Q1) at 2j, 2j+1, then the Q2 row at poly_q2 + j; the stride
+// is a multiple of 4 elements so that every slice starts 16-byte aligned in
+// FP64 and FP32.
+// Table windows of one chunk held in registers per thread while it is loaded
+// (the pipelined loop is unrolled over them): 6144 per 1024-thread workgroup,
+// 23 slices at Nx = 256.
+__host__ __device__ constexpr int poly_maxc( int /*block*/ ) { return 6; }
 __host__ __device__ inline uint32_t poly_q2( uint32_t Nx ) { return 2u * ( Nx + 2u ); }
 __host__ __device__ inline uint32_t poly_stride( uint32_t Nx ) { return ( 3u * ( Nx + 2u ) + 3u ) & ~3u; }
+// The table chunk of a 1D x-major launch for a requested chunk `req` >= 1:
+// the largest chunk up to req whose windows fit the register prefetch of a
+// workgroup of `block` threads and whose two table buffers (`elem`-byte
+// entries) fit `lds_max` bytes, pipelined; failing that, one buffer of up to
+// req slices built directly (*pipe = false).  At least 1.
+inline uint32_t poly_chunk_plan( uint32_t Nx, int block, size_t elem, uint64_t req, size_t lds_max, bool *pipe )
+{
+    const uint64_t row = uint64_t( poly_stride( Nx ) ) * elem;
+    const uint64_t pf  = uint64_t( block ) * uint64_t( poly_maxc( block ) ) / ( uint64_t( Nx ) + 2u );
+    uint64_t c = std::min( { req, pf, uint64_t( lds_max ) / ( 2 * row ) } );
+    *pipe = c >= 1;
+    if ( c == 0 ) c = std::max<uint64_t>( 1, std::min<uint64_t>( req, uint64_t( lds_max ) / row ) );
+    return uint32_t( std::min<uint64_t>( c, 0xffffffffu ) );
+}
 
 } // namespace nufi_impl
 

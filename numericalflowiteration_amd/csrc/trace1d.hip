@@ -304,7 +304,9 @@ void trace1d_strict_kernel( trace1_params p )
 
 // ---- fast FP64 with the kick-polynomial table (order 4, rho) ----------------
 // Per char-step: drift (FMA), floor, index, one 16-byte and one 8-byte LDS read,
-// two FMAs for Q0 + t (Q1 + t Q2), one FMA for the kick.
+// two FMAs for Q0 + t (Q1 + t Q2), one FMA for the kick.  The tables are built
+// in LDS from the difference history, double-buffered (poly_pipeline) when
+// p.pipe is set.
 template <int BLOCK, int NPT, bool POW2>
 __device__ __forceinline__ void trace1d_poly_body( const trace1_params &p, uint32_t wg, double *lds )
 {
@@ -321,35 +323,43 @@ __device__ __forceinline__ void trace1d_poly_body( const trace1_params &p, uint3
     }
     const double Nxd = double( p.Nx ), invNx = 1.0 / Nxd;
     const int mask = int( p.Nx ) - 1;
-    const double cd = p.dt * p.dx_inv;
+    const double cd = p.dt * p.dx_inv, kick = p.kick;
     const uint32_t sq = p.stride_q, q2 = p.q2_off, cells = p.Nx + 2;
-    poly_fetch<BLOCK,poly_maxc( BLOCK )> pf;
-    poly_prefetch_first( pf, p.dhist, p.stride_d, cells, int64_t( p.n ) - 1, p.chunk );
 
-    for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
+    // slices mlo+hi down to mlo+lo, in segments between re-wrap points so that
+    // the step loop itself carries no wrap; slice 0 (the half kick) peeled
+    auto steps = [&]( const double *Qc, int mlo, int hi, int lo )
     {
-        const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
-        const int cnt = int( mhi - mlo + 1 );
-        poly_chunk_begin( pf, lds, p.dhist, p.stride_d, sq, q2, cells, p.kick, mhi, p.chunk );
-        // segments between re-wrap points, so the step loop itself carries no wrap
-        for ( int jj = cnt - 1; jj >= 0; )
+        int m = mlo + hi;
+        const int mend = mlo + lo;
+        const double *Q = Qc + hi * sq, *Q2 = Q + q2;
+        while ( m >= mend )
         {
-            const int64_t m = mlo + jj;
-            if ( POW2 && poly_wrap_due( m ) )
+            if ( POW2 && ( m & ( POLY_WRAP - 1 ) ) == POLY_WRAP - 1 )
             {
                 #pragma unroll
                 for ( int r = 0; r < NPT; ++r ) rewrap( s[r], Nxd, invNx );
             }
-            const int jstop = POW2 ? int( poly_segment_end( m, mlo ) - mlo ) : -1;
-            for ( ; jj > jstop; --jj )
+            // slices m .. seg need no re-wrap (the next one is below seg)
+            const int seg = POW2 ? ( ( m & ~( POLY_WRAP - 1 ) ) > mend ? ( m & ~( POLY_WRAP - 1 ) ) : mend ) : mend;
+            const int s1 = seg > 0 ? seg : 1;
+            #pragma unroll 2
+            for ( ; m >= s1; --m, Q -= sq, Q2 -= sq )
             {
-                const double *Q = lds + jj * sq;
-                const double h = ( mlo + jj == 0 ) ? 0.5 : 1.0;
                 #pragma unroll
-                for ( int r = 0; r < NPT; ++r ) poly_step<POW2>( s[r], u[r], Q, q2, cd, h, mask, Nxd, invNx );
+                for ( int r = 0; r < NPT; ++r ) poly_step<POW2>( s[r], u[r], Q, Q2, cd, kick, mask, Nxd, invNx );
+            }
+            if ( seg == 0 && m == 0 )
+            {
+                #pragma unroll
+                for ( int r = 0; r < NPT; ++r ) poly_step<POW2>( s[r], u[r], Q, Q2, cd, 0.5 * kick, mask, Nxd, invNx );
+                --m; Q -= sq; Q2 -= sq;
             }
         }
-    }
+    };
+    if ( p.pipe ) poly_pipeline<BLOCK,poly_maxc( BLOCK )>( lds, p.dhist, p.stride_d, sq, q2, cells, kick, p.n, p.chunk, steps );
+    else          poly_direct<BLOCK>( lds, p.dhist, p.stride_d, sq, q2, cells, kick, p.n, p.chunk, steps );
+
     double f[NPT];
     #pragma unroll
     for ( int r = 0; r < NPT; ++r ) f[r] = id[r].active ? f0_1d( p.f0, p.x_min + s[r]*p.dx, u[r] ) : 0.0;
@@ -388,47 +398,48 @@ __device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint
     const double cd   = p.dt * p.dx_inv;
     const float cd_hi = float( cd ), cd_lo = float( cd - double( cd_hi ) );
     const uint32_t sq = p.stride_q, q2 = p.q2_off, cells = p.Nx + 2;
-    poly_fetch<BLOCK,poly_maxc( BLOCK )> pf;
-    poly_prefetch_first( pf, p.dhist, p.stride_d, cells, int64_t( p.n ) - 1, p.chunk );
 
-    for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
+    auto step = [&]( const float *Q, float hf )
     {
-        const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
-        const int cnt = int( mhi - mlo + 1 );
-        poly_chunk_begin( pf, ldsf, p.dhist, p.stride_d, sq, q2, cells, p.kick, mhi, p.chunk );
-        for ( int jj = cnt - 1; jj >= 0; --jj )
+        #pragma unroll
+        for ( int r = 0; r < NPT; ++r )
         {
-            const float *Q = ldsf + jj * sq;
-            const float hf = ( mlo + jj == 0 ) ? 0.5f : 1.0f;
-            #pragma unroll
-            for ( int r = 0; r < NPT; ++r )
-            {
-                // drift: s -= cd*u with TwoProd on the leading product
-                const float pr = cd_hi * uh[r];
-                float pe = fmaf( cd_hi, uh[r], -pr );
-                pe = fmaf( cd_hi, ul[r], fmaf( cd_lo, uh[r], pe ) );
-                float y, e;
-                two_sum( sh[r], -pr, y, e );
-                e = e + ( sl[r] - pe );
-                fast_two_sum( y, e, sh[r], sl[r] );
-                // exact periodic wrap into [0, Nx)
-                const float w = floorf( sh[r] * invNx );
-                float y2, e2;
-                fast_two_sum( -Nxf * w, sh[r], y2, e2 );
-                sh[r] = y2; sl[r] = sl[r] + e2;
-                // locate
-                const float k = floorf( sh[r] );
-                const float t = ( sh[r] - k ) + sl[r];
-                const int i = int( k );                   // in [-1, Nx]
-                const float2 q01 = reinterpret_cast<const float2*>( Q )[ i + 1 ];
-                const float kq = fmaf( fmaf( Q[ q2 + ( i + 1 ) ], t, q01.y ), t, q01.x );
-                // kick: u += h*kq with TwoSum
-                float y3, e3;
-                two_sum( uh[r], hf * kq, y3, e3 );
-                uh[r] = y3; ul[r] = ul[r] + e3;
-            }
+            // drift: s -= cd*u with TwoProd on the leading product
+            const float pr = cd_hi * uh[r];
+            float pe = fmaf( cd_hi, uh[r], -pr );
+            pe = fmaf( cd_hi, ul[r], fmaf( cd_lo, uh[r], pe ) );
+            float y, e;
+            two_sum( sh[r], -pr, y, e );
+            e = e + ( sl[r] - pe );
+            fast_two_sum( y, e, sh[r], sl[r] );
+            // exact periodic wrap into [0, Nx)
+            const float w = floorf( sh[r] * invNx );
+            float y2, e2;
+            fast_two_sum( -Nxf * w, sh[r], y2, e2 );
+            sh[r] = y2; sl[r] = sl[r] + e2;
+            // locate
+            const float k = floorf( sh[r] );
+            const float t = ( sh[r] - k ) + sl[r];
+            const int i = int( k );                   // in [-1, Nx]
+            const float2 q01 = reinterpret_cast<const float2*>( Q )[ i + 1 ];
+            const float kq = fmaf( fmaf( Q[ q2 + ( i + 1 ) ], t, q01.y ), t, q01.x );
+            // kick: u += h*kq with TwoSum
+            float y3, e3;
+            two_sum( uh[r], hf * kq, y3, e3 );
+            uh[r] = y3; ul[r] = ul[r] + e3;
         }
-    }
+    };
+    // slices mlo+hi down to mlo+lo; slice 0 (the half kick) peeled
+    auto steps = [&]( const float *Qc, int mlo, int hi, int lo )
+    {
+        int m = mlo + hi;
+        const int mend = mlo + lo, s1 = mend > 0 ? mend : 1;
+        const float *Q = Qc + hi * sq;
+        for ( ; m >= s1; --m, Q -= sq ) step( Q, 1.0f );
+        if ( mend == 0 && m == 0 ) step( Q, 0.5f );
+    };
+    if ( p.pipe ) poly_pipeline<BLOCK,poly_maxc( BLOCK )>( ldsf, p.dhist, p.stride_d, sq, q2, cells, p.kick, p.n, p.chunk, steps );
+    else          poly_direct<BLOCK>( ldsf, p.dhist, p.stride_d, sq, q2, cells, p.kick, p.n, p.chunk, steps );
 
     double f[NPT];
     #pragma unroll
@@ -556,7 +567,7 @@ hipError_t launch_batch_block( const batch_run *runs, const uint32_t *wg_run, ui
 hipError_t launch_trace1d_batch( const batch_run *runs, const uint32_t *wg_run, uint32_t n_wg, uint64_t n,
                                  int variant, const launch_shape &s, size_t lds, hipStream_t st )
 {
-    if ( variant != TRACE_FAST && variant != TRACE_FP32C ) return hipErrorInvalidValue;
+    if ( ( variant != TRACE_FAST && variant != TRACE_FP32C ) || n >= ( uint64_t( 1 ) << 31 ) ) return hipErrorInvalidValue;
     if ( n_wg == 0 ) return hipSuccess;
     switch ( s.block )
     {
@@ -607,17 +618,15 @@ void trace1d_xm_setup( trace1_params &p )
     p.ppx  = xm_ppx( p.Nu );
 }
 
-uint32_t trace1d_chunk( const trace1_params &p, int variant, const launch_shape &s )
+uint32_t trace1d_chunk( const trace1_params &p, int variant, const launch_shape &s, size_t lds_max, bool *pipe )
 {
     uint64_t c = uint64_t( s.chunk );
+    bool pp = false;
     if ( x_major( p, variant ) )
-    {
-        // a chunk's table entries fit the register prefetch of the workgroup
-        const uint64_t cap = uint64_t( s.block ) * uint64_t( poly_maxc( s.block ) ) / ( p.Nx + 2u );
-        c = std::min<uint64_t>( c, std::max<uint64_t>( cap, 1 ) );
-    }
+        c = poly_chunk_plan( p.Nx, s.block, variant == TRACE_FP32C ? sizeof(float) : sizeof(double), c, lds_max, &pp );
     const uint64_t v = p.mode == MODE_METRICS ? ( p.n > 0 ? p.n + 1 : 0 ) : p.n;
     if ( v < c ) c = v > 0 ? v : 1;
+    if ( pipe ) *pipe = pp;
     return uint32_t( c );
 }
 
@@ -643,7 +652,9 @@ size_t trace1d_slice_elems( const trace1_params &p, int variant )
 size_t trace1d_lds_bytes( const trace1_params &p, int variant, const launch_shape &s )
 {
     const size_t elem = ( variant == TRACE_FP32C && p.mode == MODE_RHO ) ? sizeof(float) : sizeof(double);
-    const size_t hist = size_t( trace1d_chunk( p, variant, s ) ) * trace1d_slice_elems( p, variant ) * elem;
+    bool pipe = false;
+    const uint32_t chunk = trace1d_chunk( p, variant, s, 160 * 1024, &pipe );
+    const size_t hist = ( pipe ? 2 : 1 ) * size_t( chunk ) * trace1d_slice_elems( p, variant ) * elem;
     // x-major: the f values of the workgroup (at most block*npt doubles) reuse the LDS
     const size_t fbuf = x_major( p, variant ) ? size_t( s.block ) * size_t( s.npt ) * sizeof(double) : 0;
     return hist > fbuf ? hist : fbuf;
@@ -651,7 +662,7 @@ size_t trace1d_lds_bytes( const trace1_params &p, int variant, const launch_shap
 
 hipError_t launch_trace1d( const trace1_params &p_in, int variant, const launch_shape &s, hipStream_t st )
 {
-    if ( !shape_built( variant, int( p_in.order ), s ) ) return hipErrorInvalidValue;
+    if ( !shape_built( variant, int( p_in.order ), s ) || p_in.n >= ( uint64_t( 1 ) << 31 ) ) return hipErrorInvalidValue;
     trace1_params p = p_in;
     if ( x_major( p, variant ) )
     {
@@ -661,7 +672,9 @@ hipError_t launch_trace1d( const trace1_params &p_in, int variant, const launch_
     }
     else if ( p.n_tasks == 0 ) return hipSuccess;
     const uint32_t grid = trace1d_grid( p, variant, s );
-    p.chunk = trace1d_chunk( p, variant, s );
+    bool pipe = false;
+    p.chunk = trace1d_chunk( p, variant, s, 160 * 1024, &pipe );
+    p.pipe = pipe ? 1u : 0u;
     const size_t lds = trace1d_lds_bytes( p, variant, s );
 
     if ( variant == TRACE_STRICT )

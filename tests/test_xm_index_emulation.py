@@ -189,29 +189,60 @@ def test_xm3_full_and_sub_ranges(n_space, n_vel):
 
 
 def poly_maxc(block):
-    return 4 if block >= 1024 else 8
+    return 6
 
 
-@pytest.mark.parametrize("Nx", [8, 16, 20, 48, 64, 100, 256, 1000, 5000])
+def poly_chunk_plan(Nx, block, elem, req, lds_max=160 * 1024):
+    """internal.hpp poly_chunk_plan: (chunk, pipelined)."""
+    row = ((3 * (Nx + 2) + 3) & ~3) * elem
+    c = min(req, block * poly_maxc(block) // (Nx + 2), lds_max // (2 * row))
+    if c >= 1:
+        return c, True
+    return max(1, min(req, lds_max // row)), False
+
+
+@pytest.mark.parametrize("Nx", [1, 2, 4, 8, 16, 20, 48, 64, 100, 256, 1000, 5000])
 def test_table_build_from_differences_stays_in_bounds(Nx):
-    """step1d.hpp poly_fetch / build_poly and trace1d_chunk: a chunk of cnt
-    slices has cnt*(Nx+2) table entries; entry j of slice jj reads difference
-    entries j..j+2 of row mlo+jj (row length stride_d = (Nx+7) & ~3 for order
-    4) and writes LDS at jj*sq + 2j, 2j+1 and jj*sq + q2 + j; the chunk of a
-    launch is capped so the prefetch (block * maxc entries) holds it, and a
-    chunk of one slice larger than that takes the direct path."""
+    """step1d.hpp poly_fetch / poly_pipeline and internal.hpp poly_chunk_plan:
+    window idx = tid + c*block of a chunk is slice jj = idx // cells, cell
+    j = idx % cells (cells = Nx+2); it reads difference entries goff..goff+2,
+    goff = jj*sd + j, of the chunk's rows (row length sd = (Nx+7) & ~3 for
+    order 4) and writes LDS at loff + j, loff + j + 1 (the pair, loff =
+    jj*sq + j) and loff + q2.  It belongs to a chunk of cnt slices iff
+    goff < cnt*sd; the others load the chunk's first window.  A pipelined
+    chunk fits the prefetch and two of its tables fit 160 KiB; otherwise one
+    table of the chunk (built directly) fits."""
     cells = Nx + 2
     sd = (Nx + 4 + 3) & ~3
     q2 = 2 * cells
     sq = (3 * cells + 3) & ~3
-    for block in (256, 512, 1024):
-        for want in (1, 4, 16, 64):
-            cap = block * poly_maxc(block) // cells
-            chunk = min(want, max(cap, 1))
-            fits = chunk * cells <= block * poly_maxc(block)
-            assert fits or chunk == 1
-            lds = chunk * sq
-            for j in (0, cells - 1):
-                assert j + 2 < sd
-                for jj in (0, chunk - 1):
-                    assert jj * sq + 2 * j + 1 < lds and jj * sq + q2 + j < lds
+    for elem in (8, 4):
+        for block in (64, 256, 512, 1024):
+            for want in (1, 4, 16, 64, 1 << 20):
+                chunk, pipe = poly_chunk_plan(Nx, block, elem, want)
+                fits = chunk * cells <= block * poly_maxc(block)
+                assert chunk <= want
+                if pipe:     # the pipelined loop: the prefetch holds a chunk, LDS two
+                    assert fits and 2 * chunk * sq * elem <= 160 * 1024
+                else:        # one buffer built directly
+                    assert chunk * sq * elem <= 160 * 1024 or chunk == 1
+                    continue
+                for cnt in sorted({1, chunk // 2 or 1, chunk}):
+                    hist = cnt * sd                       # the chunk's rows
+                    lds = cnt * sq
+                    written = set()
+                    for tid in range(block):
+                        for c in range(poly_maxc(block)):
+                            idx = tid + c * block
+                            jj, j = divmod(idx, cells)
+                            goff, loff = jj * sd + j, jj * sq + j
+                            member = goff < cnt * sd
+                            assert member == (idx < cnt * cells)
+                            g = goff if member else 0
+                            assert g + 2 < hist
+                            if member:
+                                for a in (loff + j, loff + j + 1, loff + q2):
+                                    assert a < lds and a not in written
+                                    written.add(a)
+                                assert loff + j + 1 < jj * sq + q2 and loff + q2 < (jj + 1) * sq
+                    assert len(written) == 3 * cnt * cells
