@@ -37,7 +37,7 @@ struct nufi_ctx
     uint32_t st = 0, sd = 0, sy = 0, sz = 0;
     uint64_t n_space = 0, n_vel = 0, n_nodes = 0;
     uint32_t wpx = 0;          // wave-task partials per spatial point (strict 1D, 3D)
-    uint32_t ppx = 0;          // x-major partials per spatial point (fast and FP32C)
+    uint32_t ppx = 0;          // x-major partials per spatial point (fast; 1D FP32C)
     uint32_t vgroup = 0;       // 3D x-major: velocities per partial
     int field_ok = 0;
     int n_cu = 256;
@@ -159,7 +159,8 @@ int build_tables( nufi_ctx *c )
 // partials per spatial point of the rho kernel in use
 uint32_t rho_ppx( const nufi_ctx *c )
 {
-    return c->variant != TRACE_STRICT ? c->ppx : c->wpx;
+    const bool xm = c->variant == TRACE_FAST || ( c->dim == 1 && c->variant == TRACE_FP32C );
+    return xm ? c->ppx : c->wpx;
 }
 
 // The chunk a rho launch of shape s uses and its LDS: 1D x-major launches
@@ -213,7 +214,7 @@ void default_shape( nufi_ctx *c )
         int chunk = int( std::min<size_t>( c->lds_cap, 64 * 1024 ) / per );
         chunk = std::max( 1, std::min( chunk, 1 ) );
         if ( c->variant == TRACE_STRICT ) c->shape = launch_shape { 256, 1, chunk };
-        else if ( c->variant == TRACE_FP32C ) c->shape = launch_shape { 512, 4, chunk };
+        else if ( c->variant == TRACE_FP32C ) c->shape = launch_shape { 512, 4, per * 2 <= 65536 ? 2 : 1 };
         else c->shape = launch_shape { 512, 2, chunk };
     }
 }

@@ -81,10 +81,10 @@ struct trace1_params
 
 // ---- 3D trace ----------------------------------------------------------------
 // Node q = l*Nvel + iv with l = (iz*Ny + iy)*Nx + ix and iv = (iw*Nv + iv')*Nu
-// + iu (cuda_kernel.cu:399-412).  The strict and metrics kernels use the
-// wave-task cut of the 1D case with Nu -> Nvel; the fast and FP32C rho
-// kernels the x-major layout of trace3d.hip (64 neighbouring spatial points
-// per wave, partials[(iv/vgroup)*n_space + l]).
+// + iu (cuda_kernel.cu:399-412).  The strict, FP32C and metrics kernels use
+// the wave-task cut of the 1D case with Nu -> Nvel (partials[j*n_space + l],
+// j = iv/64); the fast rho kernel the x-major layout of trace3d.hip (64
+// neighbouring spatial points per wave, partials[(iv/vgroup)*n_space + l]).
 // Velocities per x-major partial: 256, halved (down to 16) until the launch
 // has at least 8192 waves; fixed per grid, so results do not depend on the shape.
 inline uint32_t xm3_vgroup( uint64_t n_space, uint64_t n_vel )

@@ -8,13 +8,16 @@
 //
 // The workgroup streams coefficient slices through LDS (one slice is
 // (Nx+3)(Ny+3)(Nz+3) reals: 6859 = 54.9 KB FP64 at 16^3) and every node
-// gathers its 4x4x4 window from LDS.  The fast and FP32C rho kernels lay a
-// wave's lanes over 64 neighbouring spatial nodes at one velocity (x-major,
-// below), so a wave's windows overlap; each lane sums f over a fixed group
-// of consecutive velocities.  The strict and metrics kernels keep the
+// gathers its 4x4x4 window from LDS.  The fast FP64 rho kernel lays a wave's
+// lanes over 64 neighbouring spatial nodes at one velocity (x-major, below),
+// so a wave's windows overlap; each lane sums f over a fixed group of
+// consecutive velocities.  The FP32C, strict and metrics kernels keep the
 // reference's velocity-major order: a wave takes 64 velocity nodes of one
-// spatial point and reduces them with a compensated butterfly.  The partial
-// sums are added in a fixed order by the field kernel.
+// spatial point and reduces them with a compensated butterfly.  (With half
+// the LDS bytes per gather, the FP32C trace is bound by the VALU, not the
+// LDS, and velocity-major stages each slice once per workgroup instead of
+// once per velocity pass.)  The partial sums are added in a fixed order by
+// the field kernel and do not depend on the launch shape.
 //
 // Variants:
 //   strict : the reference's arithmetic (three separate tensor evaluations,
@@ -412,13 +415,31 @@ __device__ __forceinline__ void kick32( float &uh, float &ul, float q )
     uh = y; ul = ul + e;
 }
 
+// velocity-major: node r of a lane is make_node3's (wave task g, lane), the
+// wave's 64 lanes are 64 consecutive velocities of one spatial point
 template <int BLOCK, int NPT>
 __global__ __launch_bounds__( BLOCK )
-void trace3d_xm_fp32c_kernel( trace3_params p )
+void trace3d_fp32c_kernel( trace3_params p )
 {
     extern __shared__ float ldsf[];
     constexpr int WPB = BLOCK / WAVE;
-    const xm3_lane a = make_xm3<WPB>( p );
+
+    node3 id[NPT];
+    float xh[NPT], xl[NPT], yh[NPT], yl[NPT], zh[NPT], zl[NPT];
+    float uh[NPT], ul[NPT], vh[NPT], vl[NPT], wh[NPT], wl[NPT];
+    #pragma unroll
+    for ( int r = 0; r < NPT; ++r )
+    {
+        id[r] = make_node3<WPB,NPT>( p, r );
+        uint32_t ix, iy, iz; double u, v, w;
+        node_coords( p, id[r], ix, iy, iz, u, v, w );
+        xh[r] = float( ix ); xl[r] = 0.0f;
+        yh[r] = float( iy ); yl[r] = 0.0f;
+        zh[r] = float( iz ); zl[r] = 0.0f;
+        uh[r] = float( u ); ul[r] = float( u - double( uh[r] ) );
+        vh[r] = float( v ); vl[r] = float( v - double( vh[r] ) );
+        wh[r] = float( w ); wl[r] = float( w - double( wh[r] ) );
+    }
     const float Nxf = p.Nx, Nyf = p.Ny, Nzf = p.Nz;
     const float iNx = 1.0f / Nxf, iNy = 1.0f / Nyf, iNz = 1.0f / Nzf;
     const int Nxi = int( p.Nx ), Nyi = int( p.Ny ), Nzi = int( p.Nz );
@@ -428,71 +449,52 @@ void trace3d_xm_fp32c_kernel( trace3_params p )
     const float czh = float( cdz ), czl = float( cdz - double( czh ) );
     const float kx = float( -p.dt * p.dx_inv ), ky = float( -p.dt * p.dy_inv ), kz = float( -p.dt * p.dz_inv );
     const uint32_t st = p.st, sy = p.sy, sz = p.sz;
-    csum acc { 0.0, 0.0 };
 
-    for ( uint32_t pass = 0; pass < p.vgroup; pass += NPT )
+    for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
     {
-        float xh[NPT], xl[NPT], yh[NPT], yl[NPT], zh[NPT], zl[NPT];
-        float uh[NPT], ul[NPT], vh[NPT], vl[NPT], wh[NPT], wl[NPT];
-        bool act[NPT];
-        #pragma unroll
-        for ( int r = 0; r < NPT; ++r )
+        const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
+        const int cnt = int( mhi - mlo + 1 );
+        __syncthreads();
+        stage3<BLOCK>( ldsf, p.hist32 + size_t( mlo ) * st, uint32_t( cnt ) * st );
+        __syncthreads();
+        for ( int jj = cnt - 1; jj >= 0; --jj )
         {
-            double u, v, w;
-            act[r] = xm3_velocity( p, a, pass + r, u, v, w );
-            xh[r] = float( a.ix ); xl[r] = 0.0f;
-            yh[r] = float( a.iy ); yl[r] = 0.0f;
-            zh[r] = float( a.iz ); zl[r] = 0.0f;
-            uh[r] = float( u ); ul[r] = float( u - double( uh[r] ) );
-            vh[r] = float( v ); vl[r] = float( v - double( vh[r] ) );
-            wh[r] = float( w ); wl[r] = float( w - double( wh[r] ) );
-        }
-        for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
-        {
-            const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
-            const int cnt = int( mhi - mlo + 1 );
-            __syncthreads();
-            stage3<BLOCK>( ldsf, p.hist32 + size_t( mlo ) * st, uint32_t( cnt ) * st );
-            __syncthreads();
-            for ( int jj = cnt - 1; jj >= 0; --jj )
+            const float *C = ldsf + size_t( jj ) * st;
+            const float hf = ( mlo + jj == 0 ) ? 0.5f : 1.0f;
+            #pragma unroll
+            for ( int r = 0; r < NPT; ++r )
             {
-                const float *C = ldsf + size_t( jj ) * st;
-                const float hf = ( mlo + jj == 0 ) ? 0.5f : 1.0f;
-                #pragma unroll
-                for ( int r = 0; r < NPT; ++r )
-                {
-                    float tx, ty, tz;
-                    const int i = advance_axis32( xh[r], xl[r], uh[r], ul[r], cxh, cxl, Nxf, iNx, Nxi, tx );
-                    const int j = advance_axis32( yh[r], yl[r], vh[r], vl[r], cyh, cyl, Nyf, iNy, Nyi, ty );
-                    const int k = advance_axis32( zh[r], zl[r], wh[r], wl[r], czh, czl, Nzf, iNz, Nzi, tz );
-                    float Bx[4], Dx[4], By[4], Dy[4], Bz[4], Dz[4];
-                    cubic_bases( tx, Bx, Dx );
-                    cubic_bases( ty, By, Dy );
-                    cubic_bases( tz, Bz, Dz );
-                    float gx, gy, gz;
-                    grad_window( C + k*sz + j*sy + i, sy, sz, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
-                    kick32( uh[r], ul[r], ( hf * kx ) * gx );
-                    kick32( vh[r], vl[r], ( hf * ky ) * gy );
-                    kick32( wh[r], wl[r], ( hf * kz ) * gz );
-                }
+                float tx, ty, tz;
+                const int i = advance_axis32( xh[r], xl[r], uh[r], ul[r], cxh, cxl, Nxf, iNx, Nxi, tx );
+                const int j = advance_axis32( yh[r], yl[r], vh[r], vl[r], cyh, cyl, Nyf, iNy, Nyi, ty );
+                const int k = advance_axis32( zh[r], zl[r], wh[r], wl[r], czh, czl, Nzf, iNz, Nzi, tz );
+                float Bx[4], Dx[4], By[4], Dy[4], Bz[4], Dz[4];
+                cubic_bases( tx, Bx, Dx );
+                cubic_bases( ty, By, Dy );
+                cubic_bases( tz, Bz, Dz );
+                float gx, gy, gz;
+                grad_window( C + k*sz + j*sy + i, sy, sz, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
+                kick32( uh[r], ul[r], ( hf * kx ) * gx );
+                kick32( vh[r], vl[r], ( hf * ky ) * gy );
+                kick32( wh[r], wl[r], ( hf * kz ) * gz );
             }
-        }
-        #pragma unroll
-        for ( int r = 0; r < NPT; ++r )
-        {
-            double f = 0.0;
-            if ( act[r] )
-            {
-                const double x = p.x_min + ( double( xh[r] ) + double( xl[r] ) ) * p.dx;
-                const double y = p.y_min + ( double( yh[r] ) + double( yl[r] ) ) * p.dy;
-                const double z = p.z_min + ( double( zh[r] ) + double( zl[r] ) ) * p.dz;
-                f = f0_3d( p.f0, x, y, z, double( uh[r] ) + double( ul[r] ), double( vh[r] ) + double( vl[r] ),
-                           double( wh[r] ) + double( wl[r] ) );
-            }
-            acc = csum_add( acc, csum { f, 0.0 } );
         }
     }
-    if ( a.ok ) p.partials[ size_t( a.g ) * ( size_t( p.Nx ) * p.Ny * p.Nz ) + a.l ] = acc.s + acc.c;
+
+    #pragma unroll
+    for ( int r = 0; r < NPT; ++r )
+    {
+        double f = 0.0;
+        if ( id[r].active )
+        {
+            const double x = p.x_min + ( double( xh[r] ) + double( xl[r] ) ) * p.dx;
+            const double y = p.y_min + ( double( yh[r] ) + double( yl[r] ) ) * p.dy;
+            const double z = p.z_min + ( double( zh[r] ) + double( zl[r] ) ) * p.dz;
+            f = f0_3d( p.f0, x, y, z, double( uh[r] ) + double( ul[r] ), double( vh[r] ) + double( vl[r] ),
+                       double( wh[r] ) + double( wl[r] ) );
+        }
+        write_partial3( p, id[r], f );
+    }
 }
 
 // ---- strict FP64 (the reference's arithmetic, order 4) ----------------------
@@ -663,9 +665,9 @@ hipError_t launch3_npt( const trace3_params &p, int variant, int npt, uint32_t g
     {
         switch ( npt )
         {
-        case 1: return launch3( trace3d_xm_fp32c_kernel<BLOCK,1>, grid, BLOCK, lds, p, st );
-        case 2: return launch3( trace3d_xm_fp32c_kernel<BLOCK,2>, grid, BLOCK, lds, p, st );
-        case 4: return launch3( trace3d_xm_fp32c_kernel<BLOCK,4>, grid, BLOCK, lds, p, st );
+        case 1: return launch3( trace3d_fp32c_kernel<BLOCK,1>, grid, BLOCK, lds, p, st );
+        case 2: return launch3( trace3d_fp32c_kernel<BLOCK,2>, grid, BLOCK, lds, p, st );
+        case 4: return launch3( trace3d_fp32c_kernel<BLOCK,4>, grid, BLOCK, lds, p, st );
         default: return hipErrorInvalidValue;
         }
     }
@@ -708,7 +710,7 @@ hipError_t launch_trace3d( const trace3_params &p_in, int variant, const launch_
     if ( !trace3d_shape_ok( variant, int( p_in.order ), s ) ) return hipErrorInvalidValue;
     trace3_params p = p_in;
     uint32_t grid;
-    if ( p.mode == MODE_RHO && variant != TRACE_STRICT )
+    if ( p.mode == MODE_RHO && variant == TRACE_FAST )
     {
         // x-major: spatial tiles x velocity-group blocks; NPT divides vgroup
         if ( p.n_count == 0 ) return hipSuccess;
