@@ -33,6 +33,11 @@ PEAK_FP64 = 78.6     # TFLOP/s, MI355X FP64 vector
 PEAK_FP32 = 157.3    # TFLOP/s, MI355X FP32 vector
 
 
+def progress(msg):
+    """One progress line on stderr per finished leg (the JSON line stays alone on stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def sync_all(sched):
     sched.synchronize()
     try:
@@ -83,7 +88,9 @@ def landau3d_forms(args):
     c5 = nufi.landau_3d()
     t3 = None if args.no_autotune else 16
     r64, f64 = timed_runs(c5, nufi.FP64, 1, 0, FLOPS_3D, t3)
+    progress(f"3D3V Landau FP64 {r64['ms_per_run'] / 1e3:.1f} s per run")
     r32, f32 = timed_runs(c5, nufi.FP32C, 1, 0, FLOPS_3D, t3)
+    progress(f"3D3V Landau FP32C {r32['ms_per_run'] / 1e3:.1f} s per run")
     r32.update(energy_errors(f32, f64))
     r32["speedup_vs_fp64"] = r32["cs_per_s_e2e"] / r64["cs_per_s_e2e"]
     if args.dump_energies:
@@ -324,17 +331,20 @@ def main():
 
     if args.only in ("fp64", "both"):
         out["fp64"], e64 = timed_runs(conf, nufi.FP64, args.steps, args.warmup, FLOPS_1D, tune, shape)
+        progress(f"two-stream FP64 {out['fp64']['ms_per_run']:.1f} ms per run")
         with nufi.GpuScheduler(conf) as s:
             s.run()
             hist = s.history()
     if args.only in ("fp32c", "both"):
         out["fp32c"], e32 = timed_runs(conf, nufi.FP32C, args.steps, args.warmup, FLOPS_1D, tune)
+        progress(f"two-stream FP32C {out['fp32c']['ms_per_run']:.1f} ms per run")
         if e64 is not None:
             out["fp32c"].update(energy_errors(e32, e64))
             out["fp32c"]["speedup_vs_fp64"] = out["fp32c"]["cs_per_s_e2e"] / out["fp64"]["cs_per_s_e2e"]
 
     if not args.no_3d and args.only == "both" and args.case == "all":
         out["landau_3d"] = landau3d_forms(args)
+        progress("3D3V Landau done")
     if args.dump_energies and e64 is not None:
         np.save(os.path.join(args.dump_energies, "twostream_energy_fp64.npy"), e64)
         if e32 is not None:
@@ -342,7 +352,9 @@ def main():
 
     if not args.no_extra and args.only == "both" and args.case == "all":
         out["batch_sweep"] = batch_sweep()
+        progress("batched sweep done")
         out["adjoint"] = adjoint_case()
+        progress("adjoint done")
 
     prim = out.get("fp64") or out.get("fp32c")
     prim_prec = "fp64" if "fp64" in out else "fp32c"
@@ -358,6 +370,7 @@ def main():
     cpu = None
     if not args.no_cpu and hist is not None:
         cpu = cpu_baseline(conf, hist)
+        progress("CPU baseline done")
 
     line = {
         "metric": METRIC,

@@ -182,8 +182,7 @@ int effective_chunk( const nufi_ctx *c, const launch_shape &s, size_t *lds = nul
 
 // Default launch shapes (the autotuner refines them): 1D x-major 1024 threads
 // x 1 node (16 velocity rows per workgroup), the largest chunk the register
-// prefetch and 160 KiB of LDS hold; 3D 512 threads x 2 (FP64) or x 4 (FP32C)
-// nodes.
+// prefetch and 160 KiB of LDS hold; 3D 512 threads x 4 nodes.
 void default_shape( nufi_ctx *c )
 {
     const uint64_t waves = c->n_space * c->wpx;
@@ -215,7 +214,7 @@ void default_shape( nufi_ctx *c )
         chunk = std::max( 1, std::min( chunk, 1 ) );
         if ( c->variant == TRACE_STRICT ) c->shape = launch_shape { 256, 1, chunk };
         else if ( c->variant == TRACE_FP32C ) c->shape = launch_shape { 512, 4, per * 2 <= 65536 ? 2 : 1 };
-        else c->shape = launch_shape { 512, 2, chunk };
+        else c->shape = launch_shape { 512, 4, chunk };
     }
 }
 
@@ -623,6 +622,17 @@ static int step_async( nufi_ctx *c, uint64_t n, double *rho_out )
     field_params f = base_field( c );
     f.mode = FIELD_STEP;
     f.partials = c->partials;
+    if ( uint64_t( f.wpx ) * c->n_space > ( uint64_t( 1 ) << 16 ) )
+    {
+        // many partials (3D): one thread per point over many workgroups sums them
+        // in the same order, and the single field workgroup starts from rho
+        double *dst = rho_out ? rho_out : c->rho;
+        hipError_t e = launch_reduce_partials( c->partials, f.wpx, f.weight, 0, uint32_t( c->n_space ),
+                                               uint32_t( c->n_space ), dst, c->stream );
+        if ( e != hipSuccess ) return hip_fail( c, e, "reduce launch" );
+        f.partials = nullptr;
+        f.values = dst;
+    }
     f.mult = c->mult_step;
     f.slice = c->hist + n * c->st;
     f.dslice = c->dhist ? c->dhist + n * c->sd : nullptr;

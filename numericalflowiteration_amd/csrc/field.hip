@@ -29,6 +29,30 @@ using namespace nufi_dev;
 namespace
 {
 
+// Compensated sum over j = 0 .. wpx-1 of partials[j*n + i], in j order; the
+// loads go out RB at a time (clamped, unconditional) so that one point's sum
+// waits for ceil(wpx/RB) round trips instead of wpx.
+__device__ __forceinline__ double sum_partials( const double *__restrict__ partials, uint32_t wpx, size_t n,
+                                                uint32_t i )
+{
+    constexpr uint32_t RB = 16;
+    csum acc { 0.0, 0.0 };
+    for ( uint32_t j0 = 0; j0 < wpx; j0 += RB )
+    {
+        double v[RB];
+        #pragma unroll
+        for ( uint32_t k = 0; k < RB; ++k )
+        {
+            const uint32_t j = j0 + k < wpx ? j0 + k : wpx - 1;
+            v[k] = partials[ size_t( j ) * n + i ];
+        }
+        #pragma unroll
+        for ( uint32_t k = 0; k < RB; ++k )
+            if ( j0 + k < wpx ) acc = csum_add( acc, csum { v[k], 0.0 } );
+    }
+    return acc.s + acc.c;
+}
+
 __device__ double2 *fft3( double2 *A, double2 *B, const field_params &p, const double2 *tx, const double2 *ty,
                           const double2 *tz, bool inverse )
 {
@@ -68,11 +92,7 @@ __device__ __forceinline__ void field_body( const field_params &p, double2 *lds2
     {
         double v;
         if ( p.mode == FIELD_STEP && p.partials != nullptr )
-        {
-            csum acc { 0.0, 0.0 };
-            for ( uint32_t j = 0; j < p.wpx; ++j ) acc = csum_add( acc, csum { p.partials[ size_t( j ) * N + i ], 0.0 } );
-            v = p.weight * ( acc.s + acc.c );
-        }
+            v = p.weight * sum_partials( p.partials, p.wpx, N, i );
         else
         {
             v = p.values[i];
@@ -168,12 +188,7 @@ __global__ void reduce_partials_kernel( const double *partials, uint32_t wpx, do
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if ( i >= n_space ) return;
     double v = 0.0;
-    if ( i >= first && i < first + count )
-    {
-        csum acc { 0.0, 0.0 };
-        for ( uint32_t j = 0; j < wpx; ++j ) acc = csum_add( acc, csum { partials[ size_t( j ) * n_space + i ], 0.0 } );
-        v = weight * ( acc.s + acc.c );
-    }
+    if ( i >= first && i < first + count ) v = weight * sum_partials( partials, wpx, n_space, i );
     rho[i] = v;
 }
 
@@ -283,7 +298,7 @@ hipError_t launch_reduce_partials( const double *partials, uint32_t wpx, double 
                                    uint32_t first, uint32_t count, uint32_t n_space,
                                    double *rho, hipStream_t st )
 {
-    const uint32_t block = 256, grid = ( n_space + block - 1 ) / block;
+    const uint32_t block = 64, grid = ( n_space + block - 1 ) / block;     // one point per thread, many CUs
     hipLaunchKernelGGL( reduce_partials_kernel, dim3( grid ), dim3( block ), 0, st,
                         partials, wpx, weight, first, count, n_space, rho );
     return hipGetLastError();
