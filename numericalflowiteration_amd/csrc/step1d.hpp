@@ -185,8 +185,9 @@ __device__ __forceinline__ void build_poly( T *Ql, const double *dhist, uint32_t
 // between two LDS buffers.  While chunk k is traced from one buffer, the
 // windows of chunk k+1 (loaded during chunk k-1) are turned into the other
 // buffer and those of chunk k+2 are loaded, WPP windows per thread before
-// each of the chunk's PARTS parts of steps, so that the table work fills issue
-// slots the latency-bound steps leave free.  One barrier per chunk: chunk
+// each of the chunk's PARTS = MAXC/WPP parts of steps, so that the table work
+// fills issue slots the latency-bound steps leave free (fewer parts cost less
+// scalar bookkeeping per step: WPP = 3 suits the FP64 step, 2 the FP32C one).  One barrier per chunk: chunk
 // k+1's buffer was last read in chunk k-1.
 //
 // Every window's load is issued unconditionally, MAXC per thread and chunk
@@ -194,12 +195,12 @@ __device__ __forceinline__ void build_poly( T *Ql, const double *dhist, uint32_t
 // do not use): with straight-line loads the wait before turning window c into
 // the table is for the load issued one chunk earlier, not for the ones just
 // issued.
-template <int BLOCK, int MAXC, typename T, typename STEPS>
+template <int BLOCK, int MAXC, int WPP, typename T, typename STEPS>
 __device__ __forceinline__ void poly_pipeline( T *lds, const double *dhist, uint32_t sd, uint32_t sq, uint32_t q2,
                                                uint32_t cells, double kick, uint64_t n, uint32_t C, STEPS steps )
 {
     if ( n == 0 ) return;
-    constexpr int WPP = 2, PARTS = ( MAXC + WPP - 1 ) / WPP;
+    constexpr int PARTS = ( MAXC + WPP - 1 ) / WPP;
     poly_fetch<BLOCK,MAXC> pf;
     pf.init( sd, sq, cells );
     const int Ci = int( C ), top = int( n ) - 1;

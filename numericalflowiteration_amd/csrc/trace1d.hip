@@ -357,7 +357,7 @@ __device__ __forceinline__ void trace1d_poly_body( const trace1_params &p, uint3
             }
         }
     };
-    if ( p.pipe ) poly_pipeline<BLOCK,poly_maxc( BLOCK )>( lds, p.dhist, p.stride_d, sq, q2, cells, kick, p.n, p.chunk, steps );
+    if ( p.pipe ) poly_pipeline<BLOCK,poly_maxc( BLOCK ),3>( lds, p.dhist, p.stride_d, sq, q2, cells, kick, p.n, p.chunk, steps );
     else          poly_direct<BLOCK>( lds, p.dhist, p.stride_d, sq, q2, cells, kick, p.n, p.chunk, steps );
 
     double f[NPT];
@@ -438,7 +438,7 @@ __device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint
         for ( ; m >= s1; --m, Q -= sq ) step( Q, 1.0f );
         if ( mend == 0 && m == 0 ) step( Q, 0.5f );
     };
-    if ( p.pipe ) poly_pipeline<BLOCK,poly_maxc( BLOCK )>( ldsf, p.dhist, p.stride_d, sq, q2, cells, p.kick, p.n, p.chunk, steps );
+    if ( p.pipe ) poly_pipeline<BLOCK,poly_maxc( BLOCK ),2>( ldsf, p.dhist, p.stride_d, sq, q2, cells, p.kick, p.n, p.chunk, steps );
     else          poly_direct<BLOCK>( ldsf, p.dhist, p.stride_d, sq, q2, cells, p.kick, p.n, p.chunk, steps );
 
     double f[NPT];
