@@ -203,3 +203,34 @@ def test_rho_bit_identical_across_launch_shapes(conf_args, prec, variant):
             s.set_launch_shape(256, 1, 8)          # 4 velocity rows: not whole groups of 16
     for r in out[1:]:
         assert np.array_equal(r, out[0])
+
+
+@pytest.mark.parametrize("Nx,prec,variant,tol", [
+    (4096, nufi.FP64, _capi.TRACE_FAST, FP64_TOL),     # tables too wide for two LDS buffers: built per chunk
+    (4096, nufi.FP32C, _capi.TRACE_FP32C, FP32C_TOL),  # pipelined, one slice per chunk
+    (6400, nufi.FP64, _capi.TRACE_FAST, FP64_TOL),     # wider than the prefetch: built per chunk
+    (6400, nufi.FP32C, _capi.TRACE_FP32C, FP32C_TOL),
+])
+def test_wide_grid_table_paths(Nx, prec, variant, tol):
+    """The fast and FP32C kernels on grids whose kick-polynomial tables do not
+    fit the double-buffered pipeline, against the oracle on a synthetic smooth
+    periodic history (no field solve: 6400 points exceed the device field
+    kernel's single workgroup)."""
+    conf = nufi.config_1d(Nx, 64, 1 / 16, 1.0, 0.0, 4 * math.pi, -10.0, 10.0,
+                          nufi.config.F0_LANDAU_1D, 0.05, 0.5)
+    st = conf.stride_t()
+    j = np.arange(st) % Nx
+    hist = np.concatenate([0.02 * (m + 1) * np.sin(2 * np.pi * 3 * j / Nx)
+                           + 0.01 * np.cos(2 * np.pi * 7 * j / Nx + m) for m in range(4)])
+    print(f"[wide grid] Nx={Nx} {prec}", flush=True)
+    with nufi.GpuScheduler(conf, precision=prec) as s:
+        s.set_trace_variant(variant)
+        s.upload_history(hist, 0, 4)
+        for n in (1, 3, 4):
+            rho = np.zeros(conf.Nx)
+            s.compute_rho(n)
+            s.download_rho(rho)
+            exact = O.rho(conf, n, hist, exact_sum=True)
+            err = rel(mean_free(rho), mean_free(exact))
+            print(f"  n={n} rel err {err:.3e}", flush=True)
+            assert err <= tol, (n, err)
