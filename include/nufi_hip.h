@@ -189,9 +189,12 @@ int nufi_set_trace_variant(nufi_ctx *ctx, int variant);
  * trace kernel (what the autotuner sets); NUFI_ERR_UNSUPPORTED if the shape
  * is not built or does not fit the LDS.  1D FAST and FP32C kernels take
  * (block/64)*nodes_per_thread a multiple of 16 (whole velocity groups per
- * workgroup) and cap the chunk at what the workgroup's register prefetch of
- * the table holds.  The FAST and FP32C results do not depend on the shape
- * (fixed summation groups), so tuning never changes a result. */
+ * workgroup) and treat the chunk as an upper bound: the library uses the
+ * largest chunk whose tables the workgroup's register prefetch and two LDS
+ * buffers hold (one LDS buffer built per chunk for wider grids).  The FAST
+ * and FP32C results do not depend on the shape (fixed summation groups and a
+ * re-wrap schedule fixed by the slice index), so tuning never changes a
+ * result. */
 int nufi_set_launch_shape(nufi_ctx *ctx, int block, int nodes_per_thread, int chunk);
 int nufi_get_launch_shape(nufi_ctx *ctx, int *block, int *nodes_per_thread, int *chunk);
 /* Autotuner (no counterpart in the reference, whose launch shape is fixed at
