@@ -31,6 +31,8 @@ FLOPS_1D = 30.0      # algorithmic FLOPs per characteristic step, order 4, 1D
 FLOPS_3D = 475.0     # 3D
 PEAK_FP64 = 78.6     # TFLOP/s, MI355X FP64 vector
 PEAK_FP32 = 157.3    # TFLOP/s, MI355X FP32 vector
+CUS, CLOCK_GHZ = 256, 2.4
+LDS_PEAK_TBS = 128 * CUS * CLOCK_GHZ * 1e9 / 1e12   # 128 B per clock per CU
 
 
 def progress(msg):
@@ -237,6 +239,14 @@ def summarize_profiles(outdir, tag, commands):
         return {r[0]: (r[1], r[2]) for r in con.execute(
             "select kernel_name, count(*), avg(value) from counters_collection group by kernel_name")}
 
+    def per_counter(con):
+        out = {}
+        for k, name, v in con.execute("select kernel_name, counter_name, avg(s) from (select kernel_name, "
+                                      "counter_name, dispatch_id, sum(value) as s from counters_collection "
+                                      "group by dispatch_id, counter_name) group by kernel_name, counter_name"):
+            out.setdefault(k, {})[name] = v
+        return out
+
     fetch, write = db("prof_fetch"), db("prof_write")
     cfetch, cwrite = db("prof_cfetch"), db("prof_cwrite")
     result = {}
@@ -261,6 +271,42 @@ def summarize_profiles(outdir, tag, commands):
                     result = {"kernel": k, "dispatches": pf[k][0], "fetch_kib_raw": pf[k][1],
                               "write_kib_raw": pw[k][1], "fetch_scale": f_scale, "write_scale": w_scale,
                               "hbm_bytes_per_launch": b}
+        lines.append("")
+    valu, l2 = db("prof_valu"), db("prof_l2")
+    if kt is not None and ( valu is not None or l2 is not None ):
+        dur = {r[0]: r[1] for r in kt.execute("select name, average from top_kernels")}   # us
+        pv = per_counter(valu) if valu is not None else {}
+        pl = per_counter(l2) if l2 is not None else {}
+        lines += ["## VALU FLOPs, LDS bandwidth and L2 hit rate (--pmc, one pass each; per dispatch)", "",
+                  f"Peaks (MI355X, {CUS} CUs at {CLOCK_GHZ} GHz): FP64 vector {PEAK_FP64} TFLOP/s, FP32 vector "
+                  f"{PEAK_FP32} TFLOP/s, LDS {LDS_PEAK_TBS:.1f} TB/s (128 B per clock per CU). The counters are "
+                  "means per dispatch; rates divide them by the kernel's mean duration from the kernel trace. "
+                  "SQ_INSTS_VALU_FLOPS_FP64/FP32 add the FLOPs of one lane per wave instruction, so the executed "
+                  "rate is 64 times the counter (a wave has 64 lanes); it counts what the kernel executes, "
+                  "which for the 1D polynomial step is fewer FLOPs than the algorithmic 30 per char-step. "
+                  "LDS bytes are 64 x SQ_INSTS_LDS_{LOAD,STORE}_BANDWIDTH; the L2 hit rate is "
+                  "TCC_HIT / (TCC_HIT + TCC_MISS).", "",
+                  "| kernel | mean us | FP64 TFLOP/s executed | FP32 TFLOP/s executed | LDS load+store TB/s (frac of peak) | LDS bank-conflict / VALU-busy cycles | L2 hit rate |",
+                  "|---|---|---|---|---|---|---|"]
+        for k in sorted(set(pv) | set(pl)):
+            if "nufi" not in k or k not in dur:
+                continue
+            us = dur[k]
+            c = pv.get(k, {})
+            f64 = 64.0 * c.get("SQ_INSTS_VALU_FLOPS_FP64", 0.0) / ( us * 1e-6 ) / 1e12
+            f32 = 64.0 * c.get("SQ_INSTS_VALU_FLOPS_FP32", 0.0) / ( us * 1e-6 ) / 1e12
+            lds_b = 64.0 * ( c.get("SQ_INSTS_LDS_LOAD_BANDWIDTH", 0.0) + c.get("SQ_INSTS_LDS_STORE_BANDWIDTH", 0.0) )
+            lds = lds_b / ( us * 1e-6 ) / 1e12
+            busy = c.get("SQ_ACTIVE_INST_VALU", 0.0)
+            conf = c.get("SQ_LDS_BANK_CONFLICT", 0.0) / busy if busy else float("nan")
+            h = pl.get(k, {})
+            hit, miss = h.get("TCC_HIT_sum", 0.0), h.get("TCC_MISS_sum", 0.0)
+            hr = hit / ( hit + miss ) if hit + miss else float("nan")
+            lines.append(f"| `{k[:70]}` | {us:.1f} | {f64:.2f} | {f32:.2f} | {lds:.2f} ({lds / LDS_PEAK_TBS:.3f}) | "
+                         f"{conf:.3f} | {hr:.4f} |")
+            result.setdefault("counters", {})[k] = {"mean_us": us, "fp64_tflops": f64, "fp32_tflops": f32,
+                                                    "lds_tbs": lds, "lds_frac": lds / LDS_PEAK_TBS,
+                                                    "l2_hit_rate": hr}
         lines.append("")
     with open(os.path.join(ROOT, "profiles", f"{tag}.md"), "w") as f:
         f.write("\n".join(lines) + "\n")
