@@ -5,7 +5,7 @@
 //   dhist    1D: first differences of every slice (stride_d)        FP64
 //            (the fast, FP32C and adjoint kernels build the per-cell kick
 //            polynomials from it in LDS)
-//   hist32   3D FP32C: the coefficients rounded to FP32
+//   hist32   3D FP32C: the coefficients rounded to FP32 (slice stride st32: st rounded up to 4)
 //   partials compensated partial sums of f: per wave of 64 velocity nodes
 //            (strict), per x-major velocity group (fast, FP32C)
 //   rho, energies, the FFT twiddles and Fourier multipliers.
@@ -35,6 +35,7 @@ struct nufi_ctx
     int variant = TRACE_FAST;
     hipStream_t stream = nullptr;
     uint32_t st = 0, sd = 0, sy = 0, sz = 0;
+    uint32_t st32 = 0;         // hist32 slice stride: st rounded up to 4 floats
     uint64_t n_space = 0, n_vel = 0, n_nodes = 0;
     uint32_t wpx = 0;          // wave-task partials per spatial point (strict 1D, 3D)
     uint32_t ppx = 0;          // x-major partials per spatial point (fast; 1D FP32C)
@@ -169,6 +170,11 @@ int effective_chunk( const nufi_ctx *c, const launch_shape &s, size_t *lds = nul
 {
     const size_t elem = ( c->variant == TRACE_FP32C ) ? sizeof(float) : sizeof(double);
     const size_t row = c->dim == 1 ? ( c->variant == TRACE_STRICT ? c->sd : c->sq ) : c->st;
+    if ( c->dim == 3 && c->variant == TRACE_FP32C )
+    {
+        if ( lds ) *lds = ( fp32c3_dma( uint32_t( s.chunk ), c->st32 ) ? 2 : 1 ) * size_t( s.chunk ) * c->st32 * elem;
+        return s.chunk;
+    }
     if ( c->dim != 1 || c->variant == TRACE_STRICT )
     {
         if ( lds ) *lds = size_t( s.chunk ) * row * elem;
@@ -213,7 +219,7 @@ void default_shape( nufi_ctx *c )
         int chunk = int( std::min<size_t>( c->lds_cap, 64 * 1024 ) / per );
         chunk = std::max( 1, std::min( chunk, 1 ) );
         if ( c->variant == TRACE_STRICT ) c->shape = launch_shape { 256, 1, chunk };
-        else if ( c->variant == TRACE_FP32C ) c->shape = launch_shape { 512, 4, per * 2 <= 65536 ? 2 : 1 };
+        else if ( c->variant == TRACE_FP32C ) c->shape = launch_shape { 512, 4, 1 };   // two 1-slice buffers
         else c->shape = launch_shape { 512, 4, chunk };
     }
 }
@@ -270,7 +276,7 @@ int launch_trace( nufi_ctx *c, uint64_t n, uint64_t q_begin, uint64_t q_end, uin
         p.n = n; p.q_begin = q_begin; p.q_end = q_end;
         p.Nx = uint32_t( c->conf.Nx ); p.Ny = uint32_t( c->conf.Ny ); p.Nz = uint32_t( c->conf.Nz );
         p.Nu = uint32_t( c->conf.Nu ); p.Nv = uint32_t( c->conf.Nv ); p.Nw = uint32_t( c->conf.Nw );
-        p.sy = c->sy; p.sz = c->sz; p.st = c->st; p.wpx = c->wpx;
+        p.sy = c->sy; p.sz = c->sz; p.st = c->st; p.st32 = c->st32; p.wpx = c->wpx;
         p.l_first = uint32_t( l0 ); p.n_tasks = uint32_t( tasks );
         p.n_count = uint32_t( l1 - l0 + 1 ); p.vgroup = c->vgroup; p.ppx = c->ppx;
         p.order = uint32_t( c->order );
@@ -428,7 +434,8 @@ int nufi_create( const nufi_config *conf, const nufi_config *conf_metrics, int o
     if ( ( rc = dev_alloc( c, &c->hist, slices * c->st + 8, "history" ) ) ) return bail( rc );
     if ( c->dim == 1 && ( rc = dev_alloc( c, &c->dhist, slices * c->sd + 8, "difference history" ) ) ) return bail( rc );
     if ( c->dim == 1 ) c->sq = poly_stride( uint32_t( conf->Nx ) );
-    if ( precision == NUFI_FP32C && c->dim == 3 && ( rc = dev_alloc( c, &c->hist32, slices * c->st + 8, "FP32 history" ) ) )
+    c->st32 = ( c->st + 3u ) & ~3u;
+    if ( precision == NUFI_FP32C && c->dim == 3 && ( rc = dev_alloc( c, &c->hist32, slices * c->st32 + 8, "FP32 history" ) ) )
         return bail( rc );
     if ( ( rc = dev_alloc( c, &c->partials, c->n_space * std::max( c->wpx, c->ppx ), "partials" ) ) ) return bail( rc );
     if ( ( rc = dev_alloc( c, &c->rho, c->n_space, "rho" ) ) ) return bail( rc );
@@ -525,7 +532,7 @@ int nufi_upload_phi( nufi_ctx *c, uint64_t n, const double *history_base )
     HIP_TRY( c, hipMemcpyAsync( dst, history_base + n * c->st, c->st * sizeof(double), hipMemcpyHostToDevice, c->stream ) );
     hipError_t e = launch_slice_tables( c->dim, dst, c->st, uint32_t( c->conf.Nx ),
                                         c->dhist ? c->dhist + n * c->sd : nullptr,
-                                        c->hist32 ? c->hist32 + n * c->st : nullptr, c->stream );
+                                        c->hist32 ? c->hist32 + n * c->st32 : nullptr, c->stream );
     if ( e != hipSuccess ) return hip_fail( c, e, "slice table launch" );
     // the copy reads host memory the caller may reuse: complete it before returning
     HIP_TRY( c, hipStreamSynchronize( c->stream ) );
@@ -636,7 +643,7 @@ static int step_async( nufi_ctx *c, uint64_t n, double *rho_out )
     f.mult = c->mult_step;
     f.slice = c->hist + n * c->st;
     f.dslice = c->dhist ? c->dhist + n * c->sd : nullptr;
-    f.slice32 = c->hist32 ? c->hist32 + n * c->st : nullptr;
+    f.slice32 = c->hist32 ? c->hist32 + n * c->st32 : nullptr;
     f.energy = c->energies + n;
     f.rho_out = rho_out ? rho_out : c->rho;
     hipError_t e = launch_field( f, c->stream );

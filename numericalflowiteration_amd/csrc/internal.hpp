@@ -105,6 +105,8 @@ struct trace3_params
     uint64_t q_begin, q_end;
     uint32_t Nx, Ny, Nz, Nu, Nv, Nw;
     uint32_t sy, sz, st;     // strides of a slice: sy = Nx+order-1, sz = sy*(Ny+order-1)
+    uint32_t st32;           // FP32C: slice stride of hist32, st rounded up to 4 floats (16-byte slices)
+    uint32_t dma;            // FP32C rho: two LDS buffers filled by direct global->LDS loads
     uint32_t wpx;            // waves per spatial point: ceil(Nvel/64)
     uint32_t l_first, n_tasks;
     uint32_t n_count, n_st;  // x-major: spatial points of the range, 64-point tiles
@@ -140,6 +142,11 @@ size_t trace1d_slice_elems( const trace1_params &p, int variant );
 hipError_t launch_trace1d_batch( const batch_run *runs, const uint32_t *wg_run, uint32_t n_wg, uint64_t n,
                                  int variant, const launch_shape &s, size_t lds, hipStream_t st );
 size_t trace3d_lds_bytes( const trace3_params &p, int variant, const launch_shape &s );
+// 3D FP32C rho: whether two LDS buffers of `chunk` slices of st32 floats fit 160 KiB
+inline bool fp32c3_dma( uint32_t chunk, uint32_t st32 )
+{
+    return 2ull * chunk * st32 * sizeof(float) <= 160u * 1024u;
+}
 bool trace1d_shape_ok( int variant, int order, const launch_shape &s );
 // x-major fields (xw, vpw, n_xt, ppx) from Nx, Nu and ix_count, the grid of
 // a launch (x-major or wave-task decomposition, by variant and mode) and the

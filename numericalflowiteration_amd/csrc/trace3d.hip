@@ -68,6 +68,31 @@ __device__ __forceinline__ void stage3( T *__restrict__ dst, const T *__restrict
     }
 }
 
+// Direct global->LDS copy of n16 16-byte words (gfx950 global_load_lds_dwordx4):
+// lane l of a wave writes word q0 + l of dst, the wave's words starting at
+// q0; the loads hold no registers and are waited for (vmcnt) before the
+// barrier that publishes the buffer.  dst and src are 16-byte aligned.
+template <int BLOCK>
+__device__ __forceinline__ void dma_copy16( float *dst, const float *src, uint32_t n16 )
+{
+    typedef __attribute__((address_space(3))) void lds_void;
+    const uint32_t lane = threadIdx.x % WAVE;
+    const uint32_t w = __builtin_amdgcn_readfirstlane( threadIdx.x / WAVE );     // the wave index, scalar
+    for ( uint32_t q0 = w * WAVE; q0 < n16; q0 += BLOCK )                        // uniform in the wave
+    {
+        if ( q0 + lane < n16 )
+            __builtin_amdgcn_global_load_lds( reinterpret_cast<const void*>( src + size_t( q0 + lane ) * 4 ),
+                                              (lds_void*)( dst + size_t( q0 ) * 4 ), 16, 0, 0 );
+    }
+}
+
+// wait for this wave's direct LDS loads (vmcnt(0)), then the workgroup barrier
+__device__ __forceinline__ void dma_publish()
+{
+    __builtin_amdgcn_s_waitcnt( 0x0F70 );     // vmcnt(0); expcnt and lgkmcnt not waited for
+    __syncthreads();
+}
+
 struct node3
 {
     uint32_t g, l, j, iv;
@@ -448,18 +473,46 @@ void trace3d_fp32c_kernel( trace3_params p )
     const float cyh = float( cdy ), cyl = float( cdy - double( cyh ) );
     const float czh = float( cdz ), czl = float( cdz - double( czh ) );
     const float kx = float( -p.dt * p.dx_inv ), ky = float( -p.dt * p.dy_inv ), kz = float( -p.dt * p.dz_inv );
-    const uint32_t st = p.st, sy = p.sy, sz = p.sz;
+    const uint32_t st = p.st32, sy = p.sy, sz = p.sz, C = p.chunk;
 
-    for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
+    // chunk k: slices max(top - kC - C + 1, 0) .. top - kC.  With p.dma the
+    // chunks alternate between two LDS buffers: chunk k+1 is copied straight
+    // into the other buffer while chunk k is traced, and one barrier per chunk
+    // publishes it (the other buffer was last read in chunk k-1).
+    const int64_t top = int64_t( p.n ) - 1;
+    const int64_t K = ( top + int64_t( C ) ) / int64_t( C );
+    auto lo_of = [&]( int64_t k ) { const int64_t h = top - k * C; return h - int64_t( C ) + 1 > 0 ? h - int64_t( C ) + 1 : 0; };
+    float *const buf0 = ldsf, *const buf1 = ldsf + size_t( C ) * st;
+    if ( p.dma && K > 0 )
     {
-        const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
+        const int64_t l0 = lo_of( 0 );
+        dma_copy16<BLOCK>( buf0, p.hist32 + size_t( l0 ) * st, uint32_t( top - l0 + 1 ) * st / 4 );
+        dma_publish();
+    }
+    for ( int64_t k = 0; k < K; ++k )
+    {
+        const int64_t mhi = top - k * C, mlo = lo_of( k );
         const int cnt = int( mhi - mlo + 1 );
-        __syncthreads();
-        stage3<BLOCK>( ldsf, p.hist32 + size_t( mlo ) * st, uint32_t( cnt ) * st );
-        __syncthreads();
+        float *cur = buf0;
+        if ( p.dma )
+        {
+            cur = ( k & 1 ) ? buf1 : buf0;
+            if ( k + 1 < K )
+            {
+                const int64_t nlo = lo_of( k + 1 );
+                dma_copy16<BLOCK>( ( k & 1 ) ? buf0 : buf1, p.hist32 + size_t( nlo ) * st,
+                                   uint32_t( mlo - nlo ) * st / 4 );
+            }
+        }
+        else
+        {
+            __syncthreads();
+            stage3<BLOCK>( ldsf, p.hist32 + size_t( mlo ) * st, uint32_t( cnt ) * st );
+            __syncthreads();
+        }
         for ( int jj = cnt - 1; jj >= 0; --jj )
         {
-            const float *C = ldsf + size_t( jj ) * st;
+            const float *Cs = cur + size_t( jj ) * st;
             const float hf = ( mlo + jj == 0 ) ? 0.5f : 1.0f;
             #pragma unroll
             for ( int r = 0; r < NPT; ++r )
@@ -473,12 +526,13 @@ void trace3d_fp32c_kernel( trace3_params p )
                 cubic_bases( ty, By, Dy );
                 cubic_bases( tz, Bz, Dz );
                 float gx, gy, gz;
-                grad_window( C + k*sz + j*sy + i, sy, sz, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
+                grad_window( Cs + k*sz + j*sy + i, sy, sz, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
                 kick32( uh[r], ul[r], ( hf * kx ) * gx );
                 kick32( vh[r], vl[r], ( hf * ky ) * gy );
                 kick32( wh[r], wl[r], ( hf * kz ) * gz );
             }
         }
+        if ( p.dma ) dma_publish();
     }
 
     #pragma unroll
@@ -699,10 +753,11 @@ static uint64_t visited3( const trace3_params &p )
 
 size_t trace3d_lds_bytes( const trace3_params &p, int variant, const launch_shape &s )
 {
-    const size_t elem = ( variant == TRACE_FP32C && p.mode == MODE_RHO ) ? sizeof(float) : sizeof(double);
     const uint64_t v = visited3( p );
     const uint64_t c = v < uint64_t( s.chunk ) ? ( v > 0 ? v : 1 ) : uint64_t( s.chunk );
-    return size_t( c ) * p.st * elem;
+    if ( variant == TRACE_FP32C && p.mode == MODE_RHO )        // the buffers of the FP32 slices
+        return ( fp32c3_dma( uint32_t( s.chunk ), p.st32 ) ? 2 : 1 ) * size_t( c ) * p.st32 * sizeof(float);
+    return size_t( c ) * p.st * sizeof(double);
 }
 
 hipError_t launch_trace3d( const trace3_params &p_in, int variant, const launch_shape &s, hipStream_t st )
@@ -728,6 +783,7 @@ hipError_t launch_trace3d( const trace3_params &p_in, int variant, const launch_
     p.chunk = uint32_t( s.chunk );
     const uint64_t v = visited3( p );
     if ( v < p.chunk ) p.chunk = v > 0 ? uint32_t( v ) : 1u;
+    p.dma = ( p.mode == MODE_RHO && variant == TRACE_FP32C && fp32c3_dma( uint32_t( s.chunk ), p.st32 ) ) ? 1u : 0u;
     const size_t lds = trace3d_lds_bytes( p, variant, s );
     if ( lds > 160u * 1024u ) return hipErrorInvalidValue;
 

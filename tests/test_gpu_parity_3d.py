@@ -118,15 +118,19 @@ def test_q_subranges_3d():
 
 @pytest.mark.parametrize("prec", [nufi.FP64, nufi.FP32C])
 def test_rho_bit_identical_across_launch_shapes_3d(prec):
-    """Each lane of the x-major 3D kernels sums its fixed velocity group in iv
-    order, so rho does not depend on the launch shape."""
+    """Each lane of the x-major FP64 kernel sums its fixed velocity group in iv
+    order, and each wave of the velocity-major FP32C kernel its 64 velocities,
+    so rho does not depend on the launch shape (for FP32C also not on the
+    staging: two LDS buffers filled by direct loads, or, for the 16-slice
+    chunk whose two buffers would not fit 160 KiB, one staged buffer)."""
     conf = small_3d()
     hist, _ = oracle_history(conf, 6)
     st = conf.stride_t()
     out = []
     with nufi.GpuScheduler(conf, precision=prec) as s:
         s.upload_history(hist[: 5 * st], 0, 5)
-        for shape in [(256, 1, 1), (512, 4, 2), (256, 2, 1)]:
+        extra = (512, 4, 16) if prec == nufi.FP32C else (512, 4, 8)      # the LDS holds 8 FP64 slices
+        for shape in [(256, 1, 1), (512, 4, 2), (256, 2, 1), extra]:
             s.set_launch_shape(*shape)
             rho = np.zeros(conf.n_space())
             s.compute_rho(4)

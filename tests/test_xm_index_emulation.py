@@ -246,3 +246,40 @@ def test_table_build_from_differences_stays_in_bounds(Nx):
                                     written.add(a)
                                 assert loff + j + 1 < jj * sq + q2 and loff + q2 < (jj + 1) * sq
                     assert len(written) == 3 * cnt * cells
+
+
+@pytest.mark.parametrize("block", [256, 512])
+@pytest.mark.parametrize("n", [1, 3, 9])
+def test_fp32c3_direct_lds_copy_stays_in_bounds(block, n):
+    """trace3d.hip dma_copy16 and the FP32C double buffer: with st32 = st
+    rounded up to 4 floats, chunk k's n16 = cnt*st32/4 words go to word
+    q = q0 + lane of its buffer (q0 = wave*64 + i*block, lane < 64, q < n16):
+    every word of the chunk is written once, inside the buffer, from inside
+    the chunk's rows; the two buffers (chunk*st32 floats each) fit 160 KiB."""
+    for N in (8, 16):
+        st = (N + 3) ** 3
+        st32 = (st + 3) & ~3
+        for chunk in (1, 2, 16):
+            dma = 2 * chunk * st32 * 4 <= 160 * 1024
+            if not dma:
+                continue
+            top = n - 1
+            K = (top + chunk) // chunk
+            for k in range(K):
+                h = top - k * chunk
+                lo = max(h - chunk + 1, 0)
+                cnt = h - lo + 1
+                n16 = cnt * st32 // 4
+                seen = set()
+                for w in range(block // 64):
+                    q0 = w * 64
+                    while q0 < n16:
+                        for lane in range(64):
+                            q = q0 + lane
+                            if q < n16:
+                                assert q not in seen
+                                seen.add(q)
+                                assert 4 * q + 3 < chunk * st32          # inside the buffer
+                                assert lo * st32 + 4 * q + 3 < (lo + cnt) * st32   # inside the chunk's rows
+                        q0 += block
+                assert len(seen) == n16
