@@ -175,6 +175,12 @@ int effective_chunk( const nufi_ctx *c, const launch_shape &s, size_t *lds = nul
         if ( lds ) *lds = ( fp32c3_dma( uint32_t( s.chunk ), c->st32 ) ? 2 : 1 ) * size_t( s.chunk ) * c->st32 * elem;
         return s.chunk;
     }
+    if ( c->dim == 3 && c->variant == TRACE_FAST )
+    {
+        if ( lds ) *lds = fast3_dma( uint32_t( s.chunk ), c->st ) ? 2 * ( size_t( s.chunk ) * c->st + 2 ) * elem
+                                                                   : size_t( s.chunk ) * c->st * elem;
+        return s.chunk;
+    }
     if ( c->dim != 1 || c->variant == TRACE_STRICT )
     {
         if ( lds ) *lds = size_t( s.chunk ) * row * elem;

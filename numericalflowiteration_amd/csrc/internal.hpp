@@ -106,7 +106,7 @@ struct trace3_params
     uint32_t Nx, Ny, Nz, Nu, Nv, Nw;
     uint32_t sy, sz, st;     // strides of a slice: sy = Nx+order-1, sz = sy*(Ny+order-1)
     uint32_t st32;           // FP32C: slice stride of hist32, st rounded up to 4 floats (16-byte slices)
-    uint32_t dma;            // FP32C rho: two LDS buffers filled by direct global->LDS loads
+    uint32_t dma;            // fast / FP32C rho: two LDS buffers filled by direct global->LDS loads
     uint32_t wpx;            // waves per spatial point: ceil(Nvel/64)
     uint32_t l_first, n_tasks;
     uint32_t n_count, n_st;  // x-major: spatial points of the range, 64-point tiles
@@ -146,6 +146,12 @@ size_t trace3d_lds_bytes( const trace3_params &p, int variant, const launch_shap
 inline bool fp32c3_dma( uint32_t chunk, uint32_t st32 )
 {
     return 2ull * chunk * st32 * sizeof(float) <= 160u * 1024u;
+}
+// 3D fast FP64 rho: whether two LDS buffers of `chunk` slices (+ 2 doubles for
+// the 8-byte slice alignment) fit 160 KiB
+inline bool fast3_dma( uint32_t chunk, uint32_t st )
+{
+    return 2ull * ( uint64_t( chunk ) * st + 2 ) * sizeof(double) <= 160u * 1024u;
 }
 bool trace1d_shape_ok( int variant, int order, const launch_shape &s );
 // x-major fields (xw, vpw, n_xt, ppx) from Nx, Nu and ix_count, the grid of

@@ -73,7 +73,7 @@ __device__ __forceinline__ void stage3( T *__restrict__ dst, const T *__restrict
 // q0; the loads hold no registers and are waited for (vmcnt) before the
 // barrier that publishes the buffer.  dst and src are 16-byte aligned.
 template <int BLOCK>
-__device__ __forceinline__ void dma_copy16( float *dst, const float *src, uint32_t n16 )
+__device__ __forceinline__ void dma_copy16( void *dst, const void *src, uint32_t n16 )
 {
     typedef __attribute__((address_space(3))) void lds_void;
     const uint32_t lane = threadIdx.x % WAVE;
@@ -81,9 +81,20 @@ __device__ __forceinline__ void dma_copy16( float *dst, const float *src, uint32
     for ( uint32_t q0 = w * WAVE; q0 < n16; q0 += BLOCK )                        // uniform in the wave
     {
         if ( q0 + lane < n16 )
-            __builtin_amdgcn_global_load_lds( reinterpret_cast<const void*>( src + size_t( q0 + lane ) * 4 ),
-                                              (lds_void*)( dst + size_t( q0 ) * 4 ), 16, 0, 0 );
+            __builtin_amdgcn_global_load_lds( static_cast<const char*>( src ) + size_t( q0 + lane ) * 16,
+                                              (lds_void*)( static_cast<char*>( dst ) + size_t( q0 ) * 16 ), 16, 0, 0 );
     }
+}
+
+// The FP64 slices start 8-byte aligned (odd strides): copy the 16-byte words
+// that cover `count` doubles from src and return where src[0] lands in dst
+// (dst + 0 or + 1).  The history allocation has slack past its last slice.
+template <int BLOCK>
+__device__ __forceinline__ const double *dma_doubles( double *dst, const double *src, uint32_t count )
+{
+    const uint32_t mis = uint32_t( reinterpret_cast<uintptr_t>( src ) & 15u ) / 8u;
+    dma_copy16<BLOCK>( dst, src - mis, ( count + mis + 1 ) / 2 );
+    return dst + mis;
 }
 
 // wait for this wave's direct LDS loads (vmcnt(0)), then the workgroup barrier
@@ -360,6 +371,49 @@ __device__ __forceinline__ bool xm3_velocity( const trace3_params &p, const xm3_
     return a.ok && iv < nvel && q >= p.q_begin && q < p.q_end;
 }
 
+// The NPT nodes a lane traces in one velocity pass of the x-major FP64 kernel.
+template <int NPT>
+struct xm3_nodes
+{
+    double sx[NPT], sy[NPT], sz[NPT], vu[NPT], vv[NPT], vw[NPT];
+    bool act[NPT];
+
+    __device__ __forceinline__ void start( const trace3_params &p, const xm3_lane &a, uint32_t pass )
+    {
+        #pragma unroll
+        for ( int r = 0; r < NPT; ++r )
+        {
+            act[r] = xm3_velocity( p, a, pass + r, vu[r], vv[r], vw[r] );
+            sx[r] = double( a.ix ); sy[r] = double( a.iy ); sz[r] = double( a.iz );
+        }
+    }
+
+    // slices mlo+cnt-1 .. mlo with the chunk's slices at Cb + jj*st
+    __device__ __forceinline__ void steps( const fast3_consts &k3, const double *Cb, uint32_t st, int64_t mlo, int cnt )
+    {
+        for ( int jj = cnt - 1; jj >= 0; --jj )
+        {
+            const double *C = Cb + size_t( jj ) * st;
+            const double hf = ( mlo + jj == 0 ) ? 0.5 : 1.0;
+            #pragma unroll
+            for ( int r = 0; r < NPT; ++r ) step3_fast( k3, C, hf, false, sx[r], sy[r], sz[r], vu[r], vv[r], vw[r] );
+        }
+    }
+
+    // f0 at the feet, added to the lane's compensated sum
+    __device__ __forceinline__ void finish( const trace3_params &p, csum &acc ) const
+    {
+        #pragma unroll
+        for ( int r = 0; r < NPT; ++r )
+        {
+            const double f = act[r]
+                ? f0_3d( p.f0, p.x_min + sx[r]*p.dx, p.y_min + sy[r]*p.dy, p.z_min + sz[r]*p.dz, vu[r], vv[r], vw[r] )
+                : 0.0;
+            acc = csum_add( acc, csum { f, 0.0 } );
+        }
+    }
+};
+
 template <int BLOCK, int NPT>
 __global__ __launch_bounds__( BLOCK )
 void trace3d_xm_fast_kernel( trace3_params p )
@@ -370,41 +424,61 @@ void trace3d_xm_fast_kernel( trace3_params p )
     const fast3_consts k3 = make_fast3( p );
     const uint32_t st = p.st;
     csum acc { 0.0, 0.0 };
-
-    for ( uint32_t pass = 0; pass < p.vgroup; pass += NPT )
+    xm3_nodes<NPT> nd;
+    const uint32_t Cn = p.chunk;
+    const int64_t top = int64_t( p.n ) - 1;
+    const int64_t K = ( top + int64_t( Cn ) ) / int64_t( Cn );      // chunks per pass (0 at n = 0)
+    auto lo_of = [&]( int64_t k ) { const int64_t h = top - k * Cn; return h - int64_t( Cn ) + 1 > 0 ? h - int64_t( Cn ) + 1 : 0; };
+    const uint32_t P = p.vgroup / NPT;                                // passes
+    if ( p.dma && K > 0 )
     {
-        double sx[NPT], sy_[NPT], sz_[NPT], vu[NPT], vv[NPT], vw[NPT];
-        bool act[NPT];
-        #pragma unroll
-        for ( int r = 0; r < NPT; ++r )
+        // The (pass, chunk) sequence in one loop: the chunks alternate between
+        // two LDS buffers (chunk*st + 2 doubles each); the next chunk, of this
+        // pass or the next, is copied straight into the other buffer while this
+        // one is traced, and one barrier per chunk publishes it.
+        double *const buf0 = lds, *const buf1 = lds + size_t( Cn ) * st + 2;
+        const int64_t T = int64_t( P ) * K;
+        const double *C0;
         {
-            act[r] = xm3_velocity( p, a, pass + r, vu[r], vv[r], vw[r] );
-            sx[r] = double( a.ix ); sy_[r] = double( a.iy ); sz_[r] = double( a.iz );
+            const int64_t l0 = lo_of( 0 );
+            C0 = dma_doubles<BLOCK>( buf0, p.hist + size_t( l0 ) * st, uint32_t( top - l0 + 1 ) * st );
+            dma_publish();
         }
-        for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
+        const double *cur = C0;
+        for ( int64_t i = 0; i < T; ++i )
         {
-            const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
-            const int cnt = int( mhi - mlo + 1 );
-            __syncthreads();
-            stage3<BLOCK>( lds, p.hist + size_t( mlo ) * st, uint32_t( cnt ) * st );
-            __syncthreads();
-            for ( int jj = cnt - 1; jj >= 0; --jj )
+            const int64_t k = i % K;
+            if ( k == 0 ) nd.start( p, a, uint32_t( i / K ) * NPT );
+            const int64_t mlo = lo_of( k );
+            const int cnt = int( top - k * Cn - mlo + 1 );
+            const double *next = nullptr;
+            if ( i + 1 < T )
             {
-                const double *C = lds + size_t( jj ) * st;
-                const double hf = ( mlo + jj == 0 ) ? 0.5 : 1.0;
-                #pragma unroll
-                for ( int r = 0; r < NPT; ++r )
-                    step3_fast( k3, C, hf, false, sx[r], sy_[r], sz_[r], vu[r], vv[r], vw[r] );
+                const int64_t k1 = ( k + 1 ) % K, l1 = lo_of( k1 );
+                next = dma_doubles<BLOCK>( ( i & 1 ) ? buf0 : buf1, p.hist + size_t( l1 ) * st,
+                                           uint32_t( top - k1 * Cn - l1 + 1 ) * st );
             }
+            nd.steps( k3, cur, st, mlo, cnt );
+            if ( k == K - 1 ) nd.finish( p, acc );
+            dma_publish();
+            cur = next;
         }
-        #pragma unroll
-        for ( int r = 0; r < NPT; ++r )
+    }
+    else
+    {
+        for ( uint32_t pass = 0; pass < P; ++pass )
         {
-            const double f = act[r]
-                ? f0_3d( p.f0, p.x_min + sx[r]*p.dx, p.y_min + sy_[r]*p.dy, p.z_min + sz_[r]*p.dz,
-                         vu[r], vv[r], vw[r] )
-                : 0.0;
-            acc = csum_add( acc, csum { f, 0.0 } );
+            nd.start( p, a, pass * NPT );
+            for ( int64_t k = 0; k < K; ++k )
+            {
+                const int64_t mlo = lo_of( k );
+                const int cnt = int( top - k * Cn - mlo + 1 );
+                __syncthreads();
+                stage3<BLOCK>( lds, p.hist + size_t( mlo ) * st, uint32_t( cnt ) * st );
+                __syncthreads();
+                nd.steps( k3, lds, st, mlo, cnt );
+            }
+            nd.finish( p, acc );
         }
     }
     if ( a.ok ) p.partials[ size_t( a.g ) * ( size_t( p.Nx ) * p.Ny * p.Nz ) + a.l ] = acc.s + acc.c;
@@ -757,6 +831,8 @@ size_t trace3d_lds_bytes( const trace3_params &p, int variant, const launch_shap
     const uint64_t c = v < uint64_t( s.chunk ) ? ( v > 0 ? v : 1 ) : uint64_t( s.chunk );
     if ( variant == TRACE_FP32C && p.mode == MODE_RHO )        // the buffers of the FP32 slices
         return ( fp32c3_dma( uint32_t( s.chunk ), p.st32 ) ? 2 : 1 ) * size_t( c ) * p.st32 * sizeof(float);
+    if ( variant == TRACE_FAST && p.mode == MODE_RHO && fast3_dma( uint32_t( s.chunk ), p.st ) )
+        return 2 * ( size_t( c ) * p.st + 2 ) * sizeof(double);
     return size_t( c ) * p.st * sizeof(double);
 }
 
@@ -783,7 +859,8 @@ hipError_t launch_trace3d( const trace3_params &p_in, int variant, const launch_
     p.chunk = uint32_t( s.chunk );
     const uint64_t v = visited3( p );
     if ( v < p.chunk ) p.chunk = v > 0 ? uint32_t( v ) : 1u;
-    p.dma = ( p.mode == MODE_RHO && variant == TRACE_FP32C && fp32c3_dma( uint32_t( s.chunk ), p.st32 ) ) ? 1u : 0u;
+    p.dma = p.mode == MODE_RHO && ( ( variant == TRACE_FP32C && fp32c3_dma( uint32_t( s.chunk ), p.st32 ) ) ||
+                                    ( variant == TRACE_FAST && fast3_dma( uint32_t( s.chunk ), p.st ) ) ) ? 1u : 0u;
     const size_t lds = trace3d_lds_bytes( p, variant, s );
     if ( lds > 160u * 1024u ) return hipErrorInvalidValue;
 
