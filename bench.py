@@ -88,7 +88,7 @@ def landau3d_forms(args):
     import numpy as np
     import numericalflowiteration_amd as nufi
     c5 = nufi.landau_3d()
-    t3 = None if args.no_autotune else 16
+    t3 = None if args.no_autotune else c5.Nt // 2     # a mid-run step: short probes pick chunks that lose later
     r64, f64 = timed_runs(c5, nufi.FP64, 1, 0, FLOPS_3D, t3)
     progress(f"3D3V Landau FP64 {r64['ms_per_run'] / 1e3:.1f} s per run")
     r32, f32 = timed_runs(c5, nufi.FP32C, 1, 0, FLOPS_3D, t3)

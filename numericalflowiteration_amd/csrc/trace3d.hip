@@ -217,20 +217,36 @@ __device__ __forceinline__ void grad_window( const T *__restrict__ c, uint32_t s
     }
 }
 
-// one axis of the fast FP64 step: drift (unless `still`), exact wrap, locate
+// one axis of the fast FP64 step: drift (unless `still`), exact wrap, locate.
+// MASK (power-of-two axis): no wrap here, the cell index wraps by a mask and
+// the caller re-wraps the position exactly before every slice m with
+// wrap3_due(m), a schedule fixed by m alone.
+constexpr int WRAP3 = 16;
+__device__ __forceinline__ bool wrap3_due( int64_t m ) { return ( m & ( WRAP3 - 1 ) ) == WRAP3 - 1; }
+
+template <bool MASK>
 __device__ __forceinline__ int advance_axis( double &s, double vel, double cd, double N, double invN, int Ni,
                                              double &t, bool still = false )
 {
     double x = still ? s : fma( -cd, vel, s );
-    const double w = floor( x * invN );
-    x = fma( -N, w, x );
+    if ( !MASK ) x = fma( -N, floor( x * invN ), x );
     const double k = floor( x );
     t = x - k;
     int i = int( k );
-    i = ( i < 0 ) ? i + Ni : i;
-    i = ( i >= Ni ) ? i - Ni : i;
+    if ( MASK ) i &= Ni - 1;
+    else
+    {
+        i = ( i < 0 ) ? i + Ni : i;
+        i = ( i >= Ni ) ? i - Ni : i;
+    }
     s = x;
     return i;
+}
+
+// exact periodic re-wrap (N*floor(s/N) is an integer)
+__device__ __forceinline__ void rewrap3( double &s, double N, double invN )
+{
+    s = fma( -N, floor( s * invN ), s );
 }
 
 // one fast FP64 characteristic step of one node with slice C: drift (unless
@@ -254,13 +270,14 @@ __device__ __forceinline__ fast3_consts make_fast3( const trace3_params &p )
     return k;
 }
 
+template <bool MASK = false>
 __device__ __forceinline__ void step3_fast( const fast3_consts &k, const double *C, double hf, bool still,
                                             double &sx, double &sy_, double &sz_, double &vu, double &vv, double &vw )
 {
     double tx, ty, tz;
-    const int i = advance_axis( sx,  vu, k.cdx, k.Nxd, k.iNx, k.Nxi, tx, still );
-    const int j = advance_axis( sy_, vv, k.cdy, k.Nyd, k.iNy, k.Nyi, ty, still );
-    const int l = advance_axis( sz_, vw, k.cdz, k.Nzd, k.iNz, k.Nzi, tz, still );
+    const int i = advance_axis<MASK>( sx,  vu, k.cdx, k.Nxd, k.iNx, k.Nxi, tx, still );
+    const int j = advance_axis<MASK>( sy_, vv, k.cdy, k.Nyd, k.iNy, k.Nyi, ty, still );
+    const int l = advance_axis<MASK>( sz_, vw, k.cdz, k.Nzd, k.iNz, k.Nzi, tz, still );
     double Bx[4], Dx[4], By[4], Dy[4], Bz[4], Dz[4];
     cubic_bases( tx, Bx, Dx );
     cubic_bases( ty, By, Dy );
@@ -372,7 +389,7 @@ __device__ __forceinline__ bool xm3_velocity( const trace3_params &p, const xm3_
 }
 
 // The NPT nodes a lane traces in one velocity pass of the x-major FP64 kernel.
-template <int NPT>
+template <int NPT, bool MASK>
 struct xm3_nodes
 {
     double sx[NPT], sy[NPT], sz[NPT], vu[NPT], vv[NPT], vw[NPT];
@@ -395,8 +412,17 @@ struct xm3_nodes
         {
             const double *C = Cb + size_t( jj ) * st;
             const double hf = ( mlo + jj == 0 ) ? 0.5 : 1.0;
+            if ( MASK && wrap3_due( mlo + jj ) )
+            {
+                #pragma unroll
+                for ( int r = 0; r < NPT; ++r )
+                {
+                    rewrap3( sx[r], k3.Nxd, k3.iNx ); rewrap3( sy[r], k3.Nyd, k3.iNy ); rewrap3( sz[r], k3.Nzd, k3.iNz );
+                }
+            }
             #pragma unroll
-            for ( int r = 0; r < NPT; ++r ) step3_fast( k3, C, hf, false, sx[r], sy[r], sz[r], vu[r], vv[r], vw[r] );
+            for ( int r = 0; r < NPT; ++r )
+                step3_fast<MASK>( k3, C, hf, false, sx[r], sy[r], sz[r], vu[r], vv[r], vw[r] );
         }
     }
 
@@ -414,7 +440,7 @@ struct xm3_nodes
     }
 };
 
-template <int BLOCK, int NPT>
+template <int BLOCK, int NPT, bool MASK>
 __global__ __launch_bounds__( BLOCK )
 void trace3d_xm_fast_kernel( trace3_params p )
 {
@@ -424,7 +450,7 @@ void trace3d_xm_fast_kernel( trace3_params p )
     const fast3_consts k3 = make_fast3( p );
     const uint32_t st = p.st;
     csum acc { 0.0, 0.0 };
-    xm3_nodes<NPT> nd;
+    xm3_nodes<NPT,MASK> nd;
     const uint32_t Cn = p.chunk;
     const int64_t top = int64_t( p.n ) - 1;
     const int64_t K = ( top + int64_t( Cn ) ) / int64_t( Cn );      // chunks per pass (0 at n = 0)
@@ -485,6 +511,18 @@ void trace3d_xm_fast_kernel( trace3_params p )
 }
 
 // ---- FP32 compensated --------------------------------------------------------
+// Exact periodic wrap of the pair sh + sl: sh - N*w is exact (Sterbenz) for
+// w >= 1 and for w <= -2, and Fast2Sum is exact for w = -1 (|N| >= |sh|).
+__device__ __forceinline__ void rewrap32( float &sh, float &sl, float N, float invN )
+{
+    const float w = floorf( sh * invN );
+    float y2, e2;
+    fast_two_sum( -N * w, sh, y2, e2 );
+    sh = y2; sl = sl + e2;
+}
+
+// drift with TwoProd/TwoSum, wrap (unless MASK, see advance_axis), locate
+template <bool MASK>
 __device__ __forceinline__ int advance_axis32( float &sh, float &sl, float uh, float ul,
                                                float cdh, float cdl, float N, float invN, int Ni, float &t )
 {
@@ -495,15 +533,16 @@ __device__ __forceinline__ int advance_axis32( float &sh, float &sl, float uh, f
     two_sum( sh, -pr, y, e );
     e = e + ( sl - pe );
     fast_two_sum( y, e, sh, sl );
-    const float w = floorf( sh * invN );
-    float y2, e2;
-    fast_two_sum( -N * w, sh, y2, e2 );
-    sh = y2; sl = sl + e2;
+    if ( !MASK ) rewrap32( sh, sl, N, invN );
     const float k = floorf( sh );
     t = ( sh - k ) + sl;
     int i = int( k );
-    i = ( i < 0 ) ? i + Ni : i;
-    i = ( i >= Ni ) ? i - Ni : i;
+    if ( MASK ) i &= Ni - 1;
+    else
+    {
+        i = ( i < 0 ) ? i + Ni : i;
+        i = ( i >= Ni ) ? i - Ni : i;
+    }
     return i;
 }
 
@@ -516,7 +555,7 @@ __device__ __forceinline__ void kick32( float &uh, float &ul, float q )
 
 // velocity-major: node r of a lane is make_node3's (wave task g, lane), the
 // wave's 64 lanes are 64 consecutive velocities of one spatial point
-template <int BLOCK, int NPT>
+template <int BLOCK, int NPT, bool MASK>
 __global__ __launch_bounds__( BLOCK )
 void trace3d_fp32c_kernel( trace3_params p )
 {
@@ -588,13 +627,21 @@ void trace3d_fp32c_kernel( trace3_params p )
         {
             const float *Cs = cur + size_t( jj ) * st;
             const float hf = ( mlo + jj == 0 ) ? 0.5f : 1.0f;
+            if ( MASK && wrap3_due( mlo + jj ) )
+            {
+                #pragma unroll
+                for ( int r = 0; r < NPT; ++r )
+                {
+                    rewrap32( xh[r], xl[r], Nxf, iNx ); rewrap32( yh[r], yl[r], Nyf, iNy ); rewrap32( zh[r], zl[r], Nzf, iNz );
+                }
+            }
             #pragma unroll
             for ( int r = 0; r < NPT; ++r )
             {
                 float tx, ty, tz;
-                const int i = advance_axis32( xh[r], xl[r], uh[r], ul[r], cxh, cxl, Nxf, iNx, Nxi, tx );
-                const int j = advance_axis32( yh[r], yl[r], vh[r], vl[r], cyh, cyl, Nyf, iNy, Nyi, ty );
-                const int k = advance_axis32( zh[r], zl[r], wh[r], wl[r], czh, czl, Nzf, iNz, Nzi, tz );
+                const int i = advance_axis32<MASK>( xh[r], xl[r], uh[r], ul[r], cxh, cxl, Nxf, iNx, Nxi, tx );
+                const int j = advance_axis32<MASK>( yh[r], yl[r], vh[r], vl[r], cyh, cyl, Nyf, iNy, Nyi, ty );
+                const int k = advance_axis32<MASK>( zh[r], zl[r], wh[r], wl[r], czh, czl, Nzf, iNz, Nzi, tz );
                 float Bx[4], Dx[4], By[4], Dy[4], Bz[4], Dz[4];
                 cubic_bases( tx, Bx, Dx );
                 cubic_bases( ty, By, Dy );
@@ -789,21 +836,29 @@ hipError_t launch3_npt( const trace3_params &p, int variant, int npt, uint32_t g
         default: return hipErrorInvalidValue;
         }
     }
+    // power-of-two axes: the cell index wraps by a mask, positions every 16 slices
+    const bool mask = ( p.Nx & ( p.Nx - 1 ) ) == 0 && ( p.Ny & ( p.Ny - 1 ) ) == 0 && ( p.Nz & ( p.Nz - 1 ) ) == 0;
     if ( variant == TRACE_FP32C )
     {
-        switch ( npt )
+        switch ( npt * 2 + ( mask ? 1 : 0 ) )
         {
-        case 1: return launch3( trace3d_fp32c_kernel<BLOCK,1>, grid, BLOCK, lds, p, st );
-        case 2: return launch3( trace3d_fp32c_kernel<BLOCK,2>, grid, BLOCK, lds, p, st );
-        case 4: return launch3( trace3d_fp32c_kernel<BLOCK,4>, grid, BLOCK, lds, p, st );
+        case 2: return launch3( trace3d_fp32c_kernel<BLOCK,1,false>, grid, BLOCK, lds, p, st );
+        case 3: return launch3( trace3d_fp32c_kernel<BLOCK,1,true>,  grid, BLOCK, lds, p, st );
+        case 4: return launch3( trace3d_fp32c_kernel<BLOCK,2,false>, grid, BLOCK, lds, p, st );
+        case 5: return launch3( trace3d_fp32c_kernel<BLOCK,2,true>,  grid, BLOCK, lds, p, st );
+        case 8: return launch3( trace3d_fp32c_kernel<BLOCK,4,false>, grid, BLOCK, lds, p, st );
+        case 9: return launch3( trace3d_fp32c_kernel<BLOCK,4,true>,  grid, BLOCK, lds, p, st );
         default: return hipErrorInvalidValue;
         }
     }
-    switch ( npt )
+    switch ( npt * 2 + ( mask ? 1 : 0 ) )
     {
-    case 1: return launch3( trace3d_xm_fast_kernel<BLOCK,1>, grid, BLOCK, lds, p, st );
-    case 2: return launch3( trace3d_xm_fast_kernel<BLOCK,2>, grid, BLOCK, lds, p, st );
-    case 4: return launch3( trace3d_xm_fast_kernel<BLOCK,4>, grid, BLOCK, lds, p, st );
+    case 2: return launch3( trace3d_xm_fast_kernel<BLOCK,1,false>, grid, BLOCK, lds, p, st );
+    case 3: return launch3( trace3d_xm_fast_kernel<BLOCK,1,true>,  grid, BLOCK, lds, p, st );
+    case 4: return launch3( trace3d_xm_fast_kernel<BLOCK,2,false>, grid, BLOCK, lds, p, st );
+    case 5: return launch3( trace3d_xm_fast_kernel<BLOCK,2,true>,  grid, BLOCK, lds, p, st );
+    case 8: return launch3( trace3d_xm_fast_kernel<BLOCK,4,false>, grid, BLOCK, lds, p, st );
+    case 9: return launch3( trace3d_xm_fast_kernel<BLOCK,4,true>,  grid, BLOCK, lds, p, st );
     default: return hipErrorInvalidValue;
     }
 }

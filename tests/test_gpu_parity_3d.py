@@ -140,3 +140,27 @@ def test_rho_bit_identical_across_launch_shapes_3d(prec):
             s.set_launch_shape(1024, 1, 1)         # not built in 3D (would spill)
     for r in out[1:]:
         assert np.array_equal(r, out[0])
+
+
+@pytest.mark.parametrize("N", [8, 6])
+@pytest.mark.parametrize("name,prec,variant,tol", VARIANTS[1:])
+def test_rho_parity_3d_long_history(N, name, prec, variant, tol):
+    """Steps past slice 15, where the power-of-two kernels re-wrap positions
+    (their cell indices wrap by a mask in between), and a 6^3 grid, whose
+    kernels wrap exactly every step; history from the oracle's own run."""
+    conf = nufi.config_3d((N, N, N), (6, 6, 6), 1 / 8, 3.0, (0.0, 4 * np.pi), (-6.0, 6.0),
+                          nufi.config.F0_LANDAU_3D, 0.05, 0.5)
+    hist = O.new_history(conf, 24 + 1)
+    O.run(conf, 0, 24, hist)
+    print(f"[3d long history] N={N} {name}", flush=True)
+    with nufi.GpuScheduler(conf, precision=prec) as s:
+        s.set_trace_variant(variant)
+        s.upload_history(hist[: 24 * conf.stride_t()], 0, 24)
+        for n in (16, 24):
+            rho = np.zeros(conf.n_space())
+            s.compute_rho(n)
+            s.download_rho(rho)
+            exact = O.rho(conf, n, hist, exact_sum=True)
+            err = rel(mean_free(rho), mean_free(exact))
+            print(f"  n={n} rel err {err:.3e}", flush=True)
+            assert err <= tol, (n, err)
