@@ -377,7 +377,7 @@ void trace1d_poly_kernel( trace1_params p )
 // ---- FP32 compensated (order 4) --------------------------------------------
 // The FP32 kick-polynomial table: an 8-byte (Q0, Q1) and a 4-byte Q2 LDS
 // read per char-step.
-template <int BLOCK, int NPT>
+template <int BLOCK, int NPT, bool POW2>
 __device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint32_t wg, float *ldsf )
 {
     constexpr int WPB = BLOCK / WAVE;
@@ -395,10 +395,20 @@ __device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint
 
     const float Nxf   = float( p.Nx );
     const float invNx = 1.0f / Nxf;
+    const int mask    = int( p.Nx ) - 1;
     const double cd   = p.dt * p.dx_inv;
     const float cd_hi = float( cd ), cd_lo = float( cd - double( cd_hi ) );
     const uint32_t sq = p.stride_q, q2 = p.q2_off, cells = p.Nx + 2;
 
+    // exact periodic wrap of the pair: sh - Nx*w is exact (Sterbenz) for w >= 1
+    // and w <= -2, and Fast2Sum is exact for w = -1 (Nx >= |sh|)
+    auto wrap = [&]( int r )
+    {
+        const float w = floorf( sh[r] * invNx );
+        float y2, e2;
+        fast_two_sum( -Nxf * w, sh[r], y2, e2 );
+        sh[r] = y2; sl[r] = sl[r] + e2;
+    };
     auto step = [&]( const float *Q, float hf )
     {
         #pragma unroll
@@ -412,15 +422,13 @@ __device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint
             two_sum( sh[r], -pr, y, e );
             e = e + ( sl[r] - pe );
             fast_two_sum( y, e, sh[r], sl[r] );
-            // exact periodic wrap into [0, Nx)
-            const float w = floorf( sh[r] * invNx );
-            float y2, e2;
-            fast_two_sum( -Nxf * w, sh[r], y2, e2 );
-            sh[r] = y2; sl[r] = sl[r] + e2;
+            // exact periodic wrap into [0, Nx); POW2: the cell index wraps by
+            // the mask and the position is re-wrapped every POLY_WRAP slices
+            if ( !POW2 ) wrap( r );
             // locate
             const float k = floorf( sh[r] );
             const float t = ( sh[r] - k ) + sl[r];
-            const int i = int( k );                   // in [-1, Nx]
+            const int i = POW2 ? ( int( k ) & mask ) : int( k );     // in [-1, Nx]
             const float2 q01 = reinterpret_cast<const float2*>( Q )[ i + 1 ];
             const float kq = fmaf( fmaf( Q[ q2 + ( i + 1 ) ], t, q01.y ), t, q01.x );
             // kick: u += h*kq with TwoSum
@@ -429,14 +437,25 @@ __device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint
             uh[r] = y3; ul[r] = ul[r] + e3;
         }
     };
-    // slices mlo+hi down to mlo+lo; slice 0 (the half kick) peeled
+    // slices mlo+hi down to mlo+lo, in segments between re-wrap points;
+    // slice 0 (the half kick) peeled
     auto steps = [&]( const float *Qc, int mlo, int hi, int lo )
     {
         int m = mlo + hi;
-        const int mend = mlo + lo, s1 = mend > 0 ? mend : 1;
+        const int mend = mlo + lo;
         const float *Q = Qc + hi * sq;
-        for ( ; m >= s1; --m, Q -= sq ) step( Q, 1.0f );
-        if ( mend == 0 && m == 0 ) step( Q, 0.5f );
+        while ( m >= mend )
+        {
+            if ( POW2 && ( m & ( POLY_WRAP - 1 ) ) == POLY_WRAP - 1 )
+            {
+                #pragma unroll
+                for ( int r = 0; r < NPT; ++r ) wrap( r );
+            }
+            const int seg = POW2 ? ( ( m & ~( POLY_WRAP - 1 ) ) > mend ? ( m & ~( POLY_WRAP - 1 ) ) : mend ) : mend;
+            const int s1 = seg > 0 ? seg : 1;
+            for ( ; m >= s1; --m, Q -= sq ) step( Q, 1.0f );
+            if ( seg == 0 && m == 0 ) { step( Q, 0.5f ); --m; Q -= sq; }
+        }
     };
     if ( p.pipe ) poly_pipeline<BLOCK,poly_maxc( BLOCK ),2>( ldsf, p.dhist, p.stride_d, sq, q2, cells, p.kick, p.n, p.chunk, steps );
     else          poly_direct<BLOCK>( ldsf, p.dhist, p.stride_d, sq, q2, cells, p.kick, p.n, p.chunk, steps );
@@ -452,12 +471,12 @@ __device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint
     xm_reduce<BLOCK,NPT>( p, tile, id, f, reinterpret_cast<double*>( ldsf ) );
 }
 
-template <int BLOCK, int NPT>
+template <int BLOCK, int NPT, bool POW2>
 __global__ __launch_bounds__( BLOCK )
 void trace1d_fp32c_kernel( trace1_params p )
 {
     extern __shared__ float ldsf[];
-    trace1d_fp32c_body<BLOCK,NPT>( p, blockIdx.x, ldsf );
+    trace1d_fp32c_body<BLOCK,NPT,POW2>( p, blockIdx.x, ldsf );
 }
 
 // ---- batched sweep: one launch over the wave-tasks of many independent runs --
@@ -474,9 +493,14 @@ void trace1d_batch_kernel( const batch_run *__restrict__ runs, const uint32_t *_
     trace1_params p = runs[r].p;
     p.n = n;
     const uint32_t wg = blockIdx.x - runs[r].wg_begin;
-    if ( VARIANT == TRACE_FP32C ) trace1d_fp32c_body<BLOCK,NPT>( p, wg, reinterpret_cast<float*>( lds_b ) );
-    else if ( ( p.Nx & ( p.Nx - 1 ) ) == 0 ) trace1d_poly_body<BLOCK,NPT,true>( p, wg, lds_b );
-    else                          trace1d_poly_body<BLOCK,NPT,false>( p, wg, lds_b );
+    const bool pow2 = ( p.Nx & ( p.Nx - 1 ) ) == 0;
+    if ( VARIANT == TRACE_FP32C )
+    {
+        if ( pow2 ) trace1d_fp32c_body<BLOCK,NPT,true>( p, wg, reinterpret_cast<float*>( lds_b ) );
+        else        trace1d_fp32c_body<BLOCK,NPT,false>( p, wg, reinterpret_cast<float*>( lds_b ) );
+    }
+    else if ( pow2 ) trace1d_poly_body<BLOCK,NPT,true>( p, wg, lds_b );
+    else        trace1d_poly_body<BLOCK,NPT,false>( p, wg, lds_b );
 }
 
 template <typename K>
@@ -499,7 +523,11 @@ hipError_t launch_fast_or_32( const trace1_params &p, int variant, uint32_t grid
     if ( p.mode == MODE_METRICS )
         return launch_kernel( trace1d_fast_kernel<BLOCK,NPT,MODE_METRICS>, grid, BLOCK, lds, p, st );
     if ( variant == TRACE_FP32C )
-        return launch_kernel( trace1d_fp32c_kernel<BLOCK,NPT>, grid, BLOCK, lds, p, st );
+    {
+        if ( ( p.Nx & ( p.Nx - 1 ) ) == 0 )
+            return launch_kernel( trace1d_fp32c_kernel<BLOCK,NPT,true>, grid, BLOCK, lds, p, st );
+        return launch_kernel( trace1d_fp32c_kernel<BLOCK,NPT,false>, grid, BLOCK, lds, p, st );
+    }
     if ( ( p.Nx & ( p.Nx - 1 ) ) == 0 )
         return launch_kernel( trace1d_poly_kernel<BLOCK,NPT,true>, grid, BLOCK, lds, p, st );
     return launch_kernel( trace1d_poly_kernel<BLOCK,NPT,false>, grid, BLOCK, lds, p, st );
