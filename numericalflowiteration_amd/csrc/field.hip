@@ -15,7 +15,7 @@
 // slot converges to).  A real, folded multiplier commutes with the Hartley
 // kernel, so DHT-scale-DHT equals FFT-scale-IFFT and the spectral energy
 // sum_k H_k^2 w_k equals sum_k |F_k|^2 w_k.  The transform is a batched
-// radix-2 Stockham FFT along each axis (a direct DFT for an axis whose length
+// radix-4 Stockham FFT along each axis (a direct DFT for an axis whose length
 // is not a power of two), ping-ponged between two LDS buffers.
 #include "device_common.hpp"
 #include "internal.hpp"
@@ -35,7 +35,7 @@ namespace
 __device__ __forceinline__ double sum_partials( const double *__restrict__ partials, uint32_t wpx, size_t n,
                                                 uint32_t i )
 {
-    constexpr uint32_t RB = 16;
+    constexpr uint32_t RB = 64;
     csum acc { 0.0, 0.0 };
     for ( uint32_t j0 = 0; j0 < wpx; j0 += RB )
     {
@@ -182,7 +182,7 @@ void field_batch_kernel( const batch_field *__restrict__ runs, uint64_t n )
     field_body( f, lds2b );
 }
 
-__global__ void reduce_partials_kernel( const double *partials, uint32_t wpx, double weight,
+__global__ __launch_bounds__( 64 ) void reduce_partials_kernel( const double *partials, uint32_t wpx, double weight,
                                         uint32_t first, uint32_t count, uint32_t n_space, double *rho )
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
