@@ -46,6 +46,7 @@ struct nufi_ctx
     launch_shape shape { 256, 1, 1 };
 
     double *hist = nullptr, *dhist = nullptr;
+    double *qhist = nullptr;   // 1D FP64: kick-polynomial tables of every slice (stride sq), or null
     float *hist32 = nullptr;
     uint32_t sq = 0;           // 1D: LDS kick-polynomial table stride (poly_stride)
     double *partials = nullptr, *rho = nullptr, *energies = nullptr, *values = nullptr;
@@ -109,7 +110,7 @@ int dev_alloc( nufi_ctx *c, T **p, size_t count, const char *what )
 
 void free_all( nufi_ctx *c )
 {
-    void *ptrs[] = { c->hist, c->dhist, c->hist32, c->partials, c->rho, c->energies,
+    void *ptrs[] = { c->hist, c->dhist, c->qhist, c->hist32, c->partials, c->rho, c->energies,
                      c->values, c->scratch_slice, c->metrics, c->mpartials, c->energy_tmp,
                      c->rho_hist, c->lambda, c->gvec, c->grad_part, c->grad, c->tw_x, c->tw_y, c->tw_z,
                      c->mult_step, c->mult_pois, c->mult_fit, c->ewt, c->samp };
@@ -164,6 +165,12 @@ uint32_t rho_ppx( const nufi_ctx *c )
     return xm ? c->ppx : c->wpx;
 }
 
+// the 1D FP64 rho launches copy the ready tables of qhist
+bool tables_ready( const nufi_ctx *c )
+{
+    return c->dim == 1 && c->variant == TRACE_FAST && c->qhist != nullptr;
+}
+
 // The chunk a rho launch of shape s uses and its LDS: 1D x-major launches
 // plan the requested chunk within 160 KiB (poly_chunk_plan).
 int effective_chunk( const nufi_ctx *c, const launch_shape &s, size_t *lds = nullptr )
@@ -187,7 +194,8 @@ int effective_chunk( const nufi_ctx *c, const launch_shape &s, size_t *lds = nul
         return s.chunk;
     }
     bool pipe = false;
-    const uint32_t ch = poly_chunk_plan( uint32_t( c->conf.Nx ), s.block, elem, uint64_t( s.chunk ), 160 * 1024, &pipe );
+    const uint32_t ch = poly_chunk_plan( uint32_t( c->conf.Nx ), s.block, elem, uint64_t( s.chunk ), 160 * 1024, &pipe,
+                                         tables_ready( c ) );
     if ( lds ) *lds = ( pipe ? 2 : 1 ) * size_t( ch ) * row * elem;
     return int( ch );
 }
@@ -216,7 +224,8 @@ void default_shape( nufi_ctx *c )
         {
             bool pipe = false;
             c->shape = launch_shape { 1024, 1, int( poly_chunk_plan( uint32_t( c->conf.Nx ), 1024, elem, 1u << 20,
-                                                                    std::min<size_t>( c->lds_cap, 160 * 1024 ), &pipe ) ) };
+                                                                    std::min<size_t>( c->lds_cap, 160 * 1024 ), &pipe,
+                                                                    tables_ready( c ) ) ) };
         }
     }
     else
@@ -262,6 +271,7 @@ int launch_trace( nufi_ctx *c, uint64_t n, uint64_t q_begin, uint64_t q_end, uin
     {
         trace1_params p {};
         p.dhist = c->dhist; p.partials = c->partials;
+        p.qhist = c->variant == TRACE_FAST ? c->qhist : nullptr;
         p.stride_q = c->sq; p.q2_off = poly_q2( uint32_t( c->conf.Nx ) ); p.kick = -0.5 * c->conf.dt * c->conf.dx_inv;
         p.n = n; p.q_begin = q_begin; p.q_end = q_end;
         p.Nx = uint32_t( c->conf.Nx ); p.Nu = uint32_t( c->conf.Nu ); p.stride_d = c->sd; p.wpx = c->wpx;
@@ -440,6 +450,9 @@ int nufi_create( const nufi_config *conf, const nufi_config *conf_metrics, int o
     if ( ( rc = dev_alloc( c, &c->hist, slices * c->st + 8, "history" ) ) ) return bail( rc );
     if ( c->dim == 1 && ( rc = dev_alloc( c, &c->dhist, slices * c->sd + 8, "difference history" ) ) ) return bail( rc );
     if ( c->dim == 1 ) c->sq = poly_stride( uint32_t( conf->Nx ) );
+    // the FP64 rho kernel copies these tables instead of building them (3 reals per cell and slice)
+    if ( c->dim == 1 && precision == NUFI_FP64 && order == 4 &&
+         ( rc = dev_alloc( c, &c->qhist, slices * c->sq + 8, "table history" ) ) ) return bail( rc );
     c->st32 = ( c->st + 3u ) & ~3u;
     if ( precision == NUFI_FP32C && c->dim == 3 && ( rc = dev_alloc( c, &c->hist32, slices * c->st32 + 8, "FP32 history" ) ) )
         return bail( rc );
@@ -538,6 +551,7 @@ int nufi_upload_phi( nufi_ctx *c, uint64_t n, const double *history_base )
     HIP_TRY( c, hipMemcpyAsync( dst, history_base + n * c->st, c->st * sizeof(double), hipMemcpyHostToDevice, c->stream ) );
     hipError_t e = launch_slice_tables( c->dim, dst, c->st, uint32_t( c->conf.Nx ),
                                         c->dhist ? c->dhist + n * c->sd : nullptr,
+                                        c->qhist ? c->qhist + n * c->sq : nullptr,
                                         c->hist32 ? c->hist32 + n * c->st32 : nullptr, c->stream );
     if ( e != hipSuccess ) return hip_fail( c, e, "slice table launch" );
     // the copy reads host memory the caller may reuse: complete it before returning
@@ -649,6 +663,8 @@ static int step_async( nufi_ctx *c, uint64_t n, double *rho_out )
     f.mult = c->mult_step;
     f.slice = c->hist + n * c->st;
     f.dslice = c->dhist ? c->dhist + n * c->sd : nullptr;
+    f.qslice = c->qhist ? c->qhist + n * c->sq : nullptr;
+    f.q2_off = poly_q2( uint32_t( c->conf.Nx ) );
     f.slice32 = c->hist32 ? c->hist32 + n * c->st32 : nullptr;
     f.energy = c->energies + n;
     f.rho_out = rho_out ? rho_out : c->rho;

@@ -126,6 +126,42 @@ __device__ __forceinline__ double wave_sum( double v )
     return v;
 }
 
+// Direct global->LDS copy of n16 16-byte words (gfx950 global_load_lds_dwordx4):
+// lane l of a wave writes word q0 + l of dst, the wave's words starting at
+// q0 (M0 holds the wave's LDS address); the loads hold no registers and are
+// waited for (vmcnt) before the barrier that publishes the buffer (dma_publish).
+// dst and src are 16-byte aligned.  The instruction is issued from inline
+// assembly: seen as an LDS write, the compiler would make every LDS read that
+// follows wait for all copies in flight (vmcnt(0)), including the copy into
+// the other buffer that the reads do not touch.  The compiler's own waits for
+// its loads stay correct with these extra loads outstanding: vector memory
+// loads complete in issue order.
+template <int BLOCK>
+__device__ __forceinline__ void dma_copy16( void *dst, const void *src, uint32_t n16 )
+{
+    typedef __attribute__((address_space(3))) char lds_char;
+    const uint32_t lane = threadIdx.x % WAVE;
+    const uint32_t w = __builtin_amdgcn_readfirstlane( threadIdx.x / WAVE );     // the wave index, scalar
+    const uint32_t base = uint32_t( reinterpret_cast<uintptr_t>( (lds_char*)dst ) );
+    for ( uint32_t q0 = w * WAVE; q0 < n16; q0 += BLOCK )                        // uniform in the wave
+    {
+        if ( q0 + lane < n16 )
+        {
+            const char *g = static_cast<const char*>( src ) + size_t( q0 + lane ) * 16;
+            const uint32_t m0 = base + q0 * 16u;
+            asm volatile( "s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+                          :: "v"( g ), "s"( m0 ) : "memory", "m0" );
+        }
+    }
+}
+
+// wait for this wave's direct LDS loads (vmcnt(0)), then the workgroup barrier
+__device__ __forceinline__ void dma_publish()
+{
+    __builtin_amdgcn_s_waitcnt( 0x0F70 );     // vmcnt(0); expcnt and lgkmcnt not waited for
+    __syncthreads();
+}
+
 } // namespace nufi_dev
 
 #endif

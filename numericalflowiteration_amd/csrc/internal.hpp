@@ -60,6 +60,8 @@ enum trace_mode : int
 struct trace1_params
 {
     const double *dhist;     // extended differences of slice m at dhist + m*stride_d (FP64)
+    const double *qhist;     // FP64 rho: ready kick-polynomial tables, slice m at qhist + m*stride_q
+                             // in the LDS layout (null: built in LDS from dhist)
     uint32_t stride_q, q2_off;        // LDS kick-polynomial table: slice stride, Q2 row (poly_stride, poly_q2)
     double   kick;                    // -dt/(2 dx), the scale of the kick polynomials
     uint32_t pipe;                    // x-major: tables double-buffered and prefetched (poly_pipeline)
@@ -239,6 +241,8 @@ struct field_params
     const double  *ewt;          // per-mode energy weight, N entries
     double  *slice;              // history slice n (coefficients)
     double  *dslice;             // 1D differences of slice n (stride_d)
+    double  *qslice;             // 1D FP64 kick-polynomial table of slice n (poly_stride layout) or null
+    uint32_t q2_off;             // its Q2 row (poly_q2)
     float   *slice32;            // FP32 coefficients (3D FP32C) or null
     double  *energy;             // energies[n]
     uint32_t sy, sz, stride_d;
@@ -268,9 +272,10 @@ hipError_t launch_reduce_metrics( const double *partials, uint32_t n_tasks, doub
 hipError_t launch_reduce_partials( const double *partials, uint32_t wpx, double weight,
                                    uint32_t first, uint32_t count, uint32_t n_space,
                                    double *rho, hipStream_t st );
-// Differences (1D) and the FP32 copy (3D FP32C) of an uploaded slice.
+// Differences and the FP64 kick-polynomial table (1D) and the FP32 copy (3D
+// FP32C) of an uploaded slice.
 hipError_t launch_slice_tables( int dim, const double *slice, uint32_t len, uint32_t Nx, double *dslice,
-                                float *slice32, hipStream_t st );
+                                double *qslice, float *slice32, hipStream_t st );
 
 // The 1D kick-polynomial table of one slice, built in LDS by the fast, FP32C
 // and adjoint kernels from the difference history: for cell i (i = -1 .. Nx,
@@ -289,13 +294,15 @@ __host__ __device__ inline uint32_t poly_q2( uint32_t Nx ) { return 2u * ( Nx + 
 __host__ __device__ inline uint32_t poly_stride( uint32_t Nx ) { return ( 3u * ( Nx + 2u ) + 3u ) & ~3u; }
 // The table chunk of a 1D x-major launch for a requested chunk `req` >= 1:
 // the largest chunk up to req whose windows fit the register prefetch of a
-// workgroup of `block` threads and whose two table buffers (`elem`-byte
-// entries) fit `lds_max` bytes, pipelined; failing that, one buffer of up to
-// req slices built directly (*pipe = false).  At least 1.
-inline uint32_t poly_chunk_plan( uint32_t Nx, int block, size_t elem, uint64_t req, size_t lds_max, bool *pipe )
+// workgroup of `block` threads (no limit when the tables are `ready` in
+// global memory and copied, not built) and whose two table buffers
+// (`elem`-byte entries) fit `lds_max` bytes, pipelined; failing that, one
+// buffer of up to req slices built directly (*pipe = false).  At least 1.
+inline uint32_t poly_chunk_plan( uint32_t Nx, int block, size_t elem, uint64_t req, size_t lds_max, bool *pipe,
+                                 bool ready = false )
 {
     const uint64_t row = uint64_t( poly_stride( Nx ) ) * elem;
-    const uint64_t pf  = uint64_t( block ) * uint64_t( poly_maxc( block ) ) / ( uint64_t( Nx ) + 2u );
+    const uint64_t pf  = ready ? req : uint64_t( block ) * uint64_t( poly_maxc( block ) ) / ( uint64_t( Nx ) + 2u );
     uint64_t c = std::min( { req, pf, uint64_t( lds_max ) / ( 2 * row ) } );
     *pipe = c >= 1;
     if ( c == 0 ) c = std::max<uint64_t>( 1, std::min<uint64_t>( req, uint64_t( lds_max ) / row ) );

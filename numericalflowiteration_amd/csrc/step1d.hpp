@@ -11,6 +11,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "device_common.hpp"
+
 namespace nufi_dev
 {
 
@@ -245,6 +247,36 @@ __device__ __forceinline__ void poly_pipeline( T *lds, const double *dhist, uint
             if ( hi >= lo ) steps( cur, mlo, hi, lo );
         }
         __syncthreads();
+    }
+}
+
+// The same loop over tables already in global memory in the LDS layout (slice
+// m at qhist + m*sq, sq*sizeof(T) a multiple of 16 bytes): while chunk k is
+// traced from one LDS buffer, chunk k+1 is copied into the other by direct
+// global->LDS loads, which take no registers and no VALU; each wave waits for
+// its own copies and the workgroup barrier publishes them at the chunk's end,
+// when chunk k's buffer is also free again.
+template <int BLOCK, typename T, typename STEPS>
+__device__ __forceinline__ void poly_ready_pipeline( T *lds, const T *qhist, uint32_t sq, uint64_t n, uint32_t C,
+                                                     STEPS steps )
+{
+    if ( n == 0 ) return;
+    const int Ci = int( C ), top = int( n ) - 1;
+    const int K = ( top + Ci ) / Ci;
+    const uint32_t w16 = sq * uint32_t( sizeof( T ) ) / 16u;       // 16-byte words per slice
+    auto lo_of  = [&]( int k ) { const int h = top - k * Ci; return h - Ci + 1 > 0 ? h - Ci + 1 : 0; };
+    auto cnt_of = [&]( int k ) { return top - k * Ci - lo_of( k ) + 1; };
+    T *const buf0 = lds, *const buf1 = lds + size_t( C ) * sq;
+    dma_copy16<BLOCK>( buf0, qhist + size_t( lo_of( 0 ) ) * sq, uint32_t( cnt_of( 0 ) ) * w16 );
+    dma_publish();
+    for ( int k = 0; k < K; ++k )
+    {
+        T *const cur = ( k & 1 ) ? buf1 : buf0;
+        T *const nxt = ( k & 1 ) ? buf0 : buf1;
+        if ( k + 1 < K ) dma_copy16<BLOCK>( nxt, qhist + size_t( lo_of( k + 1 ) ) * sq, uint32_t( cnt_of( k + 1 ) ) * w16 );
+        const int cnt = cnt_of( k );
+        steps( cur, lo_of( k ), cnt - 1, 0 );
+        dma_publish();
     }
 }
 

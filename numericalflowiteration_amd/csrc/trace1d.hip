@@ -304,9 +304,10 @@ void trace1d_strict_kernel( trace1_params p )
 
 // ---- fast FP64 with the kick-polynomial table (order 4, rho) ----------------
 // Per char-step: drift (FMA), floor, index, one 16-byte and one 8-byte LDS read,
-// two FMAs for Q0 + t (Q1 + t Q2), one FMA for the kick.  The tables are built
-// in LDS from the difference history, double-buffered (poly_pipeline) when
-// p.pipe is set.
+// two FMAs for Q0 + t (Q1 + t Q2), one FMA for the kick.  When p.pipe is set
+// the tables are double-buffered: copied from the ready tables p.qhist by
+// direct loads (poly_ready_pipeline) or, without them, built in LDS from the
+// difference history (poly_pipeline); else built per chunk (poly_direct).
 template <int BLOCK, int NPT, bool POW2>
 __device__ __forceinline__ void trace1d_poly_body( const trace1_params &p, uint32_t wg, double *lds )
 {
@@ -357,8 +358,9 @@ __device__ __forceinline__ void trace1d_poly_body( const trace1_params &p, uint3
             }
         }
     };
-    if ( p.pipe ) poly_pipeline<BLOCK,poly_maxc( BLOCK ),3>( lds, p.dhist, p.stride_d, sq, q2, cells, kick, p.n, p.chunk, steps );
-    else          poly_direct<BLOCK>( lds, p.dhist, p.stride_d, sq, q2, cells, kick, p.n, p.chunk, steps );
+    if ( p.pipe && p.qhist ) poly_ready_pipeline<BLOCK>( lds, p.qhist, sq, p.n, p.chunk, steps );
+    else if ( p.pipe ) poly_pipeline<BLOCK,poly_maxc( BLOCK ),3>( lds, p.dhist, p.stride_d, sq, q2, cells, kick, p.n, p.chunk, steps );
+    else               poly_direct<BLOCK>( lds, p.dhist, p.stride_d, sq, q2, cells, kick, p.n, p.chunk, steps );
 
     double f[NPT];
     #pragma unroll
@@ -646,12 +648,19 @@ void trace1d_xm_setup( trace1_params &p )
     p.ppx  = xm_ppx( p.Nu );
 }
 
+// the FP64 rho kernel copies ready tables when the caller passes them
+static bool poly_ready( const trace1_params &p, int variant )
+{
+    return p.qhist != nullptr && variant == TRACE_FAST && p.mode == MODE_RHO;
+}
+
 uint32_t trace1d_chunk( const trace1_params &p, int variant, const launch_shape &s, size_t lds_max, bool *pipe )
 {
     uint64_t c = uint64_t( s.chunk );
     bool pp = false;
     if ( x_major( p, variant ) )
-        c = poly_chunk_plan( p.Nx, s.block, variant == TRACE_FP32C ? sizeof(float) : sizeof(double), c, lds_max, &pp );
+        c = poly_chunk_plan( p.Nx, s.block, variant == TRACE_FP32C ? sizeof(float) : sizeof(double), c, lds_max, &pp,
+                             poly_ready( p, variant ) );
     const uint64_t v = p.mode == MODE_METRICS ? ( p.n > 0 ? p.n + 1 : 0 ) : p.n;
     if ( v < c ) c = v > 0 ? v : 1;
     if ( pipe ) *pipe = pp;

@@ -68,24 +68,6 @@ __device__ __forceinline__ void stage3( T *__restrict__ dst, const T *__restrict
     }
 }
 
-// Direct global->LDS copy of n16 16-byte words (gfx950 global_load_lds_dwordx4):
-// lane l of a wave writes word q0 + l of dst, the wave's words starting at
-// q0; the loads hold no registers and are waited for (vmcnt) before the
-// barrier that publishes the buffer.  dst and src are 16-byte aligned.
-template <int BLOCK>
-__device__ __forceinline__ void dma_copy16( void *dst, const void *src, uint32_t n16 )
-{
-    typedef __attribute__((address_space(3))) void lds_void;
-    const uint32_t lane = threadIdx.x % WAVE;
-    const uint32_t w = __builtin_amdgcn_readfirstlane( threadIdx.x / WAVE );     // the wave index, scalar
-    for ( uint32_t q0 = w * WAVE; q0 < n16; q0 += BLOCK )                        // uniform in the wave
-    {
-        if ( q0 + lane < n16 )
-            __builtin_amdgcn_global_load_lds( static_cast<const char*>( src ) + size_t( q0 + lane ) * 16,
-                                              (lds_void*)( static_cast<char*>( dst ) + size_t( q0 ) * 16 ), 16, 0, 0 );
-    }
-}
-
 // The FP64 slices start 8-byte aligned (odd strides): copy the 16-byte words
 // that cover `count` doubles from src and return where src[0] lands in dst
 // (dst + 0 or + 1).  The history allocation has slack past its last slice.
@@ -95,13 +77,6 @@ __device__ __forceinline__ const double *dma_doubles( double *dst, const double 
     const uint32_t mis = uint32_t( reinterpret_cast<uintptr_t>( src ) & 15u ) / 8u;
     dma_copy16<BLOCK>( dst, src - mis, ( count + mis + 1 ) / 2 );
     return dst + mis;
-}
-
-// wait for this wave's direct LDS loads (vmcnt(0)), then the workgroup barrier
-__device__ __forceinline__ void dma_publish()
-{
-    __builtin_amdgcn_s_waitcnt( 0x0F70 );     // vmcnt(0); expcnt and lgkmcnt not waited for
-    __syncthreads();
 }
 
 struct node3

@@ -192,10 +192,11 @@ def poly_maxc(block):
     return 6
 
 
-def poly_chunk_plan(Nx, block, elem, req, lds_max=160 * 1024):
+def poly_chunk_plan(Nx, block, elem, req, lds_max=160 * 1024, ready=False):
     """internal.hpp poly_chunk_plan: (chunk, pipelined)."""
     row = ((3 * (Nx + 2) + 3) & ~3) * elem
-    c = min(req, block * poly_maxc(block) // (Nx + 2), lds_max // (2 * row))
+    pf = req if ready else block * poly_maxc(block) // (Nx + 2)
+    c = min(req, pf, lds_max // (2 * row))
     if c >= 1:
         return c, True
     return max(1, min(req, lds_max // row)), False
@@ -246,6 +247,55 @@ def test_table_build_from_differences_stays_in_bounds(Nx):
                                     written.add(a)
                                 assert loff + j + 1 < jj * sq + q2 and loff + q2 < (jj + 1) * sq
                     assert len(written) == 3 * cnt * cells
+
+
+@pytest.mark.parametrize("Nx", [1, 2, 16, 24, 256, 1000, 3000, 5000])
+@pytest.mark.parametrize("n", [1, 2, 13, 14, 40])
+def test_ready_table_copy_stays_in_bounds(Nx, n):
+    """step1d.hpp poly_ready_pipeline with dma_copy16: the FP64 tables of slice
+    m sit at qhist + m*sq (sq = poly_stride, a multiple of 4 doubles, so every
+    slice starts 16-byte aligned).  Chunk k (slices lo .. lo+cnt-1, highest
+    chunk first) is n16 = cnt*sq/2 words; word q = q0 + lane (q0 = wave*64 +
+    i*block, q < n16) covers doubles 2q, 2q+1 of the buffer and of the chunk's
+    rows.  Every word is copied once, inside the buffer (chunk*sq doubles), from
+    the slices below n (the history holds Nt+1 >= n slices).  The ready plan
+    has no prefetch limit; two buffers fit 160 KiB, else the direct build."""
+    cells = Nx + 2
+    sq = (3 * cells + 3) & ~3
+    assert sq % 2 == 0 and (sq * 8) % 16 == 0
+    for block in (256, 1024):
+        for want in (1, 5, 13, 16, 1 << 20):
+            chunk, pipe = poly_chunk_plan(Nx, block, 8, want, ready=True)
+            assert chunk <= want
+            if not pipe:
+                assert 2 * sq * 8 > 160 * 1024
+                continue
+            assert 2 * chunk * sq * 8 <= 160 * 1024
+            assert chunk == min(want, 160 * 1024 // (2 * sq * 8))
+            top = n - 1
+            K = (top + chunk) // chunk
+            covered = set()
+            for k in range(K):
+                h = top - k * chunk
+                lo = max(h - chunk + 1, 0)
+                cnt = h - lo + 1
+                n16 = cnt * sq // 2
+                seen = set()
+                for w in range(block // 64):
+                    q0 = w * 64
+                    while q0 < n16:
+                        for lane in range(64):
+                            q = q0 + lane
+                            if q < n16:
+                                assert q not in seen
+                                seen.add(q)
+                                assert 2 * q + 1 < chunk * sq                  # inside the buffer
+                                assert lo * sq + 2 * q + 1 < (lo + cnt) * sq   # inside the chunk's rows
+                                assert (lo + cnt) * sq <= n * sq               # slices below n
+                        q0 += block
+                assert len(seen) == n16
+                covered.update(range(lo, lo + cnt))
+            assert covered == set(range(n))
 
 
 @pytest.mark.parametrize("block", [256, 512])
