@@ -47,6 +47,7 @@ struct nufi_ctx
 
     double *hist = nullptr, *dhist = nullptr;
     double *qhist = nullptr;   // 1D FP64: kick-polynomial tables of every slice (stride sq), or null
+    float *qhist32 = nullptr;  // 1D FP32C: the FP32 tables scaled by the kick (stride sq), or null
     float *hist32 = nullptr;
     uint32_t sq = 0;           // 1D: LDS kick-polynomial table stride (poly_stride)
     double *partials = nullptr, *rho = nullptr, *energies = nullptr, *values = nullptr;
@@ -110,7 +111,7 @@ int dev_alloc( nufi_ctx *c, T **p, size_t count, const char *what )
 
 void free_all( nufi_ctx *c )
 {
-    void *ptrs[] = { c->hist, c->dhist, c->qhist, c->hist32, c->partials, c->rho, c->energies,
+    void *ptrs[] = { c->hist, c->dhist, c->qhist, c->qhist32, c->hist32, c->partials, c->rho, c->energies,
                      c->values, c->scratch_slice, c->metrics, c->mpartials, c->energy_tmp,
                      c->rho_hist, c->lambda, c->gvec, c->grad_part, c->grad, c->tw_x, c->tw_y, c->tw_z,
                      c->mult_step, c->mult_pois, c->mult_fit, c->ewt, c->samp };
@@ -165,10 +166,11 @@ uint32_t rho_ppx( const nufi_ctx *c )
     return xm ? c->ppx : c->wpx;
 }
 
-// the 1D FP64 rho launches copy the ready tables of qhist
+// the 1D rho launches copy the ready tables of qhist / qhist32
 bool tables_ready( const nufi_ctx *c )
 {
-    return c->dim == 1 && c->variant == TRACE_FAST && c->qhist != nullptr;
+    return c->dim == 1 && ( ( c->variant == TRACE_FAST && c->qhist != nullptr ) ||
+                            ( c->variant == TRACE_FP32C && c->qhist32 != nullptr ) );
 }
 
 // The chunk a rho launch of shape s uses and its LDS: 1D x-major launches
@@ -272,6 +274,7 @@ int launch_trace( nufi_ctx *c, uint64_t n, uint64_t q_begin, uint64_t q_end, uin
         trace1_params p {};
         p.dhist = c->dhist; p.partials = c->partials;
         p.qhist = c->variant == TRACE_FAST ? c->qhist : nullptr;
+        p.qhist32 = c->variant == TRACE_FP32C ? c->qhist32 : nullptr;
         p.stride_q = c->sq; p.q2_off = poly_q2( uint32_t( c->conf.Nx ) ); p.kick = -0.5 * c->conf.dt * c->conf.dx_inv;
         p.n = n; p.q_begin = q_begin; p.q_end = q_end;
         p.Nx = uint32_t( c->conf.Nx ); p.Nu = uint32_t( c->conf.Nu ); p.stride_d = c->sd; p.wpx = c->wpx;
@@ -453,6 +456,8 @@ int nufi_create( const nufi_config *conf, const nufi_config *conf_metrics, int o
     // the FP64 rho kernel copies these tables instead of building them (3 reals per cell and slice)
     if ( c->dim == 1 && precision == NUFI_FP64 && order == 4 &&
          ( rc = dev_alloc( c, &c->qhist, slices * c->sq + 8, "table history" ) ) ) return bail( rc );
+    if ( c->dim == 1 && precision == NUFI_FP32C &&
+         ( rc = dev_alloc( c, &c->qhist32, slices * c->sq + 8, "FP32 table history" ) ) ) return bail( rc );
     c->st32 = ( c->st + 3u ) & ~3u;
     if ( precision == NUFI_FP32C && c->dim == 3 && ( rc = dev_alloc( c, &c->hist32, slices * c->st32 + 8, "FP32 history" ) ) )
         return bail( rc );
@@ -552,6 +557,8 @@ int nufi_upload_phi( nufi_ctx *c, uint64_t n, const double *history_base )
     hipError_t e = launch_slice_tables( c->dim, dst, c->st, uint32_t( c->conf.Nx ),
                                         c->dhist ? c->dhist + n * c->sd : nullptr,
                                         c->qhist ? c->qhist + n * c->sq : nullptr,
+                                        c->qhist32 ? c->qhist32 + n * c->sq : nullptr,
+                                        -0.5 * c->conf.dt * c->conf.dx_inv,
                                         c->hist32 ? c->hist32 + n * c->st32 : nullptr, c->stream );
     if ( e != hipSuccess ) return hip_fail( c, e, "slice table launch" );
     // the copy reads host memory the caller may reuse: complete it before returning
@@ -664,7 +671,9 @@ static int step_async( nufi_ctx *c, uint64_t n, double *rho_out )
     f.slice = c->hist + n * c->st;
     f.dslice = c->dhist ? c->dhist + n * c->sd : nullptr;
     f.qslice = c->qhist ? c->qhist + n * c->sq : nullptr;
+    f.qslice32 = c->qhist32 ? c->qhist32 + n * c->sq : nullptr;
     f.q2_off = poly_q2( uint32_t( c->conf.Nx ) );
+    f.kick = -0.5 * c->conf.dt * c->conf.dx_inv;
     f.slice32 = c->hist32 ? c->hist32 + n * c->st32 : nullptr;
     f.energy = c->energies + n;
     f.rho_out = rho_out ? rho_out : c->rho;

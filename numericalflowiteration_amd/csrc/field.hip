@@ -145,14 +145,15 @@ __device__ __forceinline__ void field_body( const field_params &p, double2 *lds2
         if ( p.dslice )
             for ( uint32_t j = threadIdx.x; j < p.stride_d; j += FIELD_BLOCK )
                 p.dslice[j] = ( j < len + 1 ) ? R[ j % p.Nx ].x - R[ ( j + p.Nx - 1 ) % p.Nx ].x : 0.0;
-        // the kick-polynomial table from the same differences: entry j uses D[j-1], D[j], D[j+1]
-        if ( p.qslice )
+        // the kick-polynomial tables from the same differences: entry j uses D[j-1], D[j], D[j+1]
+        if ( p.qslice || p.qslice32 )
             for ( uint32_t j = threadIdx.x; j < p.Nx + 2; j += FIELD_BLOCK )
             {
                 double D[3];
                 #pragma unroll
                 for ( uint32_t l = 0; l < 3; ++l ) D[l] = R[ ( j + l ) % p.Nx ].x - R[ ( j + l + p.Nx - 1 ) % p.Nx ].x;
-                put_poly( p.qslice + 2*j, p.qslice + p.q2_off + j, D[0], D[1], D[2], 0.0 );
+                if ( p.qslice )   put_poly( p.qslice + 2*j, p.qslice + p.q2_off + j, D[0], D[1], D[2], p.kick );
+                if ( p.qslice32 ) put_poly( p.qslice32 + 2*j, p.qslice32 + p.q2_off + j, D[0], D[1], D[2], p.kick );
             }
     }
     else
@@ -206,18 +207,19 @@ __global__ __launch_bounds__( 64 ) void reduce_partials_kernel( const double *pa
 // extended differences dslice[j] = c[j mod Nx] - c[(j-1) mod Nx], j <= len,
 // and the kick-polynomial table entries j < Nx+2 from dslice[j .. j+2].
 __global__ void slice_tables_kernel( int dim, const double *slice, uint32_t len, uint32_t Nx, double *dslice,
-                                     double *qslice, float *slice32 )
+                                     double *qslice, float *qslice32, double kick, float *slice32 )
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if ( i > len ) return;
     if ( slice32 && i < len ) slice32[i] = float( slice[i] );
     if ( dim == 1 && dslice ) dslice[i] = slice[ i % Nx ] - slice[ ( i + Nx - 1 ) % Nx ];
-    if ( dim == 1 && qslice && i < Nx + 2 )
+    if ( dim == 1 && ( qslice || qslice32 ) && i < Nx + 2 )
     {
         double D[3];
         #pragma unroll
         for ( uint32_t l = 0; l < 3; ++l ) D[l] = slice[ ( i + l ) % Nx ] - slice[ ( i + l + Nx - 1 ) % Nx ];
-        put_poly( qslice + 2*i, qslice + poly_q2( Nx ) + i, D[0], D[1], D[2], 0.0 );
+        if ( qslice )   put_poly( qslice + 2*i, qslice + poly_q2( Nx ) + i, D[0], D[1], D[2], kick );
+        if ( qslice32 ) put_poly( qslice32 + 2*i, qslice32 + poly_q2( Nx ) + i, D[0], D[1], D[2], kick );
     }
 }
 
@@ -323,11 +325,11 @@ hipError_t launch_reduce_partials( const double *partials, uint32_t wpx, double 
 }
 
 hipError_t launch_slice_tables( int dim, const double *slice, uint32_t len, uint32_t Nx, double *dslice,
-                                double *qslice, float *slice32, hipStream_t st )
+                                double *qslice, float *qslice32, double kick, float *slice32, hipStream_t st )
 {
     const uint32_t block = 256, grid = ( len + 1 + block - 1 ) / block;
     hipLaunchKernelGGL( slice_tables_kernel, dim3( grid ), dim3( block ), 0, st,
-                        dim, slice, len, Nx, dslice, qslice, slice32 );
+                        dim, slice, len, Nx, dslice, qslice, qslice32, kick, slice32 );
     return hipGetLastError();
 }
 

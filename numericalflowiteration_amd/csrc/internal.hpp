@@ -62,6 +62,7 @@ struct trace1_params
     const double *dhist;     // extended differences of slice m at dhist + m*stride_d (FP64)
     const double *qhist;     // FP64 rho: ready kick-polynomial tables, slice m at qhist + m*stride_q
                              // in the LDS layout (null: built in LDS from dhist)
+    const float  *qhist32;   // FP32C rho: the same, FP32 tables scaled by the kick (or null)
     uint32_t stride_q, q2_off;        // LDS kick-polynomial table: slice stride, Q2 row (poly_stride, poly_q2)
     double   kick;                    // -dt/(2 dx), the scale of the kick polynomials
     uint32_t pipe;                    // x-major: tables double-buffered and prefetched (poly_pipeline)
@@ -242,7 +243,9 @@ struct field_params
     double  *slice;              // history slice n (coefficients)
     double  *dslice;             // 1D differences of slice n (stride_d)
     double  *qslice;             // 1D FP64 kick-polynomial table of slice n (poly_stride layout) or null
-    uint32_t q2_off;             // its Q2 row (poly_q2)
+    float   *qslice32;           // 1D FP32 table of slice n, scaled by kick, or null
+    uint32_t q2_off;             // their Q2 row (poly_q2)
+    double   kick;               // -dt/(2 dx)
     float   *slice32;            // FP32 coefficients (3D FP32C) or null
     double  *energy;             // energies[n]
     uint32_t sy, sz, stride_d;
@@ -272,10 +275,10 @@ hipError_t launch_reduce_metrics( const double *partials, uint32_t n_tasks, doub
 hipError_t launch_reduce_partials( const double *partials, uint32_t wpx, double weight,
                                    uint32_t first, uint32_t count, uint32_t n_space,
                                    double *rho, hipStream_t st );
-// Differences and the FP64 kick-polynomial table (1D) and the FP32 copy (3D
-// FP32C) of an uploaded slice.
+// Differences and the FP64 / FP32 kick-polynomial tables (1D) and the FP32
+// copy (3D FP32C) of an uploaded slice.
 hipError_t launch_slice_tables( int dim, const double *slice, uint32_t len, uint32_t Nx, double *dslice,
-                                double *qslice, float *slice32, hipStream_t st );
+                                double *qslice, float *qslice32, double kick, float *slice32, hipStream_t st );
 
 // The 1D kick-polynomial table of one slice, built in LDS by the fast, FP32C
 // and adjoint kernels from the difference history: for cell i (i = -1 .. Nx,

@@ -459,7 +459,8 @@ __device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint
             if ( seg == 0 && m == 0 ) { step( Q, 0.5f ); --m; Q -= sq; }
         }
     };
-    if ( p.pipe ) poly_pipeline<BLOCK,poly_maxc( BLOCK ),2>( ldsf, p.dhist, p.stride_d, sq, q2, cells, p.kick, p.n, p.chunk, steps );
+    if ( p.pipe && p.qhist32 ) poly_ready_pipeline<BLOCK>( ldsf, p.qhist32, sq, p.n, p.chunk, steps );
+    else if ( p.pipe ) poly_pipeline<BLOCK,poly_maxc( BLOCK ),2>( ldsf, p.dhist, p.stride_d, sq, q2, cells, p.kick, p.n, p.chunk, steps );
     else          poly_direct<BLOCK>( ldsf, p.dhist, p.stride_d, sq, q2, cells, p.kick, p.n, p.chunk, steps );
 
     double f[NPT];
@@ -648,10 +649,11 @@ void trace1d_xm_setup( trace1_params &p )
     p.ppx  = xm_ppx( p.Nu );
 }
 
-// the FP64 rho kernel copies ready tables when the caller passes them
+// the FP64 and FP32C rho kernels copy ready tables when the caller passes them
 static bool poly_ready( const trace1_params &p, int variant )
 {
-    return p.qhist != nullptr && variant == TRACE_FAST && p.mode == MODE_RHO;
+    return p.mode == MODE_RHO && ( ( variant == TRACE_FAST && p.qhist != nullptr ) ||
+                                   ( variant == TRACE_FP32C && p.qhist32 != nullptr ) );
 }
 
 uint32_t trace1d_chunk( const trace1_params &p, int variant, const launch_shape &s, size_t lds_max, bool *pipe )

@@ -258,20 +258,28 @@ def test_ready_table_copy_stays_in_bounds(Nx, n):
     chunk first) is n16 = cnt*sq/2 words; word q = q0 + lane (q0 = wave*64 +
     i*block, q < n16) covers doubles 2q, 2q+1 of the buffer and of the chunk's
     rows.  Every word is copied once, inside the buffer (chunk*sq doubles), from
-    the slices below n (the history holds Nt+1 >= n slices).  The ready plan
-    has no prefetch limit; two buffers fit 160 KiB, else the direct build."""
+    the slices below n (the history holds Nt+1 >= n slices).  The FP32 tables
+    (FP32C) have the same layout in floats: word q covers floats 4q .. 4q+3.
+    The ready plan has no prefetch limit; two buffers fit 160 KiB, else the
+    direct build."""
     cells = Nx + 2
     sq = (3 * cells + 3) & ~3
-    assert sq % 2 == 0 and (sq * 8) % 16 == 0
+    for elem in (8, 4):
+        assert (sq * elem) % 16 == 0
+        check_ready_copy(Nx, n, sq, elem)
+
+
+def check_ready_copy(Nx, n, sq, elem):
+    per = 16 // elem                       # table elements per 16-byte word
     for block in (256, 1024):
         for want in (1, 5, 13, 16, 1 << 20):
-            chunk, pipe = poly_chunk_plan(Nx, block, 8, want, ready=True)
+            chunk, pipe = poly_chunk_plan(Nx, block, elem, want, ready=True)
             assert chunk <= want
             if not pipe:
-                assert 2 * sq * 8 > 160 * 1024
+                assert 2 * sq * elem > 160 * 1024
                 continue
-            assert 2 * chunk * sq * 8 <= 160 * 1024
-            assert chunk == min(want, 160 * 1024 // (2 * sq * 8))
+            assert 2 * chunk * sq * elem <= 160 * 1024
+            assert chunk == min(want, 160 * 1024 // (2 * sq * elem))
             top = n - 1
             K = (top + chunk) // chunk
             covered = set()
@@ -279,7 +287,7 @@ def test_ready_table_copy_stays_in_bounds(Nx, n):
                 h = top - k * chunk
                 lo = max(h - chunk + 1, 0)
                 cnt = h - lo + 1
-                n16 = cnt * sq // 2
+                n16 = cnt * sq // per
                 seen = set()
                 for w in range(block // 64):
                     q0 = w * 64
@@ -289,8 +297,9 @@ def test_ready_table_copy_stays_in_bounds(Nx, n):
                             if q < n16:
                                 assert q not in seen
                                 seen.add(q)
-                                assert 2 * q + 1 < chunk * sq                  # inside the buffer
-                                assert lo * sq + 2 * q + 1 < (lo + cnt) * sq   # inside the chunk's rows
+                                last = per * q + per - 1
+                                assert last < chunk * sq                       # inside the buffer
+                                assert lo * sq + last < (lo + cnt) * sq        # inside the chunk's rows
                                 assert (lo + cnt) * sq <= n * sq               # slices below n
                         q0 += block
                 assert len(seen) == n16
