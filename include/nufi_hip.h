@@ -114,7 +114,11 @@ int nufi_device_count(int *count);
 /* ---- context ----------------------------------------------------------- */
 /*
  * Allocates, on `device`, the history of (Nt+1) coefficient slices of
- * (Nx+order-1)^dim reals each, rho and metrics buffers (cuda_kernel.cu:81-109).
+ * (Nx+order-1)^dim reals each, rho and metrics buffers (cuda_kernel.cu:81-109),
+ * plus per-slice tables derived from each slice as it is stored: in 1D the
+ * first differences (Nx+order+3 rounded to 4 reals) and, at order 4, the
+ * kick-polynomial table (3(Nx+2) rounded to 4 reals, FP32 for NUFI_FP32C);
+ * in 3D FP32C an FP32 copy of the history.
  * conf_metrics may be NULL (then conf is used for metrics, as the dim3
  * scheduler does).  Errors: NUFI_ERR_UNSUPPORTED for an order or dim that is
  * not built, NUFI_ERR_ALLOC / NUFI_ERR_DEVICE.  On failure *out is NULL and

@@ -24,10 +24,12 @@ struct nufi_batch
     hipStream_t stream = nullptr;
     std::vector<nufi_config> confs;
     uint64_t n_runs = 0, Nt_max = 0;
-    std::vector<size_t> off_hist, off_dhist, off_part, off_en, off_tab;
+    std::vector<size_t> off_hist, off_dhist, off_q, off_part, off_en, off_tab;
     std::vector<uint32_t> st, sd, sq, wpx;
 
     double *hist = nullptr, *dhist = nullptr, *partials = nullptr, *energies = nullptr;
+    double *qhist = nullptr;       // FP64: every run's ready kick-polynomial tables (stride sq)
+    float *qhist32 = nullptr;      // FP32C: the same in FP32, scaled by the run's kick
     double2 *tw = nullptr, *mult = nullptr;
     double *ewt = nullptr;
     batch_run *d_runs = nullptr;
@@ -80,7 +82,7 @@ int balloc( nufi_batch *b, T **p, size_t count, const char *what )
 void bfree_all( nufi_batch *b )
 {
     if ( b->graph ) (void)hipGraphExecDestroy( b->graph );
-    void *ptrs[] = { b->hist, b->dhist, b->partials, b->energies, b->tw, b->mult, b->ewt,
+    void *ptrs[] = { b->hist, b->dhist, b->qhist, b->qhist32, b->partials, b->energies, b->tw, b->mult, b->ewt,
                      b->d_runs, b->d_fields, b->d_wg_run };
     for ( void *p : ptrs ) if ( p ) (void)hipFree( p );
     if ( b->stream ) (void)hipStreamDestroy( b->stream );
@@ -175,7 +177,7 @@ int nufi_batch_create( const nufi_config *confs, uint64_t n_runs, int order, int
     { bhip( b, he, "hipStreamCreate" ); return bail( NUFI_ERR_DEVICE ); }
 
     // packed per-run layout, 256-byte aligned sub-buffers
-    size_t n_hist = 0, n_dhist = 0, n_part = 0, n_en = 0, n_tab = 0;
+    size_t n_hist = 0, n_dhist = 0, n_q = 0, n_part = 0, n_en = 0, n_tab = 0;
     for ( uint64_t r = 0; r < n_runs; ++r )
     {
         const nufi_config &c = confs[r];
@@ -186,6 +188,7 @@ int nufi_batch_create( const nufi_config *confs, uint64_t n_runs, int order, int
         b->st.push_back( st ); b->sd.push_back( sd ); b->sq.push_back( sq ); b->wpx.push_back( wpx );
         b->off_hist.push_back( n_hist );  n_hist  = align_up( n_hist + ( c.Nt + 1 ) * st + 8, 32 );
         b->off_dhist.push_back( n_dhist ); n_dhist = align_up( n_dhist + ( c.Nt + 1 ) * sd + 8, 32 );
+        b->off_q.push_back( n_q );        n_q     = align_up( n_q + ( c.Nt + 1 ) * sq + 8, 32 );
         b->off_part.push_back( n_part );  n_part  = align_up( n_part + c.Nx * wpx, 32 );
         b->off_en.push_back( n_en );      n_en    = align_up( n_en + c.Nt + 1, 32 );
         b->off_tab.push_back( n_tab );    n_tab   = align_up( n_tab + c.Nx, 16 );
@@ -196,6 +199,8 @@ int nufi_batch_create( const nufi_config *confs, uint64_t n_runs, int order, int
     int rc;
     if ( ( rc = balloc( b, &b->hist, n_hist, "batch history" ) ) ) return bail( rc );
     if ( ( rc = balloc( b, &b->dhist, n_dhist, "batch differences" ) ) ) return bail( rc );
+    if ( precision == NUFI_FP32C ) { if ( ( rc = balloc( b, &b->qhist32, n_q, "batch FP32 tables" ) ) ) return bail( rc ); }
+    else if ( ( rc = balloc( b, &b->qhist, n_q, "batch tables" ) ) ) return bail( rc );
     if ( ( rc = balloc( b, &b->partials, n_part, "batch partials" ) ) ) return bail( rc );
     if ( ( rc = balloc( b, &b->energies, n_en, "batch energies" ) ) ) return bail( rc );
     if ( ( rc = balloc( b, &b->tw, n_tab, "batch twiddles" ) ) ) return bail( rc );
@@ -221,6 +226,8 @@ int nufi_batch_create( const nufi_config *confs, uint64_t n_runs, int order, int
         trace1_params &p = b->h_runs[r].p;
         p = trace1_params {};
         p.dhist = b->dhist + b->off_dhist[r];
+        p.qhist = b->qhist ? b->qhist + b->off_q[r] : nullptr;
+        p.qhist32 = b->qhist32 ? b->qhist32 + b->off_q[r] : nullptr;
         p.stride_q = b->sq[r]; p.q2_off = poly_q2( uint32_t( c.Nx ) ); p.kick = -0.5 * c.dt * c.dx_inv;
         p.partials = b->partials + b->off_part[r];
         p.q_begin = 0; p.q_end = c.Nx * c.Nu;
@@ -245,9 +252,12 @@ int nufi_batch_create( const nufi_config *confs, uint64_t n_runs, int order, int
         f.mult = b->mult + b->off_tab[r]; f.ewt = b->ewt + b->off_tab[r];
         f.slice = b->hist + b->off_hist[r];
         f.dslice = b->dhist + b->off_dhist[r];
+        f.qslice = b->qhist ? b->qhist + b->off_q[r] : nullptr;
+        f.qslice32 = b->qhist32 ? b->qhist32 + b->off_q[r] : nullptr;
+        f.q2_off = poly_q2( uint32_t( c.Nx ) ); f.kick = -0.5 * c.dt * c.dx_inv;
         f.energy = b->energies + b->off_en[r];
         f.sy = b->st[r]; f.sz = b->st[r]; f.stride_d = b->sd[r];
-        fields[r].Nt = c.Nt; fields[r].st = b->st[r]; fields[r].sd = b->sd[r];
+        fields[r].Nt = c.Nt; fields[r].st = b->st[r]; fields[r].sd = b->sd[r]; fields[r].sq = b->sq[r];
     }
     if ( ( he = hipMemcpy( b->tw, tw.data(), n_tab * sizeof(double2), hipMemcpyHostToDevice ) ) != hipSuccess ||
          ( he = hipMemcpy( b->mult, mult.data(), n_tab * sizeof(double2), hipMemcpyHostToDevice ) ) != hipSuccess ||

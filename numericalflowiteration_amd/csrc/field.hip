@@ -178,8 +178,8 @@ void field_kernel( field_params p )
 }
 
 // Batched sweep: workgroup r solves run r's field at step n (runs past their
-// Nt skip it); the per-step outputs are the run's slice n, its differences and
-// energies[n].
+// Nt skip it); the per-step outputs are the run's slice n, its differences,
+// its kick-polynomial table and energies[n].
 __global__ __launch_bounds__( FIELD_BLOCK )
 void field_batch_kernel( const batch_field *__restrict__ runs, uint64_t n )
 {
@@ -189,6 +189,8 @@ void field_batch_kernel( const batch_field *__restrict__ runs, uint64_t n )
     field_params f = b.f;
     f.slice = b.f.slice + n * b.st;
     if ( f.dslice )   f.dslice   = b.f.dslice + n * b.sd;
+    if ( f.qslice )   f.qslice   = b.f.qslice + n * b.sq;
+    if ( f.qslice32 ) f.qslice32 = b.f.qslice32 + n * b.sq;
     f.energy = b.f.energy + n;
     field_body( f, lds2b );
 }

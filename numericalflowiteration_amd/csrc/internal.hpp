@@ -259,7 +259,7 @@ struct batch_field
 {
     field_params f;
     uint64_t Nt;
-    uint32_t st, sd;
+    uint32_t st, sd, sq;
 };
 hipError_t launch_field_batch( const batch_field *runs, uint32_t n_runs, uint64_t n, size_t lds, hipStream_t st );
 // LDS of the batch field launch; *tw_lds tells whether the runs' twiddles fit it
