@@ -109,8 +109,8 @@ void adjoint1d_kernel( adjoint1_params p )
         const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
         const int cnt = int( mhi - mlo + 1 );
         __syncthreads();
-        build_poly<BLOCK>( Ql, p.dhist, p.stride_d, sq, q2, mlo, cnt, p.Nx + 2, kick );
-        __syncthreads();
+        dma_copy16<BLOCK>( Ql, p.qhist + size_t( mlo ) * sq, uint32_t( cnt ) * sq / 2u );
+        dma_publish();
         for ( int jj = cnt - 1; jj >= 0; )          // segments between re-wrap points
         {
             if ( POW2 && poly_wrap_due( mlo + jj ) ) rewrap( s, Nxd, invNx );
@@ -141,9 +141,9 @@ void adjoint1d_kernel( adjoint1_params p )
         const int64_t mhi = ( mlo + p.chunk - 1 < int64_t( p.n ) - 1 ) ? mlo + p.chunk - 1 : int64_t( p.n ) - 1;
         const int cnt = int( mhi - mlo + 1 );
         __syncthreads();
-        build_poly<BLOCK>( Ql, p.dhist, p.stride_d, sq, q2, mlo, cnt, p.Nx + 2, kick );
+        dma_copy16<BLOCK>( Ql, p.qhist + size_t( mlo ) * sq, uint32_t( cnt ) * sq / 2u );
         for ( uint32_t q = threadIdx.x; q < uint32_t( cnt ) * p.Nx; q += BLOCK ) lam[q] = 0.0;
-        __syncthreads();
+        dma_publish();
         for ( int jj = 0; jj < cnt; ++jj )
         {
             const double hk = ( mlo + jj == 0 ) ? 0.5 * kick : kick;

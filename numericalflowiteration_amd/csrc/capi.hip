@@ -166,13 +166,6 @@ uint32_t rho_ppx( const nufi_ctx *c )
     return xm ? c->ppx : c->wpx;
 }
 
-// the 1D rho launches copy the ready tables of qhist / qhist32
-bool tables_ready( const nufi_ctx *c )
-{
-    return c->dim == 1 && ( ( c->variant == TRACE_FAST && c->qhist != nullptr ) ||
-                            ( c->variant == TRACE_FP32C && c->qhist32 != nullptr ) );
-}
-
 // The chunk a rho launch of shape s uses and its LDS: 1D x-major launches
 // plan the requested chunk within 160 KiB (poly_chunk_plan).
 int effective_chunk( const nufi_ctx *c, const launch_shape &s, size_t *lds = nullptr )
@@ -196,15 +189,14 @@ int effective_chunk( const nufi_ctx *c, const launch_shape &s, size_t *lds = nul
         return s.chunk;
     }
     bool pipe = false;
-    const uint32_t ch = poly_chunk_plan( uint32_t( c->conf.Nx ), s.block, elem, uint64_t( s.chunk ), 160 * 1024, &pipe,
-                                         tables_ready( c ) );
+    const uint32_t ch = poly_chunk_plan( uint32_t( c->conf.Nx ), elem, uint64_t( s.chunk ), 160 * 1024, &pipe );
     if ( lds ) *lds = ( pipe ? 2 : 1 ) * size_t( ch ) * row * elem;
     return int( ch );
 }
 
 // Default launch shapes (the autotuner refines them): 1D x-major 1024 threads
-// x 1 node (16 velocity rows per workgroup), the largest chunk the register
-// prefetch and 160 KiB of LDS hold; 3D 512 threads x 4 nodes.
+// x 1 node (16 velocity rows per workgroup), the largest chunk two table
+// buffers in 160 KiB of LDS hold; 3D 512 threads x 4 nodes.
 void default_shape( nufi_ctx *c )
 {
     const uint64_t waves = c->n_space * c->wpx;
@@ -225,9 +217,8 @@ void default_shape( nufi_ctx *c )
         else    // x-major: 16 velocity rows per workgroup, the largest chunk planned
         {
             bool pipe = false;
-            c->shape = launch_shape { 1024, 1, int( poly_chunk_plan( uint32_t( c->conf.Nx ), 1024, elem, 1u << 20,
-                                                                    std::min<size_t>( c->lds_cap, 160 * 1024 ), &pipe,
-                                                                    tables_ready( c ) ) ) };
+            c->shape = launch_shape { 1024, 1, int( poly_chunk_plan( uint32_t( c->conf.Nx ), elem, 1u << 20,
+                                                                    std::min<size_t>( c->lds_cap, 160 * 1024 ), &pipe ) ) };
         }
     }
     else
@@ -752,7 +743,7 @@ int nufi_adjoint_grad( nufi_ctx *c, double *dJ_da, int n_par, double *J_out )
     af.N = uint32_t( Nx ); af.log2x = log2_or_none( Nx );
     af.g = c->gvec; af.tw = c->tw_x; af.mult = c->mult_step; af.ewt = c->ewt; af.dt = c->conf.dt;
     adjoint1_params ap {};
-    ap.dhist = c->dhist; ap.g = c->gvec; ap.lambda = c->lambda; ap.grad_part = c->grad_part;
+    ap.qhist = c->qhist; ap.g = c->gvec; ap.lambda = c->lambda; ap.grad_part = c->grad_part;
     ap.Nx = uint32_t( Nx ); ap.Nu = uint32_t( c->conf.Nu ); ap.stride_d = c->sd;
     ap.stride_q = c->sq; ap.q2_off = poly_q2( uint32_t( Nx ) );
     ap.wpx = c->wpx;

@@ -60,12 +60,11 @@ enum trace_mode : int
 struct trace1_params
 {
     const double *dhist;     // extended differences of slice m at dhist + m*stride_d (FP64)
-    const double *qhist;     // FP64 rho: ready kick-polynomial tables, slice m at qhist + m*stride_q
-                             // in the LDS layout (null: built in LDS from dhist)
-    const float  *qhist32;   // FP32C rho: the same, FP32 tables scaled by the kick (or null)
+    const double *qhist;     // FP64 rho: kick-polynomial tables, slice m at qhist + m*stride_q, LDS layout
+    const float  *qhist32;   // FP32C rho: the same, FP32 tables scaled by the kick
     uint32_t stride_q, q2_off;        // LDS kick-polynomial table: slice stride, Q2 row (poly_stride, poly_q2)
     double   kick;                    // -dt/(2 dx), the scale of the kick polynomials
-    uint32_t pipe;                    // x-major: tables double-buffered and prefetched (poly_pipeline)
+    uint32_t pipe;                    // x-major: tables double-buffered (poly_ready_pipeline)
     double       *partials;  // velocity-major: partials[j*Nx + ix], j < wpx; x-major: j < ppx;
                              // MODE_METRICS: partials[4*g + k]
     uint64_t n;
@@ -170,7 +169,7 @@ bool trace3d_shape_ok( int variant, int order, const launch_shape &s );
 // ---- adjoint (adjoint1d.hip) ---------------------------------------------------
 struct adjoint1_params
 {
-    const double *dhist;      // extended differences (FP64), stride stride_d
+    const double *qhist;      // kick-polynomial tables (FP64), slice m at qhist + m*stride_q
     const double *g;          // dJ/drho_n (Nx)
     double       *lambda;     // folded slice adjoints, slice m at lambda + m*Nx
     double       *grad_part;  // per wave-task parameter-gradient accumulators [g][NUFI_MAX_MODES]
@@ -280,33 +279,25 @@ hipError_t launch_reduce_partials( const double *partials, uint32_t wpx, double 
 hipError_t launch_slice_tables( int dim, const double *slice, uint32_t len, uint32_t Nx, double *dslice,
                                 double *qslice, float *qslice32, double kick, float *slice32, hipStream_t st );
 
-// The 1D kick-polynomial table of one slice, built in LDS by the fast, FP32C
-// and adjoint kernels from the difference history: for cell i (i = -1 .. Nx,
-// periodic; entry j = i+1) the coefficients of 2 dphi/dt =
-// Q0 + Q1 t + Q2 t^2 on [0,1), with Q0 = D0 + D1, Q1 = 2 (D1 - D0),
-// Q2 = (D0 - D1) + (D2 - D1) from the differences D_l = c[i+l+1] - c[i+l] of
-// the cell's window (the FP32 table scaled by the kick -dt/(2 dx)).  Layout:
-// the pairs (Q0, Q1) at 2j, 2j+1, then the Q2 row at poly_q2 + j; the stride
-// is a multiple of 4 elements so that every slice starts 16-byte aligned in
-// FP64 and FP32.
-// Table windows of one chunk held in registers per thread while it is loaded
-// (the pipelined loop is unrolled over them): 6144 per 1024-thread workgroup,
-// 23 slices at Nx = 256.
-__host__ __device__ constexpr int poly_maxc( int /*block*/ ) { return 6; }
+// The 1D kick-polynomial table of one slice, written with the slice (field
+// and slice-table kernels) and copied into LDS by the fast, FP32C and adjoint
+// kernels: for cell i (i = -1 .. Nx, periodic; entry j = i+1) the
+// coefficients of 2 dphi/dt = Q0 + Q1 t + Q2 t^2 on [0,1), with Q0 = D0 + D1,
+// Q1 = 2 (D1 - D0), Q2 = (D0 - D1) + (D2 - D1) from the differences
+// D_l = c[i+l+1] - c[i+l] of the cell's window (the FP32 table scaled by the
+// kick -dt/(2 dx)).  Layout: the pairs (Q0, Q1) at 2j, 2j+1, then the Q2 row
+// at poly_q2 + j; the stride is a multiple of 4 elements so that every slice
+// starts 16-byte aligned in FP64 and FP32.
 __host__ __device__ inline uint32_t poly_q2( uint32_t Nx ) { return 2u * ( Nx + 2u ); }
 __host__ __device__ inline uint32_t poly_stride( uint32_t Nx ) { return ( 3u * ( Nx + 2u ) + 3u ) & ~3u; }
 // The table chunk of a 1D x-major launch for a requested chunk `req` >= 1:
-// the largest chunk up to req whose windows fit the register prefetch of a
-// workgroup of `block` threads (no limit when the tables are `ready` in
-// global memory and copied, not built) and whose two table buffers
-// (`elem`-byte entries) fit `lds_max` bytes, pipelined; failing that, one
-// buffer of up to req slices built directly (*pipe = false).  At least 1.
-inline uint32_t poly_chunk_plan( uint32_t Nx, int block, size_t elem, uint64_t req, size_t lds_max, bool *pipe,
-                                 bool ready = false )
+// the largest chunk up to req whose two table buffers (`elem`-byte entries)
+// fit `lds_max` bytes, double-buffered; failing that, one buffer of up to req
+// slices (*pipe = false).  At least 1.
+inline uint32_t poly_chunk_plan( uint32_t Nx, size_t elem, uint64_t req, size_t lds_max, bool *pipe )
 {
     const uint64_t row = uint64_t( poly_stride( Nx ) ) * elem;
-    const uint64_t pf  = ready ? req : uint64_t( block ) * uint64_t( poly_maxc( block ) ) / ( uint64_t( Nx ) + 2u );
-    uint64_t c = std::min( { req, pf, uint64_t( lds_max ) / ( 2 * row ) } );
+    uint64_t c = std::min( req, uint64_t( lds_max ) / ( 2 * row ) );
     *pipe = c >= 1;
     if ( c == 0 ) c = std::max<uint64_t>( 1, std::min<uint64_t>( req, uint64_t( lds_max ) / row ) );
     return uint32_t( std::min<uint64_t>( c, 0xffffffffu ) );

@@ -304,10 +304,9 @@ void trace1d_strict_kernel( trace1_params p )
 
 // ---- fast FP64 with the kick-polynomial table (order 4, rho) ----------------
 // Per char-step: drift (FMA), floor, index, one 16-byte and one 8-byte LDS read,
-// two FMAs for Q0 + t (Q1 + t Q2), one FMA for the kick.  When p.pipe is set
-// the tables are double-buffered: copied from the ready tables p.qhist by
-// direct loads (poly_ready_pipeline) or, without them, built in LDS from the
-// difference history (poly_pipeline); else built per chunk (poly_direct).
+// two FMAs for Q0 + t (Q1 + t Q2), one FMA for the kick.  The tables are
+// copied from p.qhist into LDS by direct loads, double-buffered when p.pipe is
+// set (poly_ready_pipeline), else one buffer per chunk (poly_ready_direct).
 template <int BLOCK, int NPT, bool POW2>
 __device__ __forceinline__ void trace1d_poly_body( const trace1_params &p, uint32_t wg, double *lds )
 {
@@ -325,7 +324,7 @@ __device__ __forceinline__ void trace1d_poly_body( const trace1_params &p, uint3
     const double Nxd = double( p.Nx ), invNx = 1.0 / Nxd;
     const int mask = int( p.Nx ) - 1;
     const double cd = p.dt * p.dx_inv, kick = p.kick;
-    const uint32_t sq = p.stride_q, q2 = p.q2_off, cells = p.Nx + 2;
+    const uint32_t sq = p.stride_q, q2 = p.q2_off;
 
     // slices mlo+hi down to mlo+lo, in segments between re-wrap points so that
     // the step loop itself carries no wrap; slice 0 (the half kick) peeled
@@ -358,9 +357,8 @@ __device__ __forceinline__ void trace1d_poly_body( const trace1_params &p, uint3
             }
         }
     };
-    if ( p.pipe && p.qhist ) poly_ready_pipeline<BLOCK>( lds, p.qhist, sq, p.n, p.chunk, steps );
-    else if ( p.pipe ) poly_pipeline<BLOCK,poly_maxc( BLOCK ),3>( lds, p.dhist, p.stride_d, sq, q2, cells, kick, p.n, p.chunk, steps );
-    else               poly_direct<BLOCK>( lds, p.dhist, p.stride_d, sq, q2, cells, kick, p.n, p.chunk, steps );
+    if ( p.pipe ) poly_ready_pipeline<BLOCK>( lds, p.qhist, sq, p.n, p.chunk, steps );
+    else          poly_ready_direct<BLOCK>( lds, p.qhist, sq, p.n, p.chunk, steps );
 
     double f[NPT];
     #pragma unroll
@@ -400,7 +398,7 @@ __device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint
     const int mask    = int( p.Nx ) - 1;
     const double cd   = p.dt * p.dx_inv;
     const float cd_hi = float( cd ), cd_lo = float( cd - double( cd_hi ) );
-    const uint32_t sq = p.stride_q, q2 = p.q2_off, cells = p.Nx + 2;
+    const uint32_t sq = p.stride_q, q2 = p.q2_off;
 
     // exact periodic wrap of the pair: sh - Nx*w is exact (Sterbenz) for w >= 1
     // and w <= -2, and Fast2Sum is exact for w = -1 (Nx >= |sh|)
@@ -459,9 +457,8 @@ __device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint
             if ( seg == 0 && m == 0 ) { step( Q, 0.5f ); --m; Q -= sq; }
         }
     };
-    if ( p.pipe && p.qhist32 ) poly_ready_pipeline<BLOCK>( ldsf, p.qhist32, sq, p.n, p.chunk, steps );
-    else if ( p.pipe ) poly_pipeline<BLOCK,poly_maxc( BLOCK ),2>( ldsf, p.dhist, p.stride_d, sq, q2, cells, p.kick, p.n, p.chunk, steps );
-    else          poly_direct<BLOCK>( ldsf, p.dhist, p.stride_d, sq, q2, cells, p.kick, p.n, p.chunk, steps );
+    if ( p.pipe ) poly_ready_pipeline<BLOCK>( ldsf, p.qhist32, sq, p.n, p.chunk, steps );
+    else          poly_ready_direct<BLOCK>( ldsf, p.qhist32, sq, p.n, p.chunk, steps );
 
     double f[NPT];
     #pragma unroll
@@ -649,11 +646,10 @@ void trace1d_xm_setup( trace1_params &p )
     p.ppx  = xm_ppx( p.Nu );
 }
 
-// the FP64 and FP32C rho kernels copy ready tables when the caller passes them
+// the FP64 and FP32C rho kernels read their tables from qhist / qhist32
 static bool poly_ready( const trace1_params &p, int variant )
 {
-    return p.mode == MODE_RHO && ( ( variant == TRACE_FAST && p.qhist != nullptr ) ||
-                                   ( variant == TRACE_FP32C && p.qhist32 != nullptr ) );
+    return ( variant == TRACE_FAST && p.qhist != nullptr ) || ( variant == TRACE_FP32C && p.qhist32 != nullptr );
 }
 
 uint32_t trace1d_chunk( const trace1_params &p, int variant, const launch_shape &s, size_t lds_max, bool *pipe )
@@ -661,8 +657,7 @@ uint32_t trace1d_chunk( const trace1_params &p, int variant, const launch_shape 
     uint64_t c = uint64_t( s.chunk );
     bool pp = false;
     if ( x_major( p, variant ) )
-        c = poly_chunk_plan( p.Nx, s.block, variant == TRACE_FP32C ? sizeof(float) : sizeof(double), c, lds_max, &pp,
-                             poly_ready( p, variant ) );
+        c = poly_chunk_plan( p.Nx, variant == TRACE_FP32C ? sizeof(float) : sizeof(double), c, lds_max, &pp );
     const uint64_t v = p.mode == MODE_METRICS ? ( p.n > 0 ? p.n + 1 : 0 ) : p.n;
     if ( v < c ) c = v > 0 ? v : 1;
     if ( pipe ) *pipe = pp;
@@ -710,6 +705,8 @@ hipError_t launch_trace1d( const trace1_params &p_in, int variant, const launch_
         if ( p.ix_count == 0 ) return hipSuccess;
     }
     else if ( p.n_tasks == 0 ) return hipSuccess;
+    // the x-major rho kernels copy the kick-polynomial tables of the history
+    if ( x_major( p, variant ) && !poly_ready( p, variant ) ) return hipErrorInvalidValue;
     const uint32_t grid = trace1d_grid( p, variant, s );
     bool pipe = false;
     p.chunk = trace1d_chunk( p, variant, s, 160 * 1024, &pipe );

@@ -188,65 +188,13 @@ def test_xm3_full_and_sub_ranges(n_space, n_vel):
         emulate3(n_space, n_vel, a, b, 256, 2)
 
 
-def poly_maxc(block):
-    return 6
-
-
-def poly_chunk_plan(Nx, block, elem, req, lds_max=160 * 1024, ready=False):
-    """internal.hpp poly_chunk_plan: (chunk, pipelined)."""
+def poly_chunk_plan(Nx, elem, req, lds_max=160 * 1024):
+    """internal.hpp poly_chunk_plan: (chunk, double-buffered)."""
     row = ((3 * (Nx + 2) + 3) & ~3) * elem
-    pf = req if ready else block * poly_maxc(block) // (Nx + 2)
-    c = min(req, pf, lds_max // (2 * row))
+    c = min(req, lds_max // (2 * row))
     if c >= 1:
         return c, True
     return max(1, min(req, lds_max // row)), False
-
-
-@pytest.mark.parametrize("Nx", [1, 2, 4, 8, 16, 20, 48, 64, 100, 256, 1000, 5000])
-def test_table_build_from_differences_stays_in_bounds(Nx):
-    """step1d.hpp poly_fetch / poly_pipeline and internal.hpp poly_chunk_plan:
-    window idx = tid + c*block of a chunk is slice jj = idx // cells, cell
-    j = idx % cells (cells = Nx+2); it reads difference entries goff..goff+2,
-    goff = jj*sd + j, of the chunk's rows (row length sd = (Nx+7) & ~3 for
-    order 4) and writes LDS at loff + j, loff + j + 1 (the pair, loff =
-    jj*sq + j) and loff + q2.  It belongs to a chunk of cnt slices iff
-    goff < cnt*sd; the others load the chunk's first window.  A pipelined
-    chunk fits the prefetch and two of its tables fit 160 KiB; otherwise one
-    table of the chunk (built directly) fits."""
-    cells = Nx + 2
-    sd = (Nx + 4 + 3) & ~3
-    q2 = 2 * cells
-    sq = (3 * cells + 3) & ~3
-    for elem in (8, 4):
-        for block in (64, 256, 512, 1024):
-            for want in (1, 4, 16, 64, 1 << 20):
-                chunk, pipe = poly_chunk_plan(Nx, block, elem, want)
-                fits = chunk * cells <= block * poly_maxc(block)
-                assert chunk <= want
-                if pipe:     # the pipelined loop: the prefetch holds a chunk, LDS two
-                    assert fits and 2 * chunk * sq * elem <= 160 * 1024
-                else:        # one buffer built directly
-                    assert chunk * sq * elem <= 160 * 1024 or chunk == 1
-                    continue
-                for cnt in sorted({1, chunk // 2 or 1, chunk}):
-                    hist = cnt * sd                       # the chunk's rows
-                    lds = cnt * sq
-                    written = set()
-                    for tid in range(block):
-                        for c in range(poly_maxc(block)):
-                            idx = tid + c * block
-                            jj, j = divmod(idx, cells)
-                            goff, loff = jj * sd + j, jj * sq + j
-                            member = goff < cnt * sd
-                            assert member == (idx < cnt * cells)
-                            g = goff if member else 0
-                            assert g + 2 < hist
-                            if member:
-                                for a in (loff + j, loff + j + 1, loff + q2):
-                                    assert a < lds and a not in written
-                                    written.add(a)
-                                assert loff + j + 1 < jj * sq + q2 and loff + q2 < (jj + 1) * sq
-                    assert len(written) == 3 * cnt * cells
 
 
 @pytest.mark.parametrize("Nx", [1, 2, 16, 24, 256, 1000, 3000, 5000])
@@ -260,8 +208,8 @@ def test_ready_table_copy_stays_in_bounds(Nx, n):
     rows.  Every word is copied once, inside the buffer (chunk*sq doubles), from
     the slices below n (the history holds Nt+1 >= n slices).  The FP32 tables
     (FP32C) have the same layout in floats: word q covers floats 4q .. 4q+3.
-    The ready plan has no prefetch limit; two buffers fit 160 KiB, else the
-    direct build."""
+    Two buffers fit 160 KiB, else one buffer of the chunk does
+    (poly_ready_direct, the same copy into one buffer)."""
     cells = Nx + 2
     sq = (3 * cells + 3) & ~3
     for elem in (8, 4):
@@ -273,13 +221,14 @@ def check_ready_copy(Nx, n, sq, elem):
     per = 16 // elem                       # table elements per 16-byte word
     for block in (256, 1024):
         for want in (1, 5, 13, 16, 1 << 20):
-            chunk, pipe = poly_chunk_plan(Nx, block, elem, want, ready=True)
+            chunk, pipe = poly_chunk_plan(Nx, elem, want)
             assert chunk <= want
-            if not pipe:
+            if pipe:
+                assert 2 * chunk * sq * elem <= 160 * 1024
+                assert chunk == min(want, 160 * 1024 // (2 * sq * elem))
+            else:
                 assert 2 * sq * elem > 160 * 1024
-                continue
-            assert 2 * chunk * sq * elem <= 160 * 1024
-            assert chunk == min(want, 160 * 1024 // (2 * sq * elem))
+                assert chunk * sq * elem <= 160 * 1024 or chunk == 1
             top = n - 1
             K = (top + chunk) // chunk
             covered = set()
