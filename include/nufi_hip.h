@@ -194,8 +194,8 @@ int nufi_set_trace_variant(nufi_ctx *ctx, int variant);
  * is not built or does not fit the LDS.  1D FAST and FP32C kernels take
  * (block/64)*nodes_per_thread a multiple of 16 (whole velocity groups per
  * workgroup) and treat the chunk as an upper bound: the library uses the
- * largest chunk whose tables the workgroup's register prefetch and two LDS
- * buffers hold (one LDS buffer built per chunk for wider grids).  The FAST
+ * largest chunk whose tables two LDS buffers hold (one LDS buffer per chunk
+ * for wider grids).  The FAST
  * and FP32C results do not depend on the shape (fixed summation groups and a
  * re-wrap schedule fixed by the slice index), so tuning never changes a
  * result. */
