@@ -285,9 +285,10 @@ def summarize_profiles(outdir, tag, commands):
                   "rate is 64 times the counter (a wave has 64 lanes); it counts what the kernel executes, "
                   "which for the 1D polynomial step is fewer FLOPs than the algorithmic 30 per char-step. "
                   "LDS bytes are 64 x SQ_INSTS_LDS_{LOAD,STORE}_BANDWIDTH; the L2 hit rate is "
-                  "TCC_HIT / (TCC_HIT + TCC_MISS).", "",
-                  "| kernel | mean us | FP64 TFLOP/s executed | FP32 TFLOP/s executed | LDS load+store TB/s (frac of peak) | LDS bank-conflict / VALU-busy cycles | L2 hit rate |",
-                  "|---|---|---|---|---|---|---|"]
+                  "TCC_HIT / (TCC_HIT + TCC_MISS). VALU issue is SQ_INSTS_VALU (wave instructions) per SIMD "
+                  f"({4 * CUS} SIMDs) and clock.", "",
+                  "| kernel | mean us | FP64 TFLOP/s executed | FP32 TFLOP/s executed | VALU instructions per SIMD per clock | LDS load+store TB/s (frac of peak) | LDS bank-conflict / VALU-busy cycles | L2 hit rate |",
+                  "|---|---|---|---|---|---|---|---|"]
         for k in sorted(set(pv) | set(pl)):
             if "nufi" not in k or k not in dur:
                 continue
@@ -297,14 +298,16 @@ def summarize_profiles(outdir, tag, commands):
             f32 = 64.0 * c.get("SQ_INSTS_VALU_FLOPS_FP32", 0.0) / ( us * 1e-6 ) / 1e12
             lds_b = 64.0 * ( c.get("SQ_INSTS_LDS_LOAD_BANDWIDTH", 0.0) + c.get("SQ_INSTS_LDS_STORE_BANDWIDTH", 0.0) )
             lds = lds_b / ( us * 1e-6 ) / 1e12
+            ipc = c.get("SQ_INSTS_VALU", 0.0) / ( 4 * CUS * us * 1e-6 * CLOCK_GHZ * 1e9 )
             busy = c.get("SQ_ACTIVE_INST_VALU", 0.0)
             conf = c.get("SQ_LDS_BANK_CONFLICT", 0.0) / busy if busy else float("nan")
             h = pl.get(k, {})
             hit, miss = h.get("TCC_HIT_sum", 0.0), h.get("TCC_MISS_sum", 0.0)
             hr = hit / ( hit + miss ) if hit + miss else float("nan")
-            lines.append(f"| `{k[:70]}` | {us:.1f} | {f64:.2f} | {f32:.2f} | {lds:.2f} ({lds / LDS_PEAK_TBS:.3f}) | "
-                         f"{conf:.3f} | {hr:.4f} |")
+            lines.append(f"| `{k[:70]}` | {us:.1f} | {f64:.2f} | {f32:.2f} | {ipc:.3f} | "
+                         f"{lds:.2f} ({lds / LDS_PEAK_TBS:.3f}) | {conf:.3f} | {hr:.4f} |")
             result.setdefault("counters", {})[k] = {"mean_us": us, "fp64_tflops": f64, "fp32_tflops": f32,
+                                                    "valu_insts_per_simd_clock": ipc,
                                                     "lds_tbs": lds, "lds_frac": lds / LDS_PEAK_TBS,
                                                     "l2_hit_rate": hr}
         lines.append("")
