@@ -59,6 +59,8 @@ def timed_runs(conf, precision, steps, warmup, flops_per_cs, tune_n=None, shape=
             s.run()
         tuned = None
         if tune_n is not None:          # untimed: pick the launch shape on this history
+            if warmup == 0:             # no run yet: compute the slices the probe step traces through
+                s.run(0, tune_n)
             tuned = s.autotune(tune_n, repeats=2)
         sync_all(s)
         s.set_timing(True)
