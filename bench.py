@@ -176,6 +176,64 @@ def adjoint_case():
             "forward_char_steps": conf.char_steps(0, conf.Nt + 1)}
 
 
+PEAK_PROBE_CMD = "python3 bench.py --peak-probe"
+
+
+def peak_probe():
+    """Measured MI355X peaks from libnufi_probes.so: FP64 and non-packed FP32 FMA
+    throughput with the core clock read inside the kernel, and LDS read
+    bandwidth for 8- and 16-byte reads with coherent or scattered lanes.
+    Writes profiles/peaks_mi355x.md and profiles/peaks_mi355x.json."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(ROOT, "numericalflowiteration_amd", "libnufi_probes.so"))
+    d = ctypes.c_double
+    out = {}
+    for name, fp64 in (("fp64_fma", 1), ("fp32_fma", 0)):
+        t, g = d(), d()
+        rc = lib.nufi_probe_fma(fp64, 100000, ctypes.byref(t), ctypes.byref(g))
+        if rc:
+            raise RuntimeError(f"nufi_probe_fma failed: HIP error {rc}")
+        out[name] = {"tflops": t.value, "core_ghz": g.value, "of_published": t.value / (PEAK_FP64 if fp64 else PEAK_FP32)}
+    for width in (8, 16):
+        for scattered in (0, 1):
+            t, g = d(), d()
+            rc = lib.nufi_probe_lds(width, scattered, 20000, ctypes.byref(t), ctypes.byref(g))
+            if rc:
+                raise RuntimeError(f"nufi_probe_lds failed: HIP error {rc}")
+            key = f"lds_read_b{width * 8}_{'scattered' if scattered else 'coherent'}"
+            out[key] = {"tbs": t.value, "core_ghz": g.value,
+                        "bytes_per_clk_per_cu": t.value * 1e12 / (CUS * g.value * 1e9) if g.value else None}
+    lines = ["# Measured MI355X peaks (microbenchmarks on the box)", "",
+             "Command (from the repository root on the MI355X box):", "", f"    {PEAK_PROBE_CMD}", "",
+             "FMA: 8 independent FMA chains per lane, 1024 threads per CU, one launch; the core clock is the "
+             "shader clock counter (s_memtime) against the 100 MHz real-time counter (s_memrealtime) over each "
+             "workgroup. LDS: 256-thread workgroups, 4 per CU, 8 reads per iteration from a 32 KiB array, lanes "
+             "on consecutive entries (coherent) or hashed over the array (scattered).", "",
+             "| probe | rate | core clock GHz | note |", "|---|---|---|---|"]
+    for k in ("fp64_fma", "fp32_fma"):
+        v = out[k]
+        lines.append(f"| {k} | {v['tflops']:.1f} TFLOP/s | {v['core_ghz']:.2f} | {v['of_published']:.2f} of the "
+                     f"published {PEAK_FP64 if k == 'fp64_fma' else PEAK_FP32} TFLOP/s |")
+    for k, v in out.items():
+        if k.startswith("lds"):
+            lines.append(f"| {k} | {v['tbs']:.1f} TB/s | {v['core_ghz']:.2f} | "
+                         f"{v['bytes_per_clk_per_cu']:.0f} B per clock per CU |")
+    with open(os.path.join(ROOT, "profiles", "peaks_mi355x.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    with open(os.path.join(ROOT, "profiles", "peaks_mi355x.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    return out
+
+
+def read_measured_peak(key):
+    """A measured peak from the committed probe summary (profiles/peaks_mi355x.json), if any."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "peaks_mi355x.json")) as f:
+            return json.load(f)[key]["tflops"]
+    except Exception:
+        return None
+
+
 def read_traffic(name):
     """HBM bytes per trace launch from the committed rocprofv3 --pmc summary, if any."""
     path = os.path.join(ROOT, "profiles", name)
@@ -281,7 +339,8 @@ def summarize_profiles(outdir, tag, commands):
         pl = per_counter(l2) if l2 is not None else {}
         lines += ["## VALU FLOPs, LDS bandwidth and L2 hit rate (--pmc, one pass each; per dispatch)", "",
                   f"Peaks (MI355X, {CUS} CUs at {CLOCK_GHZ} GHz): FP64 vector {PEAK_FP64} TFLOP/s, FP32 vector "
-                  f"{PEAK_FP32} TFLOP/s, LDS {LDS_PEAK_TBS:.1f} TB/s (128 B per clock per CU). The counters are "
+                  f"{PEAK_FP32} TFLOP/s, LDS {LDS_PEAK_TBS:.1f} TB/s (128 B per clock per CU, the CDNA3 figure; "
+                  "profiles/peaks_mi355x.md has the FMA and LDS rates measured on the box). The counters are "
                   "means per dispatch; rates divide them by the kernel's mean duration from the kernel trace. "
                   "SQ_INSTS_VALU_FLOPS_FP64/FP32 add the FLOPs of one lane per wave instruction, so the executed "
                   "rate is 64 times the counter (a wave has 64 lanes); it counts what the kernel executes, "
@@ -339,6 +398,8 @@ def main():
                          "(for profiling one kernel at a time; the JSON line then carries that case)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--dump-energies", default="", help="directory for the per-step energy arrays (.npy)")
+    ap.add_argument("--peak-probe", action="store_true",
+                    help="only measure the FMA and LDS peaks (writes profiles/peaks_mi355x.*), then exit")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy N MiB device-to-device (counter calibration), then exit")
     ap.add_argument("--summarize-profiles", nargs=2, metavar=("OUTDIR", "TAG"),
@@ -356,6 +417,10 @@ def main():
         print("bench.py: this build measures one GPU only (the rho path runs on a single device; "
               "the multi-GPU split is a design note, not built) — run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.peak_probe:
+        print(json.dumps(peak_probe()))
+        return
 
     if args.copy_probe:
         import torch
@@ -417,6 +482,8 @@ def main():
         "traffic_note": "HBM bytes per trace launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary",
         "kernel": "trace1d_poly_kernel" if prim_prec == "fp64" else "trace1d_fp32c_kernel",
         "flops_per_char_step": FLOPS_1D,
+        "peak_measured": read_measured_peak("fp64_fma" if prim_prec == "fp64" else "fp32_fma"),
+        "peak_measured_note": "FMA-chain probe on the box (profiles/peaks_mi355x.md); peak stays the published figure",
     }
     cpu = None
     if not args.no_cpu and hist is not None:

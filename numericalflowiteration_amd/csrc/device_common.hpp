@@ -149,8 +149,7 @@ __device__ __forceinline__ void dma_copy16( void *dst, const void *src, uint32_t
         {
             const char *g = static_cast<const char*>( src ) + size_t( q0 + lane ) * 16;
             const uint32_t m0 = base + q0 * 16u;
-            asm volatile( "s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
-                          :: "v"( g ), "s"( m0 ) : "memory", "m0" );
+            asm volatile( "s_nop 0\n\tglobal_load_lds_dwordx4 %0, off" :: "v"( g ), "{m0}"( m0 ) : "memory" );
         }
     }
 }
