@@ -193,7 +193,13 @@ def peak_probe():
         rc = lib.nufi_probe_fma(fp64, 100000, ctypes.byref(t), ctypes.byref(g))
         if rc:
             raise RuntimeError(f"nufi_probe_fma failed: HIP error {rc}")
-        out[name] = {"tflops": t.value, "core_ghz": g.value, "of_published": t.value / (PEAK_FP64 if fp64 else PEAK_FP32)}
+        out[name] = {"tflops": t.value, "core_ghz": g.value, "of_published": t.value / (PEAK_FP64 if fp64 else PEAK_FP32),
+                     "insts_per_simd_clock": t.value * 1e12 / (2 * 64 * 4 * CUS * g.value * 1e9) if g.value else None}
+    r, g = d(), d()
+    rc = lib.nufi_probe_int(100000, ctypes.byref(r), ctypes.byref(g))
+    if rc:
+        raise RuntimeError(f"nufi_probe_int failed: HIP error {rc}")
+    out["int32_xor_add"] = {"insts_per_simd_clock": r.value, "core_ghz": g.value}
     for width in (8, 16):
         for scattered in (0, 1):
             t, g = d(), d()
@@ -205,7 +211,7 @@ def peak_probe():
                         "bytes_per_clk_per_cu": t.value * 1e12 / (CUS * g.value * 1e9) if g.value else None}
     lines = ["# Measured MI355X peaks (microbenchmarks on the box)", "",
              "Command (from the repository root on the MI355X box):", "", f"    {PEAK_PROBE_CMD}", "",
-             "FMA: 8 independent FMA chains per lane, 1024 threads per CU, one launch; the core clock is the "
+             "FMA (and int32): 8 independent chains per lane, 1024 threads per CU, one launch; the core clock is the "
              "shader clock counter (s_memtime) against the 100 MHz real-time counter (s_memrealtime) over each "
              "workgroup. LDS: 256-thread workgroups, 4 per CU, 8 reads per iteration from a 32 KiB array, lanes "
              "on consecutive entries (coherent) or hashed over the array (scattered).", "",
@@ -213,7 +219,11 @@ def peak_probe():
     for k in ("fp64_fma", "fp32_fma"):
         v = out[k]
         lines.append(f"| {k} | {v['tflops']:.1f} TFLOP/s | {v['core_ghz']:.2f} | {v['of_published']:.2f} of the "
-                     f"published {PEAK_FP64 if k == 'fp64_fma' else PEAK_FP32} TFLOP/s |")
+                     f"published {PEAK_FP64 if k == 'fp64_fma' else PEAK_FP32} TFLOP/s; "
+                     f"{v['insts_per_simd_clock']:.3f} wave instructions per SIMD per clock |")
+    v = out["int32_xor_add"]
+    lines.append(f"| int32_xor_add | {v['insts_per_simd_clock']:.3f} wave instructions per SIMD per clock | "
+                 f"{v['core_ghz']:.2f} | 32-bit integer chains, same shape |")
     for k, v in out.items():
         if k.startswith("lds"):
             lines.append(f"| {k} | {v['tbs']:.1f} TB/s | {v['core_ghz']:.2f} | "

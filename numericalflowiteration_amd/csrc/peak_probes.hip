@@ -5,10 +5,11 @@
 //                   the core clock during the kernel from the shader clock
 //                   counter (s_memtime) against the 100 MHz real-time counter
 //                   (s_memrealtime);
+//   nufi_probe_int  32-bit integer VALU issue (xor + add chains), the same shape;
 //   nufi_probe_lds  LDS read bandwidth: 8- or 16-byte reads, lanes on
 //                   consecutive entries or hashed over the array, every index
 //                   masked into a 32 KiB LDS array.
-// Both return through out-parameters and 0 on success (a hipError_t
+// All return through out-parameters and 0 on success (a hipError_t
 // otherwise); they allocate and free their own small device buffers.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -44,6 +45,26 @@ __global__ __launch_bounds__( FMA_BLOCK ) void fma_probe( T *out, uint64_t *clk,
         for ( int k = 0; k < FMA_CHAINS; ++k ) x[k] = x[k] * a + b;
     }
     T s = 0;
+    #pragma unroll
+    for ( int k = 0; k < FMA_CHAINS; ++k ) s += x[k];
+    out[ blockIdx.x * FMA_BLOCK + threadIdx.x ] = s;
+    __syncthreads();
+    if ( threadIdx.x == 0 ) stamp( clk, c0, r0 );
+}
+
+__global__ __launch_bounds__( FMA_BLOCK ) void int_probe( uint32_t *out, uint64_t *clk, int iters, uint32_t a )
+{
+    uint64_t c0 = 0, r0 = 0;
+    if ( threadIdx.x == 0 ) { c0 = __builtin_amdgcn_s_memtime(); r0 = __builtin_amdgcn_s_memrealtime(); }
+    uint32_t x[FMA_CHAINS];
+    #pragma unroll
+    for ( int k = 0; k < FMA_CHAINS; ++k ) x[k] = threadIdx.x + uint32_t( k );
+    for ( int it = 0; it < iters; ++it )
+    {
+        #pragma unroll
+        for ( int k = 0; k < FMA_CHAINS; ++k ) x[k] = ( x[k] ^ a ) + 0x9e37u;
+    }
+    uint32_t s = 0;
     #pragma unroll
     for ( int k = 0; k < FMA_CHAINS; ++k ) s += x[k];
     out[ blockIdx.x * FMA_BLOCK + threadIdx.x ] = s;
@@ -155,6 +176,35 @@ extern "C" int nufi_probe_fma( int fp64, int iters, double *tflops, double *ghz 
         const double flops = 2.0 * FMA_CHAINS * double( iters ) * double( grid ) * FMA_BLOCK;
         *tflops = flops / ( double( ms ) * 1e-3 ) / 1e12;
         *ghz = clock_ghz( h, grid );
+    }
+    delete[] h;
+    if ( clk ) (void)hipFree( clk );
+    if ( out ) (void)hipFree( out );
+    return int( e );
+}
+
+// wave instructions per SIMD per clock of the core clock measured in the kernel
+extern "C" int nufi_probe_int( int iters, double *insts_per_simd_clock, double *ghz )
+{
+    const int grid = cu_count();
+    if ( grid <= 0 || iters <= 0 || insts_per_simd_clock == nullptr || ghz == nullptr ) return int( hipErrorInvalidValue );
+    uint32_t *out = nullptr;
+    uint64_t *clk = nullptr;
+    hipError_t e = hipMalloc( reinterpret_cast<void**>( &out ), size_t( grid ) * FMA_BLOCK * sizeof( uint32_t ) );
+    if ( e == hipSuccess ) e = hipMalloc( reinterpret_cast<void**>( &clk ), size_t( grid ) * 2 * sizeof( uint64_t ) );
+    uint64_t *h = new uint64_t[ size_t( grid ) * 2 ];
+    float ms = 0.0f;
+    if ( e == hipSuccess )
+        e = timed( [&]( bool warm )
+                   {
+                       hipLaunchKernelGGL( int_probe, dim3( grid ), dim3( FMA_BLOCK ), 0, nullptr, out, clk,
+                                           warm ? 100 : iters, 0x1234u );
+                   }, clk, h, grid, &ms );
+    if ( e == hipSuccess )
+    {
+        *ghz = clock_ghz( h, grid );
+        const double insts = 2.0 * FMA_CHAINS * double( iters ) * double( grid ) * FMA_BLOCK / 64.0;   // wave instructions
+        *insts_per_simd_clock = *ghz > 0.0 ? insts / ( double( ms ) * 1e-3 ) / ( 4.0 * grid ) / ( *ghz * 1e9 ) : 0.0;
     }
     delete[] h;
     if ( clk ) (void)hipFree( clk );
