@@ -45,6 +45,18 @@ def test_binding_covers_exactly_the_header():
     assert set(_capi.SIGNATURES) == set(declared_functions())
 
 
+def test_peak_probe_library_exports_its_probes():
+    """bench.py --peak-probe loads libnufi_probes.so (a separate library: the
+    microbenchmarks are not part of the rho-path ABI)."""
+    path = os.path.join(ROOT, "numericalflowiteration_amd", "libnufi_probes.so")
+    lib = ctypes.CDLL(path)
+    for name in ("nufi_probe_fma", "nufi_probe_int", "nufi_probe_lds"):
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", _capi.library_path()], capture_output=True, text=True,
+                         check=True).stdout
+    assert "nufi_probe" not in out
+
+
 def test_struct_layout_matches_header(tmp_path):
     """Compile the header with gcc and compare sizeof/offsetof with the ctypes mirror."""
     fields = [f for f, _ in C.Config._fields_]
