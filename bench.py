@@ -50,7 +50,7 @@ def sync_all(sched):
         pass
 
 
-def timed_runs(conf, precision, steps, warmup, flops_per_cs, tune_n=None, shape=None):
+def timed_runs(conf, precision, steps, warmup, flops_per_cs, tune_n=None, shape=None, keep_history=False):
     import numericalflowiteration_amd as nufi
     with nufi.GpuScheduler(conf, precision=precision) as s:
         if shape is not None:
@@ -72,6 +72,7 @@ def timed_runs(conf, precision, steps, warmup, flops_per_cs, tune_n=None, shape=
         wall = time.perf_counter() - t0
         launches, kernel_ms, csteps = s.trace_timing()
         shape = s.launch_shape()
+        history = s.history() if keep_history else None
     cs_run = conf.char_steps(0, conf.Nt + 1)
     achieved = flops_per_cs * csteps / launches / (kernel_ms / 1e3 / launches) / 1e12   # TFLOP/s
     return dict(
@@ -82,7 +83,7 @@ def timed_runs(conf, precision, steps, warmup, flops_per_cs, tune_n=None, shape=
         trace_flops_per_launch=flops_per_cs * csteps / launches,
         achieved_tflops=achieved, launch_shape=list(shape),
         autotune_probe=None if tuned is None else {"n": tune_n, "best_ms": tuned[1]},
-    ), energies
+    ), energies, history
 
 
 def landau3d_forms(args):
@@ -91,11 +92,12 @@ def landau3d_forms(args):
     import numericalflowiteration_amd as nufi
     c5 = nufi.landau_3d()
     t3 = None if args.no_autotune else c5.Nt // 2     # a mid-run step: short probes pick chunks that lose later
-    r64, f64 = timed_runs(c5, nufi.FP64, 1, 0, FLOPS_3D, t3)
+    r64, f64, h64 = timed_runs(c5, nufi.FP64, 1, 0, FLOPS_3D, t3, keep_history=True)
     progress(f"3D3V Landau FP64 {r64['ms_per_run'] / 1e3:.1f} s per run")
-    r32, f32 = timed_runs(c5, nufi.FP32C, 1, 0, FLOPS_3D, t3)
+    r32, f32, h32 = timed_runs(c5, nufi.FP32C, 1, 0, FLOPS_3D, t3, keep_history=True)
     progress(f"3D3V Landau FP32C {r32['ms_per_run'] / 1e3:.1f} s per run")
     r32.update(energy_errors(f32, f64))
+    r32.update(field_errors(h32, h64, c5.stride_t()))
     r32["speedup_vs_fp64"] = r32["cs_per_s_e2e"] / r64["cs_per_s_e2e"]
     if args.dump_energies:
         np.save(os.path.join(args.dump_energies, "landau3d_energy_fp64.npy"), f64)
@@ -103,6 +105,15 @@ def landau3d_forms(args):
     return {"fp64": r64, "fp32c": r32,
             "workload": "3D3V Landau, x in [0,4pi)^3, (1+0.01 sum cos(0.5x_d)) M3(v), "
                         "v in [-6,6]^3, 16^3 x 32^3 nodes, dt=1/8, T=20 (161 steps)"}
+
+
+def field_errors(h, ref, stride):
+    """Normwise relative error of the E-field spline coefficients (the history
+    slices) of a run against the FP64 run, maximum over the time steps."""
+    import numpy as np
+    a, b = h.reshape(-1, stride), ref.reshape(-1, stride)
+    errs = np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
+    return {"field_coeff_max_rel_err_vs_fp64": float(errs.max()), "field_coeff_err_at_step": int(errs.argmax())}
 
 
 def energy_errors(e, ref):
@@ -456,16 +467,18 @@ def main():
         return
 
     if args.only in ("fp64", "both"):
-        out["fp64"], e64 = timed_runs(conf, nufi.FP64, args.steps, args.warmup, FLOPS_1D, tune, shape)
+        out["fp64"], e64, _ = timed_runs(conf, nufi.FP64, args.steps, args.warmup, FLOPS_1D, tune, shape)
         progress(f"two-stream FP64 {out['fp64']['ms_per_run']:.1f} ms per run")
         with nufi.GpuScheduler(conf) as s:
             s.run()
             hist = s.history()
     if args.only in ("fp32c", "both"):
-        out["fp32c"], e32 = timed_runs(conf, nufi.FP32C, args.steps, args.warmup, FLOPS_1D, tune)
+        out["fp32c"], e32, h32 = timed_runs(conf, nufi.FP32C, args.steps, args.warmup, FLOPS_1D, tune,
+                                            keep_history=True)
         progress(f"two-stream FP32C {out['fp32c']['ms_per_run']:.1f} ms per run")
         if e64 is not None:
             out["fp32c"].update(energy_errors(e32, e64))
+            out["fp32c"].update(field_errors(h32, hist, conf.stride_t()))
             out["fp32c"]["speedup_vs_fp64"] = out["fp32c"]["cs_per_s_e2e"] / out["fp64"]["cs_per_s_e2e"]
 
     if not args.no_3d and args.only == "both" and args.case == "all":
