@@ -11,6 +11,7 @@
  */
 #include "nufi_oracle.hpp"
 #include "dual.hpp"
+#include "opcount.hpp"
 
 #include <omp.h>
 #include <cstdio>
@@ -493,6 +494,43 @@ int oracle_metrics3d( const nufi_config *c, int order, uint64_t n, const double 
 }
 
 int oracle_max_threads( void ) { return omp_get_max_threads(); }
+
+/* Arithmetic of one characteristic step (drift, E, kick) in the reference's
+ * formulation, order 4, counted on opcount (opcount.hpp). */
+int oracle_count_step_ops( const nufi_config *c, long *ops )
+{
+    if ( c == nullptr || ops == nullptr ) return NUFI_ERR_INVALID;
+    if ( c->dim == 1 )
+    {
+        const grid1<opcount> g( *c );
+        std::vector<opcount> slice( c->Nx + 3 );
+        for ( size_t i = 0; i < slice.size(); ++i ) slice[i] = opcount::variable( 0.01 * double( i % 7 ) );
+        opcount x = opcount::variable( c->x_min + 0.37 * c->Lx ), u = opcount::variable( 0.7 );
+        opcount::n = 0;
+        x = x - g.dt*u;
+        const opcount E = -field1d<opcount,4,1>( x, slice.data(), g );
+        u = u + g.dt*E;
+        *ops = opcount::n;
+        return 0;
+    }
+    if ( c->dim == 3 )
+    {
+        const grid3<opcount> g( *c );
+        std::vector<opcount> slice( ( c->Nx + 3 ) * ( c->Ny + 3 ) * ( c->Nz + 3 ) );
+        for ( size_t i = 0; i < slice.size(); ++i ) slice[i] = opcount::variable( 0.01 * double( i % 7 ) );
+        opcount x = opcount::variable( c->x_min + 0.37 * c->Lx ), y = opcount::variable( c->y_min + 0.61 * c->Ly ),
+                z = opcount::variable( c->z_min + 0.23 * c->Lz );
+        opcount u = opcount::variable( 0.7 ), v = opcount::variable( -0.4 ), w = opcount::variable( 1.1 );
+        opcount Ex, Ey, Ez;
+        opcount::n = 0;
+        x = x - g.dt*u; y = y - g.dt*v; z = z - g.dt*w;
+        efield3d<opcount,4>( x, y, z, slice.data(), g, Ex, Ey, Ez );
+        u = u + g.dt*Ex; v = v + g.dt*Ey; w = w + g.dt*Ez;
+        *ops = opcount::n;
+        return 0;
+    }
+    return NUFI_ERR_UNSUPPORTED;
+}
 
 /* J = sum_{n<n_end} dt*energy_n and its forward-mode tangent dJ/d(parameters). */
 int oracle_tangent_run1d( const nufi_config *c, uint64_t n_end, double *J, double *dJ, int nthreads )

@@ -60,6 +60,7 @@ def lib():
             "oracle_metrics1d": [_cp, _cp, _int, _u64, _dp, _dp, _int],
             "oracle_metrics3d": [_cp, _int, _u64, _dp, _dp, _int],
             "oracle_tangent_run1d": [_cp, _u64, _dp, _dp, _int],
+            "oracle_count_step_ops": [_cp, ctypes.POINTER(ctypes.c_long)],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -116,6 +117,13 @@ def field3d(conf: Config, slice_, x, y, z, d=(0, 0, 0), order: int = 4) -> float
     _check(lib().oracle_field3d(ctypes.byref(conf), order, d[0], d[1], d[2], x, y, z, _p(s), _p(out)),
            "field3d")
     return float(out[0])
+
+
+def count_step_ops(conf: Config) -> int:
+    """Arithmetic operations of one characteristic step in the reference's formulation (order 4)."""
+    n = ctypes.c_long(0)
+    _check(lib().oracle_count_step_ops(ctypes.byref(conf), ctypes.byref(n)), "count_step_ops")
+    return int(n.value)
 
 
 def trace1d(conf: Config, n: int, x: float, u: float, hist, half_kick=False, order: int = 4) -> float:
