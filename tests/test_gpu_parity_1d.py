@@ -206,16 +206,17 @@ def test_rho_bit_identical_across_launch_shapes(conf_args, prec, variant):
 
 
 @pytest.mark.parametrize("Nx,prec,variant,tol", [
-    (4096, nufi.FP64, _capi.TRACE_FAST, FP64_TOL),     # tables too wide for two LDS buffers: built per chunk
-    (4096, nufi.FP32C, _capi.TRACE_FP32C, FP32C_TOL),  # pipelined, one slice per chunk
-    (6400, nufi.FP64, _capi.TRACE_FAST, FP64_TOL),     # wider than the prefetch: built per chunk
-    (6400, nufi.FP32C, _capi.TRACE_FP32C, FP32C_TOL),
+    (4096, nufi.FP64, _capi.TRACE_FAST, FP64_TOL),     # two FP64 tables exceed 160 KiB: one buffer per chunk
+    (4096, nufi.FP32C, _capi.TRACE_FP32C, FP32C_TOL),  # double-buffered, one slice per chunk
+    (6400, nufi.FP64, _capi.TRACE_FAST, FP64_TOL),     # one buffer per chunk
+    (6400, nufi.FP32C, _capi.TRACE_FP32C, FP32C_TOL),  # double-buffered, one slice per chunk
+    (8192, nufi.FP32C, _capi.TRACE_FP32C, FP32C_TOL),  # two FP32 tables exceed 160 KiB: one buffer per chunk
 ])
 def test_wide_grid_table_paths(Nx, prec, variant, tol):
-    """The fast and FP32C kernels on grids whose kick-polynomial tables do not
-    fit the double-buffered pipeline, against the oracle on a synthetic smooth
-    periodic history (no field solve: 6400 points exceed the device field
-    kernel's single workgroup)."""
+    """The fast and FP32C kernels on wide grids, where the kick-polynomial
+    tables of a chunk take one slice per LDS buffer or a single buffer, against
+    the oracle on a synthetic smooth periodic history (no field solve: these
+    grids exceed the device field kernel's single workgroup)."""
     conf = nufi.config_1d(Nx, 64, 1 / 16, 1.0, 0.0, 4 * math.pi, -10.0, 10.0,
                           nufi.config.F0_LANDAU_1D, 0.05, 0.5)
     st = conf.stride_t()
