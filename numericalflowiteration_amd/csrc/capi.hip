@@ -136,6 +136,27 @@ int validate( const nufi_config *c, int order )
     return NUFI_OK;
 }
 
+// The LDS that one history slice takes in the trace kernel of `variant` (the
+// smallest staging any of its launch shapes uses): 1D fast / FP32C one row of
+// kick-polynomial tables, 1D strict one difference row, 3D one slice (FP32
+// for FP32C).  Host-only, so grids no kernel can stage are refused before any
+// device work.
+constexpr size_t LDS_LIMIT = 160 * 1024;    // gfx950 LDS per workgroup
+
+size_t slice_lds_bytes( const nufi_config *c, int order, int variant )
+{
+    const uint64_t o1 = uint64_t( order ) - 1;
+    if ( c->dim == 1 )
+    {
+        if ( variant == TRACE_STRICT ) return size_t( ( c->Nx + order + 3 ) & ~uint64_t( 3 ) ) * sizeof(double);
+        const size_t elem = variant == TRACE_FP32C ? sizeof(float) : sizeof(double);
+        return size_t( poly_stride( uint32_t( c->Nx ) ) ) * elem;
+    }
+    const uint64_t st = ( c->Nx + o1 ) * ( c->Ny + o1 ) * ( c->Nz + o1 );
+    if ( variant == TRACE_FP32C ) return size_t( ( st + 3 ) & ~uint64_t( 3 ) ) * sizeof(float);
+    return size_t( st ) * sizeof(double);
+}
+
 // Host precomputation of the Fourier tables (long double), uploaded once.
 int build_tables( nufi_ctx *c )
 {
@@ -308,6 +329,14 @@ int launch_trace( nufi_ctx *c, uint64_t n, uint64_t q_begin, uint64_t q_end, uin
     return NUFI_OK;
 }
 
+// The scale the field and slice-table kernels apply to the per-slice tables:
+// 1D the kick polynomials' -dt/(2 dx); 3D the FP32 slice copy's fp32c3_scale.
+double table_kick( const nufi_ctx *c )
+{
+    if ( c->dim == 1 ) return -0.5 * c->conf.dt * c->conf.dx_inv;
+    return fp32c3_scale( c->conf.dt, c->conf.dx_inv );
+}
+
 double rho_weight( const nufi_ctx *c )
 {
     // -du per node (cuda_kernel.cu:47-50); 3D: -du*dv*dw (cuda_kernel.cu:419-425)
@@ -384,6 +413,16 @@ int nufi_create( const nufi_config *conf, const nufi_config *conf_metrics, int o
     if ( conf->dim == 3 && order != 4 )
         return fail( nullptr, NUFI_ERR_UNSUPPORTED, "nufi_create: 3D is built for order 4 only" );
 
+    // the trace kernels stage whole slices in LDS: 1D FP64 grids too wide for
+    // one kick-polynomial row take the strict kernel (one difference row)
+    int variant = precision == NUFI_FP32C ? TRACE_FP32C : ( ( conf->dim == 1 && order == 4 ) || conf->dim == 3 ? TRACE_FAST : TRACE_STRICT );
+    if ( variant == TRACE_FAST && conf->dim == 1 && slice_lds_bytes( conf, order, TRACE_FAST ) > LDS_LIMIT )
+        variant = TRACE_STRICT;
+    if ( slice_lds_bytes( conf, order, variant ) > LDS_LIMIT )
+        return fail( nullptr, NUFI_ERR_UNSUPPORTED,
+                     "nufi_create: one history slice of this grid (" + std::to_string( slice_lds_bytes( conf, order, variant ) ) +
+                     " bytes in the trace kernel's layout) exceeds the 160 KiB of LDS the kernel stages it in" );
+
     int ndev = 0;
     hipError_t he = hipGetDeviceCount( &ndev );
     if ( he != hipSuccess || ndev == 0 )
@@ -395,7 +434,7 @@ int nufi_create( const nufi_config *conf, const nufi_config *conf_metrics, int o
     c->conf = *conf;
     c->conf_m = conf_metrics ? *conf_metrics : *conf;
     c->order = order; c->precision = precision; c->device = device; c->dim = conf->dim;
-    c->variant = precision == NUFI_FP32C ? TRACE_FP32C : ( ( c->dim == 1 && order == 4 ) || c->dim == 3 ? TRACE_FAST : TRACE_STRICT );
+    c->variant = variant;
 
     auto bail = [&]( int code ) { std::string m = c->err; free_all( c ); delete c; return fail( nullptr, code, m ); };
 
@@ -549,7 +588,7 @@ int nufi_upload_phi( nufi_ctx *c, uint64_t n, const double *history_base )
                                         c->dhist ? c->dhist + n * c->sd : nullptr,
                                         c->qhist ? c->qhist + n * c->sq : nullptr,
                                         c->qhist32 ? c->qhist32 + n * c->sq : nullptr,
-                                        -0.5 * c->conf.dt * c->conf.dx_inv,
+                                        table_kick( c ),
                                         c->hist32 ? c->hist32 + n * c->st32 : nullptr, c->stream );
     if ( e != hipSuccess ) return hip_fail( c, e, "slice table launch" );
     // the copy reads host memory the caller may reuse: complete it before returning
@@ -565,6 +604,8 @@ int nufi_compute_metrics( nufi_ctx *c, uint64_t n, uint64_t q_begin, uint64_t q_
     if ( c == nullptr ) return NUFI_ERR_INVALID;
     if ( n > c->conf.Nt ) return fail( c, NUFI_ERR_RANGE, "compute_metrics: time step out of range" );
     if ( q_begin > q_end || q_end > c->m_nodes ) return fail( c, NUFI_ERR_RANGE, "compute_metrics: node range out of range" );
+    if ( slice_lds_bytes( &c->conf, c->order, c->dim == 1 || c->variant == TRACE_STRICT ? TRACE_STRICT : TRACE_FAST ) > LDS_LIMIT )
+        return fail( c, NUFI_ERR_UNSUPPORTED, "compute_metrics: one FP64 slice of this grid exceeds the LDS" );
     HIP_TRY( c, hipSetDevice( c->device ) );
     uint64_t tasks = 0;
     if ( q_begin < q_end )
@@ -664,7 +705,7 @@ static int step_async( nufi_ctx *c, uint64_t n, double *rho_out )
     f.qslice = c->qhist ? c->qhist + n * c->sq : nullptr;
     f.qslice32 = c->qhist32 ? c->qhist32 + n * c->sq : nullptr;
     f.q2_off = poly_q2( uint32_t( c->conf.Nx ) );
-    f.kick = -0.5 * c->conf.dt * c->conf.dx_inv;
+    f.kick = table_kick( c );
     f.slice32 = c->hist32 ? c->hist32 + n * c->st32 : nullptr;
     f.energy = c->energies + n;
     f.rho_out = rho_out ? rho_out : c->rho;
@@ -730,11 +771,21 @@ int nufi_adjoint_grad( nufi_ctx *c, double *dJ_da, int n_par, double *J_out )
         if ( ( rc = dev_alloc( c, &c->grad_part, size_t( n_tasks ) * NUFI_MAX_MODES, "adjoint partials" ) ) ) return rc;
         if ( ( rc = dev_alloc( c, &c->grad, NUFI_MAX_MODES, "adjoint gradient" ) ) ) return rc;
     }
-    // forward run (the FP64 fast trace, as nufi_run), keeping rho_n
+    // forward run (the FP64 fast trace, as nufi_run), keeping rho_n; a STRICT
+    // context's launch shape is not one the fast kernel takes, so the fast
+    // default shape runs it and both are restored afterwards
     const int saved_variant = c->variant;
-    c->variant = TRACE_FAST;
+    const launch_shape saved_shape = c->shape;
+    if ( saved_variant != TRACE_FAST )
+    {
+        if ( slice_lds_bytes( &c->conf, c->order, TRACE_FAST ) > LDS_LIMIT )
+            return fail( c, NUFI_ERR_UNSUPPORTED, "adjoint_grad: grid too large for the fast trace" );
+        c->variant = TRACE_FAST;
+        default_shape( c );
+    }
     for ( uint64_t n = 0; n <= Nt && ( rc = step_async( c, n, c->rho_hist + n * Nx ) ) == NUFI_OK; ++n ) {}
     c->variant = saved_variant;
+    c->shape = saved_shape;
     if ( rc ) return rc;
     HIP_TRY( c, hipMemsetAsync( c->lambda, 0, ( Nt + 1 ) * Nx * sizeof(double), c->stream ) );
     HIP_TRY( c, hipMemsetAsync( c->grad_part, 0, size_t( n_tasks ) * NUFI_MAX_MODES * sizeof(double), c->stream ) );
@@ -963,6 +1014,8 @@ int nufi_set_trace_variant( nufi_ctx *c, int variant )
         return fail( c, NUFI_ERR_UNSUPPORTED, "set_trace_variant: FP64 contexts take FAST or STRICT" );
     if ( variant == TRACE_FAST && c->order != 4 )
         return fail( c, NUFI_ERR_UNSUPPORTED, "set_trace_variant: FAST is built for order 4 only" );
+    if ( slice_lds_bytes( &c->conf, c->order, variant ) > LDS_LIMIT )
+        return fail( c, NUFI_ERR_UNSUPPORTED, "set_trace_variant: one slice of this grid exceeds the LDS in that kernel's layout" );
     c->variant = variant;
     default_shape( c );
     return NUFI_OK;

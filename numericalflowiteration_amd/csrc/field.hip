@@ -165,7 +165,7 @@ __device__ __forceinline__ void field_body( const field_params &p, double2 *lds2
             const uint32_t i = e2 % ex, j = ( e2 / ex ) % ey, k = e2 / ( ex * ey );
             const double c = R[ ( ( k % p.Nz ) * p.Ny + ( j % p.Ny ) ) * p.Nx + ( i % p.Nx ) ].x;
             p.slice[ k * p.sz + j * p.sy + i ] = c;
-            if ( p.slice32 ) p.slice32[ k * p.sz + j * p.sy + i ] = float( c );
+            if ( p.slice32 ) p.slice32[ k * p.sz + j * p.sy + i ] = float( c * p.kick );
         }
     }
 }
@@ -205,7 +205,8 @@ __global__ __launch_bounds__( 64 ) void reduce_partials_kernel( const double *pa
     rho[i] = v;
 }
 
-// FP32 copy of an uploaded slice of `len` reals and, in 1D (period Nx), the
+// FP32 copy of an uploaded slice of `len` reals (3D: scaled by `kick`, which
+// is then fp32c3_scale) and, in 1D (period Nx), the
 // extended differences dslice[j] = c[j mod Nx] - c[(j-1) mod Nx], j <= len,
 // and the kick-polynomial table entries j < Nx+2 from dslice[j .. j+2].
 __global__ void slice_tables_kernel( int dim, const double *slice, uint32_t len, uint32_t Nx, double *dslice,
@@ -213,7 +214,7 @@ __global__ void slice_tables_kernel( int dim, const double *slice, uint32_t len,
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if ( i > len ) return;
-    if ( slice32 && i < len ) slice32[i] = float( slice[i] );
+    if ( slice32 && i < len ) slice32[i] = float( slice[i] * kick );
     if ( dim == 1 && dslice ) dslice[i] = slice[ i % Nx ] - slice[ ( i + Nx - 1 ) % Nx ];
     if ( dim == 1 && ( qslice || qslice32 ) && i < Nx + 2 )
     {

@@ -244,8 +244,8 @@ struct field_params
     double  *qslice;             // 1D FP64 kick-polynomial table of slice n (poly_stride layout) or null
     float   *qslice32;           // 1D FP32 table of slice n, scaled by kick, or null
     uint32_t q2_off;             // their Q2 row (poly_q2)
-    double   kick;               // -dt/(2 dx)
-    float   *slice32;            // FP32 coefficients (3D FP32C) or null
+    double   kick;               // 1D: -dt/(2 dx); 3D: the FP32 slice scale fp32c3_scale
+    float   *slice32;            // FP32 coefficients times kick (3D FP32C) or null
     double  *energy;             // energies[n]
     uint32_t sy, sz, stride_d;
 };
@@ -274,6 +274,12 @@ hipError_t launch_reduce_metrics( const double *partials, uint32_t n_tasks, doub
 hipError_t launch_reduce_partials( const double *partials, uint32_t wpx, double weight,
                                    uint32_t first, uint32_t count, uint32_t n_space,
                                    double *rho, hipStream_t st );
+// The 3D FP32C trace reads the slice as float( c * fp32c3_scale ): the kick
+// -dt/dx over 72 = 2*6*6, the factor of one derivative basis (2 D) and two
+// value bases (6 B) in their integer-coefficient form, so the trace's kick is
+// the gradient itself and no rounded constant scales every node's kick alike.
+inline double fp32c3_scale( double dt, double dx_inv ) { return -dt * dx_inv / 72.0; }
+
 // Differences and the FP64 / FP32 kick-polynomial tables (1D) and the FP32
 // copy (3D FP32C) of an uploaded slice.
 hipError_t launch_slice_tables( int dim, const double *slice, uint32_t len, uint32_t Nx, double *dslice,

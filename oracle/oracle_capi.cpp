@@ -178,6 +178,26 @@ struct rho3d_op
     }
 };
 
+/* rho at a list of spatial points (the x loop of rho3d_op restricted to
+ * them, OpenMP over the points as bin/test_nufi_cpu_3d.cpp:68-72 is over x):
+ * isolated steps on a full-size 3D grid at a sample of points. */
+template <int order>
+struct rho3d_points_op
+{
+    static int run( const nufi_config *c, uint64_t n, const double *hist, const uint64_t *points,
+                    uint64_t count, double *rho, int nthreads, int exact_sum )
+    {
+        grid3<double> g( *c );
+        const uint64_t N = g.Nx*g.Ny*g.Nz;
+        for ( uint64_t i = 0; i < count; ++i ) if ( points[i] >= N ) return NUFI_ERR_RANGE;
+        #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+        for ( long long i = 0; i < (long long)count; ++i )
+            rho[i] = exact_sum ? rho3d<double,order,long double>( n, (size_t)points[i], hist, g )
+                               : rho3d<double,order>( n, (size_t)points[i], hist, g );
+        return 0;
+    }
+};
+
 /* ---- fit ---------------------------------------------------------------- */
 template <int order>
 struct fit1d_op
@@ -420,6 +440,13 @@ int oracle_rho1d_exact_sum( const nufi_config *c, int order, uint64_t n, const d
 int oracle_rho3d_exact_sum( const nufi_config *c, int order, uint64_t n, const double *hist, double *rho,
                             int nthreads )
 { if ( bad( c, 3 ) ) return NUFI_ERR_INVALID; return dispatch_order<rho3d_op>( order, c, n, hist, rho, nthreads, 1 ); }
+
+int oracle_rho3d_points( const nufi_config *c, int order, uint64_t n, const double *hist, const uint64_t *points,
+                         uint64_t count, double *rho, int nthreads, int exact_sum )
+{
+    if ( bad( c, 3 ) || ( count && ( points == nullptr || rho == nullptr ) ) ) return NUFI_ERR_INVALID;
+    return dispatch_order<rho3d_points_op>( order, c, n, hist, points, count, rho, nthreads, exact_sum );
+}
 
 int oracle_poisson1d( const nufi_config *c, double *data, double *energy )
 {

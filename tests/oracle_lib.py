@@ -49,6 +49,7 @@ def lib():
             "oracle_rho3d": [_cp, _int, _u64, _dp, _dp, _int],
             "oracle_rho1d_exact_sum": [_cp, _int, _u64, _dp, _dp, _int],
             "oracle_rho3d_exact_sum": [_cp, _int, _u64, _dp, _dp, _int],
+            "oracle_rho3d_points": [_cp, _int, _u64, _dp, ctypes.POINTER(ctypes.c_uint64), _u64, _dp, _int, _int],
             "oracle_poisson1d": [_cp, _dp, _dp],
             "oracle_poisson3d": [_cp, _dp, _dp],
             "oracle_fit1d": [_cp, _int, _dp, _dp],
@@ -157,6 +158,17 @@ def rho(conf: Config, n: int, hist, order: int = 4, nthreads: int | None = None,
     else:
         f = lib().oracle_rho1d if conf.dim == 1 else lib().oracle_rho3d
     _check(f(ctypes.byref(conf), order, n, _p(hist), _p(out), nthreads or nthreads_default()), "rho")
+    return out
+
+
+def rho3d_points(conf: Config, n: int, hist, points, order: int = 4, nthreads: int | None = None,
+                 exact_sum: bool = False) -> np.ndarray:
+    """3D rho (rho.hpp:428-462) at the spatial points l = i + j Nx + k Nx Ny listed."""
+    pts = np.ascontiguousarray(points, dtype=np.uint64)
+    out = np.zeros(pts.size)
+    _check(lib().oracle_rho3d_points(ctypes.byref(conf), order, n, _p(hist),
+                                     pts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), pts.size, _p(out),
+                                     nthreads or nthreads_default(), int(exact_sum)), "rho3d_points")
     return out
 
 

@@ -130,3 +130,21 @@ def test_sweep_is_seeded_and_in_range():
     # SplitMix64 known answer for seed 0 (Vigna's reference implementation)
     rng = C.SplitMix64(0)
     assert rng.next_u64() == 0xE220A8397B1DCDAF
+
+
+@pytest.mark.parametrize("conf,prec", [
+    (lambda: nufi.config_1d(30000, 64, 1 / 16, 1.0, 0.0, 1.0, -1.0, 1.0, C.F0_LANDAU_1D), C.FP64),
+    (lambda: nufi.config_1d(16384, 64, 1 / 16, 1.0, 0.0, 1.0, -1.0, 1.0, C.F0_LANDAU_1D), C.FP32C),
+    (lambda: nufi.config_3d((25, 25, 25), (4, 4, 4), 0.1, 1.0, (0.0, 1.0), (-1.0, 1.0), C.F0_LANDAU_3D), C.FP64),
+    (lambda: nufi.config_3d((32, 32, 32), (4, 4, 4), 0.1, 1.0, (0.0, 1.0), (-1.0, 1.0), C.F0_LANDAU_3D), C.FP32C),
+])
+def test_create_refuses_grids_whose_slice_exceeds_the_lds(conf, prec):
+    """One history slice in the trace kernel's LDS layout must fit 160 KiB;
+    nufi_create refuses larger grids on the host, before touching a device."""
+    lib = _capi.load()
+    ctx = ctypes.c_void_p()
+    c = conf()
+    rc = lib.nufi_create(ctypes.byref(c), None, 4, prec, 0, ctypes.byref(ctx))
+    assert rc == _capi.ERR_UNSUPPORTED, rc
+    assert b"160 KiB" in lib.nufi_last_error(None)
+    assert not ctx.value

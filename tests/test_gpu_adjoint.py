@@ -79,6 +79,24 @@ def test_torch_autograd_wrapper():
     assert np.allclose(a.grad.numpy(), 2.0 * g, rtol=1e-12, atol=0)
 
 
+def test_adjoint_of_a_strict_context():
+    """The forward pass runs the fast kernel whatever variant the context is
+    set to (with a launch shape that kernel takes), and the variant and shape
+    are restored after it."""
+    from numericalflowiteration_amd import _capi
+    conf = small_multimode()
+    J0, g0 = A.energy_and_grad(conf, nufi.adjoint_modes())
+    with nufi.GpuScheduler(A._with_params(conf, nufi.adjoint_modes())) as s:
+        s.set_trace_variant(_capi.TRACE_STRICT)
+        shape = s.launch_shape()
+        J, g = s.adjoint_grad()
+        assert s.launch_shape() == shape
+        e = s.run()                                     # the strict kernel still runs
+    print(f"[adjoint strict context] J {J:.15e} vs {J0:.15e}", flush=True)
+    assert J == J0 and np.array_equal(g, g0)
+    assert abs(conf.dt * float(np.sum(e)) - J0) <= 1e-12 * abs(J0)
+
+
 def test_adjoint_rejects_unsupported():
     with nufi.GpuScheduler(nufi.landau_3d(N=8, V=8, T=1.0)) as s:
         with pytest.raises(nufi.NufiError):

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import numericalflowiteration_amd as nufi
+from tests import oracle_lib as O
 
 pytestmark = pytest.mark.gpu
 
@@ -93,3 +94,46 @@ def test_load_rejects_another_grid(tmp_path):
     with nufi.GpuScheduler(conf_1d()) as b:
         with pytest.raises(nufi.NufiIOError):
             b.load_history(str(tmp_path / "missing.bin"))
+
+
+def write_reference_text(path, conf, hist, n_slices, order=4):
+    """The potential file of bin/test_nufi_cpu_3d_isolated.cpp:128-135, 158-162:
+    six header lines (std::to_string values), a blank line, then one value per
+    line with std::setprecision(16) (C's %.16g)."""
+    st = conf.stride_t(order)
+    ny, nz = (conf.Ny, conf.Nz) if conf.dim == 3 else (1, 1)
+    with open(path, "w") as f:
+        f.write(f"Nt = {conf.Nt}\ndt = {conf.dt:.6f}\nNx = {conf.Nx}\nNy = {ny}\nNz = {nz}\norder = {order}\n\n")
+        f.write("".join(f"{v:.16g}\n" for v in hist[: n_slices * st]))
+
+
+@pytest.mark.parametrize("dim", [1, 3])
+def test_isolated_steps_from_a_reference_text_file_against_the_oracle(tmp_path, dim):
+    """The reference's isolated-step harness (bin/test_nufi_cpu_3d_isolated.cpp:
+    169-228): an oracle run writes its history in the reference's 16-digit text
+    layout; the file is loaded into a context and steps n are recomputed on the
+    device from slices 0..n-1 of the file, against the oracle's step on the
+    values the file holds (rho -> Poisson -> fit; energy and slice <= 1e-12).
+    Steps run from the last down so each reads only slices the file gave."""
+    conf = conf_1d() if dim == 1 else conf_3d()
+    st = conf.stride_t()
+    hist = O.new_history(conf, conf.Nt + 1)
+    O.run(conf, 0, conf.Nt + 1, hist)
+    path = str(tmp_path / "potential.txt")
+    write_reference_text(path, conf, hist, conf.Nt + 1)
+    vals = np.loadtxt(path, skiprows=6)
+    assert vals.size == (conf.Nt + 1) * st
+    file_hist = np.zeros(hist.size)
+    file_hist[: vals.size] = vals
+    with nufi.GpuScheduler(conf) as b:
+        assert b.load_history(path) == conf.Nt + 1
+        for n in (conf.Nt, 5, 1):
+            e = b.step(n)
+            got = b.history(n, n + 1)
+            h = file_hist.copy()
+            e_ref, _ = O.step(conf, n, h)
+            err_e = abs(e - e_ref) / e_ref
+            err_s = np.linalg.norm(got - h[n * st:(n + 1) * st]) / np.linalg.norm(h[n * st:(n + 1) * st])
+            print(f"[history text] dim={dim} step {n}: energy rel err {err_e:.3e}, slice rel err {err_s:.3e}",
+                  flush=True)
+            assert err_e <= 1e-12 and err_s <= 1e-12, (n, err_e, err_s)

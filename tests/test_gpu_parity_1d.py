@@ -93,8 +93,8 @@ def test_step_parity_slice_and_energy(name, prec, variant, tol):
         got = s.history(20, 21)
     ref = hist[20 * st: 21 * st]
     print(f"  slice rel err {rel(got, ref):.3e} energy rel err {abs(e - energies[20]) / energies[20]:.3e}", flush=True)
-    assert rel(got, ref) <= tol * 10
-    assert abs(e - energies[20]) <= tol * 10 * energies[20]
+    assert rel(got, ref) <= tol
+    assert abs(e - energies[20]) <= tol * energies[20]
 
 
 def test_q_subranges_accumulate_to_full_rho():
@@ -150,8 +150,10 @@ def test_non_power_of_two_grid(Nx):
         e = s.run()
         h = s.history()
     st = conf.stride_t()
-    assert rel(h, hist[: (conf.Nt + 1) * st]) < 1e-11
-    assert np.max(np.abs(e - e_ref) / e_ref) < 1e-11
+    err_h, err_e = rel(h, hist[: (conf.Nt + 1) * st]), np.max(np.abs(e - e_ref) / e_ref)
+    print(f"[non-power-of-two] Nx={Nx}: history {err_h:.3e} energy {err_e:.3e}", flush=True)
+    assert err_h <= FP64_TOL
+    assert err_e <= FP64_TOL
 
 
 @pytest.mark.parametrize("name,prec,variant,tol", VARIANTS)
@@ -164,10 +166,10 @@ def test_whole_run_energy(name, prec, variant, tol):
         e = s.run()
     err = np.max(np.abs(e - e_ref) / e_ref)
     print(f"[whole run] {name}: max energy rel err {err:.3e}", flush=True)
-    assert err <= (1e-10 if prec == nufi.FP64 else 1e-5)
+    assert err <= (FP64_TOL if prec == nufi.FP64 else 1e-5)
 
 
-@pytest.mark.parametrize("order", [3, 5, 6])
+@pytest.mark.parametrize("order", [3, 5, 6, 7, 8])
 def test_strict_other_orders(order):
     conf = nufi.landau_1d(Nx=32, Nu=128, T=1.0)
     hist = O.new_history(conf, conf.Nt + 1, order=order)
@@ -176,8 +178,10 @@ def test_strict_other_orders(order):
         e = s.run()
         h = s.history()
     st = conf.stride_t(order)
-    assert rel(h, hist[: (conf.Nt + 1) * st]) < 1e-11
-    assert np.max(np.abs(e - e_ref) / e_ref) < 1e-11
+    err_h, err_e = rel(h, hist[: (conf.Nt + 1) * st]), np.max(np.abs(e - e_ref) / e_ref)
+    print(f"[strict order {order}] history {err_h:.3e} energy {err_e:.3e}", flush=True)
+    assert err_h <= FP64_TOL
+    assert err_e <= FP64_TOL
 
 
 @pytest.mark.parametrize("conf_args", [dict(Nx=64, Nu=256, T=1.0), dict(Nx=48, Nu=200, T=1.0)])
@@ -211,6 +215,7 @@ def test_rho_bit_identical_across_launch_shapes(conf_args, prec, variant):
     (6400, nufi.FP64, _capi.TRACE_FAST, FP64_TOL),     # one buffer per chunk
     (6400, nufi.FP32C, _capi.TRACE_FP32C, FP32C_TOL),  # double-buffered, one slice per chunk
     (8192, nufi.FP32C, _capi.TRACE_FP32C, FP32C_TOL),  # two FP32 tables exceed 160 KiB: one buffer per chunk
+    (8192, nufi.FP64, _capi.TRACE_STRICT, FP64_TOL),   # one FP64 table row exceeds 160 KiB: the strict kernel
 ])
 def test_wide_grid_table_paths(Nx, prec, variant, tol):
     """The fast and FP32C kernels on wide grids, where the kick-polynomial
@@ -225,6 +230,9 @@ def test_wide_grid_table_paths(Nx, prec, variant, tol):
                            + 0.01 * np.cos(2 * np.pi * 7 * j / Nx + m) for m in range(4)])
     print(f"[wide grid] Nx={Nx} {prec}", flush=True)
     with nufi.GpuScheduler(conf, precision=prec) as s:
+        if variant == _capi.TRACE_STRICT:                  # the context chose it: FAST cannot stage a row
+            with pytest.raises(nufi.NufiError):
+                s.set_trace_variant(_capi.TRACE_FAST)
         s.set_trace_variant(variant)
         s.upload_history(hist, 0, 4)
         for n in (1, 3, 4):

@@ -79,8 +79,8 @@ def test_step_parity_3d(name, prec, variant, tol):
     ref = hist[5 * st: 6 * st]
     print(f"[3d step] {name}: slice {rel(got, ref):.3e} energy {abs(e - energies[5]) / energies[5]:.3e}",
           flush=True)
-    assert rel(got, ref) <= tol * 10
-    assert abs(e - energies[5]) <= tol * 10 * energies[5]
+    assert rel(got, ref) <= tol
+    assert abs(e - energies[5]) <= tol * energies[5]
 
 
 def test_poisson_fit_3d_against_oracle():

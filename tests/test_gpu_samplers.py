@@ -30,7 +30,7 @@ def points(conf, count=300, seed=5):
     return x, v
 
 
-@pytest.mark.parametrize("order", [4, 3, 5])
+@pytest.mark.parametrize("order", [4, 3, 5, 6, 7, 8])
 def test_sample_field_against_oracle(order):
     """A point whose wrapped position rounds to exactly Lx (knot Nx; here
     x = -1e-17) makes the reference read
@@ -58,20 +58,23 @@ def test_sample_field_against_oracle(order):
             assert eE <= 1e-14 and eR <= 1e-14
 
 
-def test_sample_f_and_phase_flow_against_oracle():
-    conf, hist = setup()
+@pytest.mark.parametrize("order", [4, 7, 8])
+def test_sample_f_and_phase_flow_against_oracle(order):
+    conf, hist = setup(order)
     x, v = points(conf, 200)
-    with nufi.GpuScheduler(conf) as s:
+    with nufi.GpuScheduler(conf, order=order) as s:
         s.upload_history(hist, 0, conf.Nt + 1)
         for n in [0, 1, 2, 5, conf.Nt]:
             f = s.sample_f(n, x, v)
-            f_ref = np.array([O.trace1d(conf, n, xi, vi, hist, half_kick=True) for xi, vi in zip(x, v)])
+            f_ref = np.array([O.trace1d(conf, n, xi, vi, hist, half_kick=True, order=order)
+                              for xi, vi in zip(x, v)])
             ef = np.max(np.abs(f - f_ref) / np.maximum(np.abs(f_ref), 1e-300))
             xo, vo = s.phase_flow(n, x, v)
-            ref = np.array([O.phase_flow1d(conf, n, xi, vi, hist) for xi, vi in zip(x, v)])
+            ref = np.array([O.phase_flow1d(conf, n, xi, vi, hist, order=order) for xi, vi in zip(x, v)])
             ex = np.max(np.abs(xo - ref[:, 0])) / conf.Lx
             ev = np.max(np.abs(vo - ref[:, 1])) / np.max(np.abs(ref[:, 1]))
-            print(f"[samplers] n={n}: f err {ef:.2e}, flow x err {ex:.2e}, v err {ev:.2e}", flush=True)
+            print(f"[samplers] order={order} n={n}: f err {ef:.2e}, flow x err {ex:.2e}, v err {ev:.2e}",
+                  flush=True)
             assert ef <= 1e-12 and ex <= 1e-12 and ev <= 1e-12
             # x - Lx floor(x / Lx) gives exactly Lx when x / Lx rounds just below 1
             # (here x = x_min + Lx); the reference's wrap has that edge too
