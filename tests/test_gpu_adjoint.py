@@ -92,8 +92,11 @@ def test_adjoint_of_a_strict_context():
         J, g = s.adjoint_grad()
         assert s.launch_shape() == shape
         e = s.run()                                     # the strict kernel still runs
-    print(f"[adjoint strict context] J {J:.15e} vs {J0:.15e}", flush=True)
-    assert J == J0 and np.array_equal(g, g0)
+    print(f"[adjoint strict context] J {J:.15e} vs {J0:.15e}; grad max rel diff "
+          f"{np.max(np.abs(g - g0) / np.abs(g0)):.3e}", flush=True)
+    # the forward run is deterministic; the slice adjoints are summed with
+    # atomics, so the gradient agrees to rounding, not bit for bit
+    assert J == J0 and np.allclose(g, g0, rtol=1e-12, atol=0)
     assert abs(conf.dt * float(np.sum(e)) - J0) <= 1e-12 * abs(J0)
 
 
