@@ -74,7 +74,10 @@ int balloc( nufi_batch *b, T **p, size_t count, const char *what )
     if ( count == 0 ) count = 1;
     hipError_t e = hipMalloc( reinterpret_cast<void**>( p ), count * sizeof(T) );
     if ( e != hipSuccess ) return bhip( b, e, what );
-    e = hipMemset( *p, 0, count * sizeof(T) );
+    // zeroed and waited for (a null-stream memset is not ordered before the
+    // work of the batch's non-blocking stream)
+    e = hipMemsetAsync( *p, 0, count * sizeof(T), b->stream );
+    if ( e == hipSuccess ) e = hipStreamSynchronize( b->stream );
     if ( e != hipSuccess ) return bhip( b, e, what );
     return NUFI_OK;
 }

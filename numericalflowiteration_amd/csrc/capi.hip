@@ -104,7 +104,11 @@ int dev_alloc( nufi_ctx *c, T **p, size_t count, const char *what )
     if ( count == 0 ) count = 1;
     hipError_t e = hipMalloc( reinterpret_cast<void**>( p ), count * sizeof(T) );
     if ( e != hipSuccess ) return hip_fail( c, e, what );
-    e = hipMemset( *p, 0, count * sizeof(T) );
+    // zeroed and waited for: a plain hipMemset goes to the null stream, which
+    // the context's non-blocking stream does not wait for, so a late memset
+    // could clear what the first kernels on the context stream write
+    e = hipMemsetAsync( *p, 0, count * sizeof(T), c->stream );
+    if ( e == hipSuccess ) e = hipStreamSynchronize( c->stream );
     if ( e != hipSuccess ) return hip_fail( c, e, what );
     return NUFI_OK;
 }
@@ -311,15 +315,7 @@ int launch_trace( nufi_ctx *c, uint64_t n, uint64_t q_begin, uint64_t q_end, uin
         p.l_first = uint32_t( l0 ); p.n_tasks = uint32_t( tasks );
         p.n_count = uint32_t( l1 - l0 + 1 ); p.vgroup = c->vgroup; p.ppx = c->ppx;
         p.order = uint32_t( c->order );
-        const nufi_config &g = c->conf;
-        p.x_min = g.x_min; p.y_min = g.y_min; p.z_min = g.z_min;
-        p.dx = g.dx; p.dy = g.dy; p.dz = g.dz;
-        p.dx_inv = g.dx_inv; p.dy_inv = g.dy_inv; p.dz_inv = g.dz_inv;
-        p.Lx = g.Lx; p.Ly = g.Ly; p.Lz = g.Lz;
-        p.Lx_inv = g.Lx_inv; p.Ly_inv = g.Ly_inv; p.Lz_inv = g.Lz_inv;
-        p.u_min = g.u_min; p.v_min = g.v_min; p.w_min = g.w_min;
-        p.du = g.du; p.dv = g.dv; p.dw = g.dw; p.dt = g.dt;
-        p.f0 = g.f0;
+        fill_trace3_geometry( p, c->conf );
         e = launch_trace3d( p, c->variant, c->shape, c->stream );
     }
     if ( e != hipSuccess ) return hip_fail( c, e, "trace kernel launch" );
@@ -362,6 +358,10 @@ field_params base_field( nufi_ctx *c )
 }
 
 } // namespace
+
+int nufi_impl::validate_config( const nufi_config *c, int order ) { return validate( c, order ); }
+
+void nufi_impl::set_context_free_error( const std::string &msg ) { g_create_error = msg; }
 
 extern "C" {
 

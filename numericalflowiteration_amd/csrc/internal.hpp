@@ -121,6 +121,24 @@ struct trace3_params
     nufi_f0 f0;
 };
 
+// The field and velocity grid of a 3D trace launch from a config.
+inline void fill_trace3_geometry( trace3_params &p, const nufi_config &g )
+{
+    p.x_min = g.x_min; p.y_min = g.y_min; p.z_min = g.z_min;
+    p.dx = g.dx; p.dy = g.dy; p.dz = g.dz;
+    p.dx_inv = g.dx_inv; p.dy_inv = g.dy_inv; p.dz_inv = g.dz_inv;
+    p.Lx = g.Lx; p.Ly = g.Ly; p.Lz = g.Lz;
+    p.Lx_inv = g.Lx_inv; p.Ly_inv = g.Ly_inv; p.Lz_inv = g.Lz_inv;
+    p.u_min = g.u_min; p.v_min = g.v_min; p.w_min = g.w_min;
+    p.du = g.du; p.dv = g.dv; p.dw = g.dw; p.dt = g.dt;
+    p.f0 = g.f0;
+}
+
+// NUFI_OK if the config and order describe a grid the library accepts (capi.hip).
+int validate_config( const nufi_config *c, int order );
+// The message nufi_last_error(NULL) returns on this thread (capi.hip).
+void set_context_free_error( const std::string &msg );
+
 // One run of a batched sweep (device-resident parameter block).
 struct batch_run
 {
