@@ -232,6 +232,28 @@ int nufi_trace_timing(nufi_ctx *ctx, uint64_t *launches, double *kernel_ms,
  */
 int nufi_adjoint_grad(nufi_ctx *ctx, double *dJ_da, int n_par, double *J_out);
 
+/* ---- rho without allocation ------------------------------------------- */
+/*
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): rho of time step n of a 3D order-4 FP64 grid, from slices
+ * 0..n-1 of a device history the caller owns (the reference layout: slice m
+ * at hist_dev + m*stride_t, stride_t = (Nx+3)(Ny+3)(Nz+3); hist_dev 16-byte
+ * aligned, hist_len >= n*stride_t + 1 reals) into rho_dev (Nx*Ny*Nz reals,
+ * device, overwritten with -du dv dw sum_v f: what nufi_compute_rho over all
+ * nodes then nufi_download_rho into zeros gives, bit for bit).  Asynchronous
+ * on `stream` (a hipStream_t; NULL is the default stream) of the current
+ * device.  Allocates nothing: the partial sums live in one fixed device array
+ * of NUFI_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; grids
+ * whose partials do not fit it (ceil(Nvel/vgroup)*Nx*Ny*Nz reals, vgroup = 256
+ * velocities per partial at 16^3 x 32^3) give NUFI_ERR_UNSUPPORTED.  Every call
+ * on one GPU shares that array: calls that overlap on one GPU race.  Errors
+ * are reported by nufi_last_error(NULL).
+ */
+#define NUFI_NOALLOC_SCRATCH_BYTES ( 8u << 20 )
+int nufi_rho3d_noalloc(const nufi_config *conf, uint64_t n, const double *hist_dev, uint64_t hist_len,
+                       double *rho_dev, void *stream);
+
 /* ---- batched sweep: many independent 1D runs on one device --------------- */
 /*
  * No counterpart in the reference (which runs one configuration per

@@ -74,6 +74,7 @@ SIGNATURES = {
     "nufi_set_timing": (_int, [_vp, _int]),
     "nufi_trace_timing": (_int, [_vp, ctypes.POINTER(_u64), _dp, _dp]),
     "nufi_adjoint_grad": (_int, [_vp, _dp, _int, _dp]),
+    "nufi_rho3d_noalloc": (_int, [_cp, _u64, _vp, _u64, _vp, _vp]),
     "nufi_batch_create": (_int, [_cp, _u64, _int, _int, _int, ctypes.POINTER(_vp)]),
     "nufi_batch_destroy": (_int, [_vp]),
     "nufi_batch_last_error": (ctypes.c_char_p, [_vp]),
