@@ -1,25 +1,34 @@
 #!/usr/bin/env python3
 """Benchmark of the NuFI rho path on one MI355X.
 
-Workload (one "step" of this benchmark = one whole run to time T, i.e. every
-time step 0..Nt of the device-resident loop: trace kernel + field kernel per
-time step): 1D two-stream instability, x in [0,10pi), f0 = (1+0.001cos(0.2x))
-v^2 exp(-v^2/2)/sqrt(2pi), v in [-8,8], Nx=256 order-4 splines, Nv=1024,
-dt=1/32, T=50 (1600 steps).  It runs in both forms: the plain FP64 method and
-the FP32-compensated trace; the reported `value` is the FP64 form's
-characteristic steps per second end to end (time-to-T), the FP32C form and
-its electric-energy error against the FP64 run are reported beside it.
-The 3D3V Landau case (16^3 x 32^3, dt=1/8, T=20) runs once per form as well
-(--no-3d skips it), and so do the 512-run batched sweep and the adjoint case
-(--no-extra skips them); their figures are reported in `forms`.
+Headline workload: 3D3V linear Landau damping, the plain FP64 method:
+x in [0,4pi)^3, f0 = (1 + 0.01 sum_d cos(0.5 x_d)) exp(-|v|^2/2)/(2pi)^(3/2),
+v in [-6,6]^3, 16^3 order-4 tensor B-splines, 32^3 velocity nodes per point,
+dt = 1/8, T = 20 (161 time steps, 1.73e12 characteristic steps per run).
 
-Prints ONE JSON line (rank 0, N = 1).
+One bench step is one device time step of that run (the trace of all 1.34e8
+nodes through the stored field history, the rho partial sums, the Poisson
+solve and spline fit, the slice stored) at n = 80, the step whose cost is the
+run's mean (sum_n n / 161 = 80), recomputed from the run's own history (the
+isolated-step pattern of bin/test_nufi_cpu_3d_isolated.cpp:169-228): W untimed
+steps, then K timed; `value` is the char-steps per second of the K steps.
+Beside it, per form (FP64 and the FP32-compensated trace): time-to-T of the
+whole run, the FP32C errors against the FP64 run (maximum over the time steps
+of the normwise relative error of the mean-free rho, of E at the nodes and of
+the electric energy), the trace kernel's roofline from HIP events on the
+context stream, the CPU baseline (the oracle on isolated steps at a sample of
+spatial points, extrapolated over the run) and parity against the oracle at
+full size.  The other workloads (1D two-stream, 1D Landau, the 512-run batched
+sweep, the adjoint) are reported under `forms` with their own CPU baselines.
+
+Prints ONE JSON line (N = 1).
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -33,6 +42,7 @@ PEAK_FP64 = 78.6     # TFLOP/s, MI355X FP64 vector
 PEAK_FP32 = 157.3    # TFLOP/s, MI355X FP32 vector
 CUS, CLOCK_GHZ = 256, 2.4
 LDS_PEAK_TBS = 128 * CUS * CLOCK_GHZ * 1e9 / 1e12   # 128 B per clock per CU
+PROFILE_KERNEL = "trace"                              # summarize_profiles: the kernel whose traffic is kept
 
 
 def progress(msg):
@@ -40,8 +50,32 @@ def progress(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
-def sync_all(sched):
-    sched.synchronize()
+def cpu_threads():
+    """Threads for the CPU baseline: the physical cores lscpu reports, capped by
+    this process's CPU affinity and OMP_NUM_THREADS; returns (threads, physical
+    cores, CPUs in the affinity mask)."""
+    phys = None
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, check=True).stdout
+        phys = len({ln for ln in out.splitlines() if ln and not ln.startswith("#")}) or None
+    except Exception:
+        pass
+    avail = len(os.sched_getaffinity(0))
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(v for v in (phys, avail, env) if v)
+    return threads, phys, avail
+
+
+def mean_free(a):
+    return a - a.mean()
+
+
+def rel(a, b):
+    import numpy as np
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def sync():
     try:
         import torch
         if torch.cuda.is_available():
@@ -50,94 +84,229 @@ def sync_all(sched):
         pass
 
 
-def timed_runs(conf, precision, steps, warmup, flops_per_cs, tune_n=None, shape=None, keep_history=False):
+# ---- 3D3V Landau (headline) ---------------------------------------------------
+
+def landau3d_form(conf, precision, steps, warmup, autotune):
+    """One context: (optionally autotuned) whole run to T (time-to-T), then the
+    bench steps at n = Nt/2 with every trace launch of the context timed by HIP
+    events on its stream.  Returns (figures, energies, history, scheduler)."""
+    import numericalflowiteration_amd as nufi
+    n_mid = conf.Nt // 2
+    s = nufi.GpuScheduler(conf, precision=precision)
+    tuned = None
+    if autotune:
+        s.run(0, n_mid + 1)
+        tuned = s.autotune(n_mid, repeats=2)
+    s.set_timing(True)            # every trace launch from here on: the whole run, the bench steps
+    s.synchronize()
+    sync()
+    t0 = time.perf_counter()
+    energies = s.run()
+    s.synchronize()
+    t_run = time.perf_counter() - t0
+    history = s.history()
+    for _ in range(warmup):
+        s.step(n_mid, wait=False)
+    s.synchronize()
+    sync()                        # device-wide synchronisation on both sides of the timed steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step(n_mid, wait=False)
+    s.synchronize()
+    sync()
+    t_steps = time.perf_counter() - t0
+    cs_run = conf.char_steps(0, conf.Nt + 1)
+    cs_step = conf.n_nodes() * n_mid
+    r = {"time_to_T_s": t_run, "cs_per_s_time_to_T": cs_run / t_run, "char_steps_per_run": cs_run,
+         "bench_step_n": n_mid, "char_steps_per_step": cs_step, "ms_per_step": t_steps / max(steps, 1) * 1e3,
+         "cs_per_s_steps": cs_step * steps / t_steps if steps else None,
+         "launch_shape": list(s.launch_shape()),
+         "autotune_probe": None if tuned is None else {"n": n_mid, "best_ms": tuned[1]}}
+    return r, energies, history, s
+
+
+def trace_roofline(s, flops_per_cs, kernel, peak):
+    """The trace kernel over every launch timed on the context (HIP events on its
+    stream): algorithmic FLOPs per launch over the mean launch duration."""
+    launches, kernel_ms, csteps = s.trace_timing()
+    flops = flops_per_cs * csteps / launches
+    avg_s = kernel_ms / launches / 1e3
+    return {"kernel": kernel, "launches": launches, "avg_launch_ms": avg_s * 1e3,
+            "flops_per_launch": flops, "achieved": flops / avg_s / 1e12, "peak": peak,
+            "frac": flops / avg_s / 1e12 / peak, "cs_per_s_kernel": csteps / (kernel_ms / 1e3)}
+
+
+def cpu_landau3d(conf, hist, s_gpu, threads, budget_points=48):
+    """The oracle (the reference's dim3 eval_rho, rho.hpp:428-462, OpenMP over the
+    sampled points) on isolated steps n = 1, 40, 80, 120, 160 of the GPU FP64
+    history at `budget_points` spatial points each; char-steps/s per step, the
+    rate interpolated over n for the CPU time-to-T.  The same points give the
+    full-size parity of the GPU's rho (reference summation order, and the
+    exact v-sum at 16 points of steps 80 and 160)."""
+    import numpy as np
+    from tests import oracle_lib as O
+    st = conf.stride_t()
+    h = np.zeros(hist.size + 8)
+    h[: hist.size] = hist
+    rng = np.random.default_rng(7)
+    pts = np.concatenate([[0, conf.n_space() - 1],
+                          rng.choice(np.arange(1, conf.n_space() - 1), budget_points - 2, replace=False)])
+    nvel = conf.n_vel()
+    steps = [1, conf.Nt // 4, conf.Nt // 2, 3 * conf.Nt // 4, conf.Nt]
+    rates, par_ref, par_exact = {}, {}, {}
+    total_cs, total_t = 0, 0.0
+    for n in steps:
+        rho = np.zeros(conf.n_space())
+        s_gpu.compute_rho(n)
+        s_gpu.download_rho(rho)
+        t = time.perf_counter()
+        ref = O.rho3d_points(conf, n, h, pts, nthreads=threads)
+        dt = time.perf_counter() - t
+        rates[n] = pts.size * nvel * n / dt
+        total_cs += pts.size * nvel * n
+        total_t += dt
+        scale = np.linalg.norm(mean_free(rho)[pts])
+        par_ref[n] = float(np.linalg.norm((rho[pts] + 1.0) - ref) / scale)
+        if n in (conf.Nt // 2, conf.Nt):
+            ex = O.rho3d_points(conf, n, h, pts[:16], nthreads=threads, exact_sum=True)
+            par_exact[n] = float(np.linalg.norm((rho[pts[:16]] + 1.0) - ex) / np.linalg.norm(mean_free(rho)[pts[:16]]))
+    ns = np.array(steps, dtype=float)
+    rs = np.array([rates[n] for n in steps])
+    t_T = sum(conf.n_nodes() * n / float(np.interp(n, ns, rs)) for n in range(1, conf.Nt + 1))
+    cpu = {"value": conf.char_steps(0, conf.Nt + 1) / t_T, "unit": "char-steps/s", "cores": threads, "kind": "port",
+           "time_to_T_s_extrapolated": t_T,
+           "sample": f"the oracle's rho (reference summation order) of isolated time steps n={steps} of the 3D "
+                     f"Landau run at {pts.size} of the {conf.n_space()} spatial points (all {nvel} velocities each), "
+                     f"from the GPU FP64 history; {total_cs:.3e} char-steps in {total_t:.1f} s; rate per step "
+                     f"interpolated over n = 1..{conf.Nt} for the time-to-T (Poisson and fit excluded: O(N) per step)",
+           "rate_by_step": {str(n): rates[n] for n in steps}}
+    parity = {"rho_rel_err_vs_oracle_reference_sum": par_ref, "rho_rel_err_vs_oracle_exact_sum_16pts": par_exact,
+              "points": int(pts.size), "note": "GPU FP64 rho(n) on the run's history vs the oracle at the sampled "
+                                               "points, scaled by the mean-free GPU rho there"}
+    return cpu, parity
+
+
+# ---- 1D two-stream ---------------------------------------------------------------
+
+def timed_runs(conf, precision, runs, warmup, tune_n=None):
+    """`runs` whole runs to T after `warmup` untimed ones (autotuned at tune_n)."""
     import numericalflowiteration_amd as nufi
     with nufi.GpuScheduler(conf, precision=precision) as s:
-        if shape is not None:
-            s.set_launch_shape(*shape)
         for _ in range(warmup):
             s.run()
-        tuned = None
-        if tune_n is not None:          # untimed: pick the launch shape on this history
-            if warmup == 0:             # no run yet: compute the slices the probe step traces through
+        if tune_n is not None:
+            if warmup == 0:
                 s.run(0, tune_n)
-            tuned = s.autotune(tune_n, repeats=2)
-        sync_all(s)
+            s.autotune(tune_n, repeats=2)
         s.set_timing(True)
+        s.synchronize()
         t0 = time.perf_counter()
         energies = None
-        for _ in range(steps):
+        for _ in range(runs):
             energies = s.run()
-        sync_all(s)
+        s.synchronize()
         wall = time.perf_counter() - t0
-        launches, kernel_ms, csteps = s.trace_timing()
+        roof = trace_roofline(s, FLOPS_1D, "trace1d", PEAK_FP64)
+        history = s.history()
         shape = s.launch_shape()
-        history = s.history() if keep_history else None
     cs_run = conf.char_steps(0, conf.Nt + 1)
-    achieved = flops_per_cs * csteps / launches / (kernel_ms / 1e3 / launches) / 1e12   # TFLOP/s
-    return dict(
-        wall_s=wall, ms_per_run=wall / steps * 1e3, char_steps_per_run=cs_run,
-        cs_per_s_e2e=cs_run * steps / wall,
-        cs_per_s_kernel=csteps / (kernel_ms / 1e3),
-        trace_launches=launches, trace_avg_ms=kernel_ms / launches,
-        trace_flops_per_launch=flops_per_cs * csteps / launches,
-        achieved_tflops=achieved, launch_shape=list(shape),
-        autotune_probe=None if tuned is None else {"n": tune_n, "best_ms": tuned[1]},
-    ), energies, history
+    return {"runs": runs, "ms_per_run": wall / runs * 1e3, "char_steps_per_run": cs_run,
+            "cs_per_s_time_to_T": cs_run * runs / wall, "cs_per_s_kernel": roof["cs_per_s_kernel"],
+            "trace_avg_ms": roof["avg_launch_ms"], "achieved_tflops": roof["achieved"],
+            "launch_shape": list(shape)}, energies, history
 
 
-def landau3d_forms(args):
-    """The 3D3V Landau case, one run to T per form (FP64, then FP32C)."""
+def twostream(args, threads, with_cpu):
     import numpy as np
     import numericalflowiteration_amd as nufi
-    c5 = nufi.landau_3d()
-    t3 = None if args.no_autotune else c5.Nt // 2     # a mid-run step: short probes pick chunks that lose later
-    r64, f64, h64 = timed_runs(c5, nufi.FP64, 1, 0, FLOPS_3D, t3, keep_history=True)
-    progress(f"3D3V Landau FP64 {r64['ms_per_run'] / 1e3:.1f} s per run")
-    r32, f32, h32 = timed_runs(c5, nufi.FP32C, 1, 0, FLOPS_3D, t3, keep_history=True)
-    progress(f"3D3V Landau FP32C {r32['ms_per_run'] / 1e3:.1f} s per run")
-    r32.update(energy_errors(f32, f64))
-    r32.update(field_errors(h32, h64, c5.stride_t()))
-    r32["speedup_vs_fp64"] = r32["cs_per_s_e2e"] / r64["cs_per_s_e2e"]
-    if args.dump_energies:
-        np.save(os.path.join(args.dump_energies, "landau3d_energy_fp64.npy"), f64)
-        np.save(os.path.join(args.dump_energies, "landau3d_energy_fp32c.npy"), f32)
-    return {"fp64": r64, "fp32c": r32,
-            "workload": "3D3V Landau, x in [0,4pi)^3, (1+0.01 sum cos(0.5x_d)) M3(v), "
-                        "v in [-6,6]^3, 16^3 x 32^3 nodes, dt=1/8, T=20 (161 steps)"}
+    from numericalflowiteration_amd import diagnostics as D
+    conf = nufi.twostream_1d()
+    tune = None if args.no_autotune else conf.Nt
+    r64, e64, h64 = timed_runs(conf, nufi.FP64, args.runs_1d, 1, tune)
+    r32, e32, h32 = timed_runs(conf, nufi.FP32C, args.runs_1d, 1, tune)
+    r32.update(D.run_errors(conf, h32, e32, h64, e64))
+    r32["speedup_vs_fp64"] = r32["cs_per_s_time_to_T"] / r64["cs_per_s_time_to_T"]
+    out = {"workload": "1D two-stream, x in [0,10pi), (1+0.001cos(0.2x)) v^2 M(v), v in [-8,8], Nx=256 order 4, "
+                       "Nv=1024, dt=1/32, T=50 (1601 steps)", "fp64": r64, "fp32c": r32}
+    progress(f"two-stream FP64 {r64['ms_per_run']:.1f} ms, FP32C {r32['ms_per_run']:.1f} ms per run")
+    if with_cpu:
+        from tests import oracle_lib as O
+        st = conf.stride_t()
+        hist = np.zeros(h64.size + 8)
+        hist[: h64.size] = h64
+        par, total_cs, total_t, done = {}, 0, 0.0, []
+        with nufi.GpuScheduler(conf) as s:
+            s.upload_history(h64[: (conf.Nt + 1) * st], 0, conf.Nt + 1)
+            for n in range(conf.Nt, 0, -50):
+                rho = np.zeros(conf.Nx)
+                s.compute_rho(n)
+                s.download_rho(rho)
+                t = time.perf_counter()
+                ref = O.rho(conf, n, hist, nthreads=threads)
+                dt = time.perf_counter() - t
+                total_cs += conf.n_nodes() * n
+                total_t += dt
+                done.append(n)
+                par[n] = rel(mean_free(rho + 1.0), mean_free(ref))
+                if total_t > 12.0:
+                    break
+            exact = {}
+            for n in (conf.Nt, conf.Nt // 2, 1):
+                rho = np.zeros(conf.Nx)
+                s.compute_rho(n)
+                s.download_rho(rho)
+                exact[n] = rel(mean_free(rho), mean_free(O.rho(conf, n, hist, nthreads=threads, exact_sum=True)))
+        rate = total_cs / total_t
+        out["cpu_baseline"] = {"value": rate, "unit": "char-steps/s", "cores": threads, "kind": "port",
+                               "time_to_T_s_extrapolated": conf.char_steps(0, conf.Nt + 1) / rate,
+                               "sample": f"the oracle's rho of isolated steps n={done} of the GPU FP64 history "
+                                         f"(all 262144 nodes each); {total_cs:.3e} char-steps in {total_t:.1f} s"}
+        out["parity_vs_oracle"] = {"rho_rel_err_reference_sum": {str(k): v for k, v in par.items()},
+                                   "rho_rel_err_exact_sum": {str(k): v for k, v in exact.items()},
+                                   "note": "mean-free rho, GPU FP64 vs the oracle on the same history"}
+        progress("two-stream CPU baseline done")
+    return out
 
 
-def field_errors(h, ref, stride):
-    """Normwise relative error of the E-field spline coefficients (the history
-    slices) of a run against the FP64 run, maximum over the time steps."""
+# ---- 1D Landau ----------------------------------------------------------------------
+
+def landau1d(args, threads, with_cpu):
     import numpy as np
-    a, b = h.reshape(-1, stride), ref.reshape(-1, stride)
-    errs = np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
-    return {"field_coeff_max_rel_err_vs_fp64": float(errs.max()), "field_coeff_err_at_step": int(errs.argmax())}
+    import numericalflowiteration_amd as nufi
+    from tests import oracle_lib as O
+    conf = nufi.landau_1d()
+    r64, e64, _ = timed_runs(conf, nufi.FP64, max(args.runs_1d, 5), 1, None if args.no_autotune else conf.Nt)
+    out = {"workload": "1D Landau, x in [0,4pi), (1+0.01cos(0.5x)) M(v), v in [-10,10], Nx=64 order 4, Nv=256, "
+                       "dt=1/16, T=30 (481 steps)", "fp64": r64}
+    progress(f"1D Landau FP64 {r64['ms_per_run']:.2f} ms per run")
+    if with_cpu:
+        h = O.new_history(conf, conf.Nt + 1)
+        t = time.perf_counter()
+        e = O.run(conf, 0, conf.Nt + 1, h, nthreads=threads)
+        dt = time.perf_counter() - t
+        cs = conf.char_steps(0, conf.Nt + 1)
+        out["cpu_baseline"] = {"value": cs / dt, "unit": "char-steps/s", "cores": threads, "kind": "port",
+                               "time_to_T_s": dt, "sample": "the whole run to T on the CPU (the oracle: rho, "
+                                                            "DHT Poisson, circulant fit per step)"}
+        out["parity_vs_oracle"] = {"energy_max_rel_err": float(np.max(np.abs(e64 - e) / np.abs(e))),
+                                   "note": "energies of the whole GPU FP64 run vs the oracle's run"}
+        progress("1D Landau CPU baseline done")
+    return out
 
 
-def energy_errors(e, ref):
-    """Electric-energy error of a run against the FP64 run: the pointwise maximum
-    of |e-ref|/ref, and the time-normwise ||e-ref||_2/||ref||_2.  The pointwise
-    figure is ill-conditioned where the energy of an oscillating mode passes near
-    zero (Landau damping); both are reported."""
-    import numpy as np
-    d = np.abs(e - ref)
-    return {"energy_max_rel_err_vs_fp64": float(np.max(d / np.abs(ref))),
-            "energy_normwise_rel_err_vs_fp64": float(np.linalg.norm(e - ref) / np.linalg.norm(ref)),
-            "energy_err_at_max_pointwise": int(np.argmax(d / np.abs(ref)))}
+# ---- batched sweep -------------------------------------------------------------------
 
-
-def batch_sweep():
+def batch_sweep(args, threads, with_cpu):
     """The 512-run sweep (alpha~U[0.001,0.5], k~U[0.3,0.6], L=2pi/k, Nx in {32..256},
     Nv=4Nx, v in [-10,10], Maxwellian or two-stream, dt=1/16, T=20) as one batch:
     shared launches per step, hipGraph replay, autotuned shape; FP32C and FP64."""
     import numpy as np
     import numericalflowiteration_amd as nufi
+    from numericalflowiteration_amd import diagnostics as D
     confs = [nufi.sweep_config(r) for r in nufi.sweep_runs()]
     res = {"workload": "512 independent 1D runs (seeded sweep), dt=1/16, T=20, one batch",
            "runs": len(confs), "nx_histogram": {str(n): sum(c.Nx == n for c in confs) for n in (32, 64, 128, 256)}}
-    energies = {}
+    energies, hists = {}, {}
     for name, prec in (("fp32c", nufi.FP32C), ("fp64", nufi.FP64)):
         with nufi.BatchScheduler(confs, precision=prec) as b:
             b.run(graph=True)                          # warm-up, captures the graph
@@ -151,14 +320,41 @@ def batch_sweep():
             wall = time.perf_counter() - t0
             cs = b.char_steps(0, max(c.Nt for c in confs) + 1)
             energies[name] = [b.energies(r) for r in range(len(confs))]
-        res[name] = {"wall_s": wall, "char_steps": cs, "cs_per_s_e2e": cs / wall, "launch_shape": list(shape)}
-    errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(energies["fp32c"], energies["fp64"])]
-    res["fp32c"]["energy_normwise_rel_err_vs_fp64_max_over_runs"] = max(errs)
-    res["fp32c"]["speedup_vs_fp64"] = res["fp32c"]["cs_per_s_e2e"] / res["fp64"]["cs_per_s_e2e"]
+            hists[name] = [b.history(r) for r in range(len(confs))]
+        res[name] = {"wall_s": wall, "char_steps": cs, "cs_per_s_time_to_T": cs / wall, "launch_shape": list(shape)}
+    errs = [D.run_errors(c, h32, e32, h64, e64) for c, h32, e32, h64, e64 in
+            zip(confs, hists["fp32c"], energies["fp32c"], hists["fp64"], energies["fp64"])]
+    for key in ("rho_max_rel_err", "E_max_rel_err", "energy_max_rel_err", "energy_normwise_rel_err"):
+        res["fp32c"][key + "_max_over_runs"] = max(e[key] for e in errs)
+    res["fp32c"]["speedup_vs_fp64"] = res["fp32c"]["cs_per_s_time_to_T"] / res["fp64"]["cs_per_s_time_to_T"]
+    progress("batched sweep done")
+    if with_cpu:
+        from tests import oracle_lib as O
+        picks = [next(i for i, c in enumerate(confs) if c.Nx == nx) for nx in (32, 64, 128, 256)]
+        total_cs, total_t = 0, 0.0
+        for i in picks:
+            c = confs[i]
+            h = np.zeros(hists["fp64"][i].size + 8)
+            h[: hists["fp64"][i].size] = hists["fp64"][i]
+            t = time.perf_counter()
+            O.rho(c, c.Nt, h, nthreads=threads)
+            total_t += time.perf_counter() - t
+            total_cs += c.n_nodes() * c.Nt
+        rate = total_cs / total_t
+        cs_all = res["fp64"]["char_steps"]
+        res["cpu_baseline"] = {"value": rate, "unit": "char-steps/s", "cores": threads, "kind": "port",
+                               "time_to_T_s_extrapolated": cs_all / rate,
+                               "sample": f"the oracle's rho of the last step (n=320) of the first run with each Nx "
+                                         f"(runs {picks}), serially, from the GPU FP64 histories; "
+                                         f"{total_cs:.3e} char-steps in {total_t:.1f} s; scaled by the batch's "
+                                         f"char-steps"}
+        progress("batch CPU baseline done")
     return res
 
 
-def adjoint_case():
+# ---- adjoint -------------------------------------------------------------------------
+
+def adjoint_case(threads, with_cpu):
     """Gradient of J = sum_n dt*energy_n w.r.t. the 8 multimode amplitudes (Nx=128,
     Nv=512, x in [0,4pi), v in [-10,10], dt=1/16, T=20) by the device adjoint,
     checked against central differences (h=1e-6) of the device forward model."""
@@ -167,24 +363,45 @@ def adjoint_case():
     from numericalflowiteration_amd import adjoint as A
     a0 = nufi.adjoint_modes()
     conf = nufi.multimode_1d(a0)
-    A.energy_and_grad(conf, a0)                           # warm-up
-    t0 = time.perf_counter()
-    J, g = A.energy_and_grad(conf, a0)
-    wall = time.perf_counter() - t0
-    h, fd = 1e-6, []
-    for m in range(len(a0)):
-        ap, am = list(a0), list(a0)
-        ap[m] += h
-        am[m] -= h
-        Jp = A.energy_and_grad(conf, ap)[0]
-        Jm = A.energy_and_grad(conf, am)[0]
-        fd.append((Jp - Jm) / (2 * h))
-    fd = np.array(fd)
-    return {"workload": "adjoint: 1D multimode (8 amplitudes), Nx=128, Nv=512, dt=1/16, T=20 (321 steps)",
-            "J": J, "grad": g.tolist(), "fd_device_h1e-6": fd.tolist(),
-            "max_rel_err_vs_fd": float(np.max(np.abs(g - fd)) / np.max(np.abs(fd))),
-            "forward_plus_adjoint_s": wall,
-            "forward_char_steps": conf.char_steps(0, conf.Nt + 1)}
+    out = {"workload": "adjoint: 1D multimode (8 amplitudes), Nx=128, Nv=512, dt=1/16, T=20 (321 steps)",
+           "forward_char_steps": conf.char_steps(0, conf.Nt + 1)}
+    for name, prec in (("fp64", nufi.FP64), ("fp32c", nufi.FP32C)):
+        try:
+            A.energy_and_grad(conf, a0, precision=prec)                      # warm-up
+        except nufi.NufiError as e:
+            out[name] = {"error": str(e)}
+            continue
+        t0 = time.perf_counter()
+        J, g = A.energy_and_grad(conf, a0, precision=prec)
+        wall = time.perf_counter() - t0
+        h, fd = 1e-6, []
+        for m in range(len(a0)):
+            ap, am = list(a0), list(a0)
+            ap[m] += h
+            am[m] -= h
+            fd.append((A.energy_and_grad(conf, ap, precision=prec)[0] - A.energy_and_grad(conf, am, precision=prec)[0])
+                      / (2 * h))
+        fd = np.array(fd)
+        out[name] = {"J": J, "grad": g.tolist(), "fd_device_h1e-6": fd.tolist(),
+                     "max_rel_err_vs_fd": float(np.max(np.abs(g - fd)) / np.max(np.abs(fd))),
+                     "forward_plus_adjoint_s": wall}
+    if "grad" in out.get("fp32c", {}) and "grad" in out.get("fp64", {}):
+        g64, g32 = np.array(out["fp64"]["grad"]), np.array(out["fp32c"]["grad"])
+        out["fp32c"]["grad_max_rel_err_vs_fp64"] = float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64)))
+    progress("adjoint done")
+    if with_cpu:
+        from tests import oracle_lib as O
+        h = O.new_history(conf, conf.Nt + 1)
+        t = time.perf_counter()
+        e = O.run(conf, 0, conf.Nt + 1, h, nthreads=threads)
+        dt = time.perf_counter() - t
+        Jc = conf.dt * float(np.sum(e))
+        out["cpu_baseline"] = {"value": conf.char_steps(0, conf.Nt + 1) / dt, "unit": "char-steps/s",
+                               "cores": threads, "kind": "port", "time_to_T_s": dt,
+                               "sample": "the whole forward run on the CPU (the oracle)"}
+        out["parity_vs_oracle"] = {"J_rel_err_fp64": abs(out["fp64"]["J"] - Jc) / abs(Jc)}
+        progress("adjoint CPU baseline done")
+    return out
 
 
 PEAK_PROBE_CMD = "python3 bench.py --peak-probe"
@@ -266,34 +483,6 @@ def read_traffic(name):
         return None
 
 
-def cpu_baseline(conf, hist_gpu, seconds_budget=15.0):
-    """The oracle (CPU restatement of the reference path, OpenMP over x as
-    bin/test_nufi_cpu_1d.cpp:65-70) timed on isolated steps n of this workload,
-    each from the GPU FP64 run's own history (bin/test_nufi_cpu_3d_isolated.cpp
-    pattern).  Returns char-steps/s."""
-    import numpy as np
-    from tests import oracle_lib as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    st = conf.stride_t()
-    hist = np.zeros(hist_gpu.size + 8)
-    hist[: hist_gpu.size] = hist_gpu
-    total_cs, total_t, steps_done = 0, 0.0, []
-    for n in range(conf.Nt, 0, -50):
-        t = time.perf_counter()
-        O.rho(conf, n, hist, nthreads=threads)
-        dt = time.perf_counter() - t
-        total_cs += conf.n_nodes() * n
-        total_t += dt
-        steps_done.append(n)
-        if total_t > seconds_budget:
-            break
-    assert hist_gpu.size >= (conf.Nt + 1) * st
-    return dict(value=total_cs / total_t, unit="char-steps/s", cores=threads, kind="port",
-                sample=f"rho of isolated time steps n={steps_done} of the 1D two-stream run (Nx=256, Nv=1024), "
-                       f"each from the GPU FP64 history; {total_cs:.3e} char-steps in {total_t:.1f} s; "
-                       f"OpenMP over x ({threads} threads)")
-
-
 def summarize_profiles(outdir, tag, commands):
     """Turn rocprofv3 result databases (kernel trace + FETCH_SIZE / WRITE_SIZE
     passes + the copy-kernel calibration passes) into the committed summaries
@@ -348,7 +537,7 @@ def summarize_profiles(outdir, tag, commands):
             if k in pw and "nufi" in k:
                 b = pf[k][1] * f_scale * 1024 + pw[k][1] * w_scale * 1024
                 lines.append(f"| `{k[:80]}` | {pf[k][0]} | {pf[k][1]:.1f} | {pw[k][1]:.1f} | {b:.4g} |")
-                if "trace" in k:
+                if PROFILE_KERNEL in k:
                     result = {"kernel": k, "dispatches": pf[k][0], "fetch_kib_raw": pf[k][1],
                               "write_kib_raw": pw[k][1], "fetch_scale": f_scale, "write_scale": w_scale,
                               "hbm_bytes_per_launch": b}
@@ -407,17 +596,17 @@ COPY_PROBE_MIB = 256
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-3d", action="store_true", help="skip the 3D3V Landau case")
-    ap.add_argument("--no-extra", action="store_true", help="skip the batched sweep and the adjoint case")
-    ap.add_argument("--shape", default="", help="B,N,C: fixed launch shape of the primary FP64 run (no autotune)")
-    ap.add_argument("--only", choices=["fp64", "fp32c", "both"], default="both")
-    ap.add_argument("--case", choices=["all", "twostream", "landau3d"], default="all",
-                    help="twostream: only the 1D two-stream runs; landau3d: only the 3D3V Landau runs "
-                         "(for profiling one kernel at a time; the JSON line then carries that case)")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--steps", type=int, default=20, help="timed bench steps (3D Landau time step n=80)")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed bench steps before them")
+    ap.add_argument("--runs-1d", type=int, default=5, help="timed whole runs of the 1D workloads")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines and oracle parity")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 1D, batch and adjoint workloads")
+    ap.add_argument("--only", choices=["fp64", "fp32c", "both"], default="both", help="3D Landau forms to run")
+    ap.add_argument("--case", choices=["all", "landau3d", "twostream"], default="all",
+                    help="landau3d / twostream: that workload alone (profiling one kernel at a time)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="autotune the 3D Landau launch shapes at n=80 first (the defaults are the tuned shapes)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes of the 1D workloads")
     ap.add_argument("--dump-energies", default="", help="directory for the per-step energy arrays (.npy)")
     ap.add_argument("--peak-probe", action="store_true",
                     help="only measure the FMA and LDS peaks (writes profiles/peaks_mi355x.*), then exit")
@@ -426,9 +615,12 @@ def main():
     ap.add_argument("--summarize-profiles", nargs=2, metavar=("OUTDIR", "TAG"),
                     help="write profiles/TAG.md (+ _pmc.json) from rocprofv3 databases under OUTDIR")
     ap.add_argument("--profile-commands", default="", help="';;'-separated commands recorded in the summary")
+    ap.add_argument("--profile-kernel", default="trace", help="summary: the kernel whose HBM traffic is kept")
     args = ap.parse_args()
 
     if args.summarize_profiles:
+        global PROFILE_KERNEL
+        PROFILE_KERNEL = args.profile_kernel
         r = summarize_profiles(args.summarize_profiles[0], args.summarize_profiles[1],
                                [c for c in args.profile_commands.split(";;") if c])
         print(json.dumps(r))
@@ -453,87 +645,99 @@ def main():
         print(json.dumps({"copy_probe_bytes": a.numel() * 8}))
         return
 
+    # the CPU baseline's OpenMP runtime reads these when the oracle library loads
+    threads, phys, avail = cpu_threads()
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+    with_cpu = not args.no_cpu
+
     import numpy as np
     import numericalflowiteration_amd as nufi
+    from numericalflowiteration_amd import diagnostics as D
 
-    conf = nufi.twostream_1d()
-    out = {}
-    e64 = e32 = None
-    hist = None
-    tune = None if ( args.no_autotune or args.shape ) else conf.Nt
-    shape = tuple( int( v ) for v in args.shape.split( "," ) ) if args.shape else None
-    if args.case == "landau3d":
-        print(json.dumps({"case": "landau3d", "forms": landau3d_forms(args)}))
+    if args.case == "twostream":
+        print(json.dumps({"case": "twostream", "forms": twostream(args, threads, with_cpu)}))
         return
 
+    c5 = nufi.landau_3d()
+    l3 = {"workload": "3D3V Landau, x in [0,4pi)^3, (1+0.01 sum cos(0.5x_d)) M3(v), v in [-6,6]^3, "
+                      "16^3 x 32^3 nodes, dt=1/8, T=20 (161 steps)"}
+    cpu = parity = None
+    e64 = h64 = None
     if args.only in ("fp64", "both"):
-        out["fp64"], e64, _ = timed_runs(conf, nufi.FP64, args.steps, args.warmup, FLOPS_1D, tune, shape)
-        progress(f"two-stream FP64 {out['fp64']['ms_per_run']:.1f} ms per run")
-        with nufi.GpuScheduler(conf) as s:
-            s.run()
-            hist = s.history()
+        r64, e64, h64, s64 = landau3d_form(c5, nufi.FP64, args.steps, args.warmup, args.autotune)
+        l3["fp64"] = r64
+        progress(f"3D Landau FP64: {r64['time_to_T_s']:.2f} s to T, {r64['ms_per_step']:.1f} ms per step n=80")
+        if with_cpu and args.case == "all":
+            cpu, parity = cpu_landau3d(c5, h64, s64, threads)
+            progress("3D Landau CPU baseline and parity done")
+        # every trace launch of this context (run to T, bench steps, parity steps) is timed
+        r64["trace_roofline"] = trace_roofline(s64, FLOPS_3D, "trace3d_xm_fast_kernel", PEAK_FP64)
+        s64.close()
     if args.only in ("fp32c", "both"):
-        out["fp32c"], e32, h32 = timed_runs(conf, nufi.FP32C, args.steps, args.warmup, FLOPS_1D, tune,
-                                            keep_history=True)
-        progress(f"two-stream FP32C {out['fp32c']['ms_per_run']:.1f} ms per run")
+        r32, e32, h32, s32 = landau3d_form(c5, nufi.FP32C, args.steps, args.warmup, args.autotune)
+        r32["trace_roofline"] = trace_roofline(s32, FLOPS_3D, "trace3d_fp32c_kernel", PEAK_FP64)
+        s32.close()
         if e64 is not None:
-            out["fp32c"].update(energy_errors(e32, e64))
-            out["fp32c"].update(field_errors(h32, hist, conf.stride_t()))
-            out["fp32c"]["speedup_vs_fp64"] = out["fp32c"]["cs_per_s_e2e"] / out["fp64"]["cs_per_s_e2e"]
-
-    if not args.no_3d and args.only == "both" and args.case == "all":
-        out["landau_3d"] = landau3d_forms(args)
-        progress("3D3V Landau done")
+            r32.update(D.run_errors(c5, h32, e32, h64, e64))
+            r32["speedup_vs_fp64_time_to_T"] = l3["fp64"]["time_to_T_s"] / r32["time_to_T_s"]
+            r32["speedup_vs_fp64_steps"] = l3["fp64"]["ms_per_step"] / r32["ms_per_step"]
+        l3["fp32c"] = r32
+        progress(f"3D Landau FP32C: {r32['time_to_T_s']:.2f} s to T, {r32['ms_per_step']:.1f} ms per step n=80")
+        if args.dump_energies:
+            np.save(os.path.join(args.dump_energies, "landau3d_energy_fp32c.npy"), e32)
     if args.dump_energies and e64 is not None:
-        np.save(os.path.join(args.dump_energies, "twostream_energy_fp64.npy"), e64)
-        if e32 is not None:
-            np.save(os.path.join(args.dump_energies, "twostream_energy_fp32c.npy"), e32)
+        np.save(os.path.join(args.dump_energies, "landau3d_energy_fp64.npy"), e64)
 
-    if not args.no_extra and args.only == "both" and args.case == "all":
-        out["batch_sweep"] = batch_sweep()
-        progress("batched sweep done")
-        out["adjoint"] = adjoint_case()
-        progress("adjoint done")
+    forms = {}
+    if not args.no_extra and args.case == "all":
+        forms["twostream_1d"] = twostream(args, threads, with_cpu)
+        forms["landau_1d"] = landau1d(args, threads, with_cpu)
+        forms["batch_sweep"] = batch_sweep(args, threads, with_cpu)
+        forms["adjoint"] = adjoint_case(threads, with_cpu)
 
-    prim = out.get("fp64") or out.get("fp32c")
-    prim_prec = "fp64" if "fp64" in out else "fp32c"
-    peak = PEAK_FP64 if prim_prec == "fp64" else PEAK_FP32
-    roofline = {
-        "bound": "valu", "achieved": prim["achieved_tflops"], "peak": peak, "unit": "TFLOP/s",
-        "frac": prim["achieved_tflops"] / peak,
-        "traffic": read_traffic(f"trace1d_twostream_{prim_prec}_pmc.json"),
-        "traffic_note": "HBM bytes per trace launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary",
-        "kernel": "trace1d_poly_kernel" if prim_prec == "fp64" else "trace1d_fp32c_kernel",
-        "flops_per_char_step": FLOPS_1D,
-        "peak_measured": read_measured_peak("fp64_fma" if prim_prec == "fp64" else "fp32_fma"),
+    prim = l3.get("fp64") or l3.get("fp32c")
+    prim_prec = "fp64" if "fp64" in l3 else "fp32c"
+    roofline = dict(prim["trace_roofline"])
+    roofline.update({
+        "bound": "valu", "unit": "TFLOP/s", "flops_per_char_step": FLOPS_3D,
+        "traffic": read_traffic("trace3d_landau3d_pmc.json"),
+        "traffic_note": "HBM bytes per trace launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary "
+                        "(profiles/trace3d_landau3d.md)",
+        "achieved_note": "algorithmic FLOPs (475 per char-step) per launch over the mean launch duration, HIP "
+                         "events on the context stream around every trace launch of the run to T and of the "
+                         "bench steps",
+        "peak_measured": read_measured_peak("fp64_fma"),
         "peak_measured_note": "FMA-chain probe on the box (profiles/peaks_mi355x.md); peak stays the published figure",
-    }
-    cpu = None
-    if not args.no_cpu and hist is not None:
-        cpu = cpu_baseline(conf, hist)
-        progress("CPU baseline done")
-
+    })
+    if cpu is not None:
+        cpu["cores_physical_lscpu"] = phys
+        cpu["cpus_in_affinity_mask"] = avail
+        cpu["omp_proc_bind"] = os.environ.get("OMP_PROC_BIND")
     line = {
         "metric": METRIC,
-        "value": prim["cs_per_s_e2e"],
+        "value": prim["cs_per_s_steps"],
         "unit": "char-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": prim["ms_per_run"],
+        "ms_per_step": prim["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if prim_prec == "fp64" else "f32 (compensated) + f64",
         "data": "synthetic: analytic initial condition, no dataset",
-        "config": {"workload": "1D two-stream instability, Nx=256 order-4 B-splines, Nv=1024, "
-                               "v in [-8,8], x in [0,10pi), dt=1/32, T=50 (1601 time steps incl. n=0)",
-                   "step": "one whole run to T (time-to-T); value = char-steps of the run / time-to-T",
-                   "char_steps_per_run": prim["char_steps_per_run"],
+        "config": {"workload": "3D3V linear Landau damping, x in [0,4pi)^3, v in [-6,6]^3, 16^3 order-4 tensor "
+                               "B-splines x 32^3 velocity nodes, dt=1/8, T=20 (161 time steps), FP64",
+                   "step": "one device time step (trace of all nodes + rho + Poisson + fit + store) at n=80, the "
+                           "run's mean-cost step, recomputed from the run's own history",
+                   "char_steps_per_step": prim["char_steps_per_step"],
+                   "time_to_T_s": prim["time_to_T_s"], "cs_per_s_time_to_T": prim["cs_per_s_time_to_T"],
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "forms": out,
+        "parity_vs_oracle": parity,
+        "landau_3d": l3,
+        "forms": forms,
     }
     print(json.dumps(line), flush=True)
 

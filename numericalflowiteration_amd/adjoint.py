@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import torch
 
-from .config import F0_MULTIMODE_1D, Config
+from .config import F0_MULTIMODE_1D, FP64, Config
 from .scheduler import GpuScheduler
 
 
@@ -30,24 +30,25 @@ def _with_params(conf: Config, values) -> Config:
     return c
 
 
-def energy_and_grad(conf: Config, values, device: int = 0):
-    """(J, dJ/da) for the parameters `values` of conf's f0 family."""
-    with GpuScheduler(_with_params(conf, values), device=device) as s:
+def energy_and_grad(conf: Config, values, device: int = 0, precision: int = FP64):
+    """(J, dJ/da) for the parameters `values` of conf's f0 family, with the trace
+    of the given precision (FP64 or FP32C) in the forward and reverse sweeps."""
+    with GpuScheduler(_with_params(conf, values), device=device, precision=precision) as s:
         return s.adjoint_grad()
 
 
 class TimeIntegratedEnergy(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, a: torch.Tensor, conf: Config):
-        J, g = energy_and_grad(conf, a.detach().cpu().double().tolist())
+    def forward(ctx, a: torch.Tensor, conf: Config, precision: int = FP64):
+        J, g = energy_and_grad(conf, a.detach().cpu().double().tolist(), precision=precision)
         ctx.save_for_backward(torch.as_tensor(g, dtype=a.dtype))
         return torch.as_tensor(J, dtype=a.dtype)
 
     @staticmethod
     def backward(ctx, grad_out):
         (g,) = ctx.saved_tensors
-        return grad_out * g, None
+        return grad_out * g, None, None
 
 
-def time_integrated_energy(a: torch.Tensor, conf: Config) -> torch.Tensor:
-    return TimeIntegratedEnergy.apply(a, conf)
+def time_integrated_energy(a: torch.Tensor, conf: Config, precision: int = FP64) -> torch.Tensor:
+    return TimeIntegratedEnergy.apply(a, conf, precision)

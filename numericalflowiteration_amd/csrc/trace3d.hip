@@ -553,7 +553,7 @@ __device__ __forceinline__ void kick32( float &uh, float &ul, float q )
 // velocity-major: node r of a lane is make_node3's (wave task g, lane), the
 // wave's 64 lanes are 64 consecutive velocities of one spatial point
 template <int BLOCK, int NPT, bool MASK>
-__global__ __launch_bounds__( BLOCK )
+__global__ __launch_bounds__( BLOCK ) __attribute__(( amdgpu_waves_per_eu( 4 ) ))
 void trace3d_fp32c_kernel( trace3_params p )
 {
     extern __shared__ float ldsf[];

@@ -26,18 +26,14 @@ _VAL = (1.0 / 6.0, 4.0 / 6.0, 1.0 / 6.0)
 _DER = (-0.5, 0.0, 0.5)
 
 
-def _grid_shape(conf: Config):
+def _coeffs(conf: Config, slices) -> np.ndarray:
+    """Slices as an array indexed [..., k, j, i] (3D) or [..., i] (1D), padded
+    extent N+3 per axis; `slices` holds one slice or several (a leading axis)."""
+    s = np.asarray(slices, dtype=np.float64)
     if conf.dim == 1:
-        return (conf.Nx,)
-    return (conf.Nz, conf.Ny, conf.Nx)
-
-
-def _coeffs(conf: Config, slice_) -> np.ndarray:
-    """The slice as an array indexed [k, j, i] (3D) or [i] (1D), padded extent N+3."""
-    s = np.asarray(slice_, dtype=np.float64)
-    if conf.dim == 1:
-        return s[: conf.Nx + 3]
-    return s[: (conf.Nx + 3) * (conf.Ny + 3) * (conf.Nz + 3)].reshape(conf.Nz + 3, conf.Ny + 3, conf.Nx + 3)
+        return s[..., : conf.Nx + 3]
+    p = (conf.Nz + 3, conf.Ny + 3, conf.Nx + 3)
+    return s[..., : p[0] * p[1] * p[2]].reshape(s.shape[:-1] + p)
 
 
 def _stencil(c: np.ndarray, axis: int, w, n: int) -> np.ndarray:
@@ -49,29 +45,32 @@ def _stencil(c: np.ndarray, axis: int, w, n: int) -> np.ndarray:
     return out
 
 
-def node_potential(conf: Config, slice_) -> np.ndarray:
-    """phi at the grid nodes (shape (Nx,) or (Nz, Ny, Nx))."""
-    c = _coeffs(conf, slice_)
+def _axes(conf: Config):
+    """(axis, N, d_inv) of x, y, z in the coefficient array (x fastest)."""
     if conf.dim == 1:
-        return _stencil(c, 0, _VAL, conf.Nx)
-    c = _stencil(c, 2, _VAL, conf.Nx)
-    c = _stencil(c, 1, _VAL, conf.Ny)
-    return _stencil(c, 0, _VAL, conf.Nz)
+        return [(-1, conf.Nx, conf.dx_inv)]
+    return [(-1, conf.Nx, conf.dx_inv), (-2, conf.Ny, conf.dy_inv), (-3, conf.Nz, conf.dz_inv)]
 
 
-def node_field(conf: Config, slice_) -> np.ndarray:
-    """E = -grad phi at the grid nodes: shape (Nx,) in 1D, (3, Nz, Ny, Nx) in 3D
-    (components x, y, z)."""
-    c = _coeffs(conf, slice_)
-    if conf.dim == 1:
-        return -_stencil(c, 0, _DER, conf.Nx) * conf.dx_inv
+def node_potential(conf: Config, slices) -> np.ndarray:
+    """phi at the grid nodes: shape (..., Nx) in 1D, (..., Nz, Ny, Nx) in 3D."""
+    c = _coeffs(conf, slices)
+    for axis, n, _ in _axes(conf):
+        c = _stencil(c, axis, _VAL, n)
+    return c
+
+
+def node_field(conf: Config, slices) -> np.ndarray:
+    """E = -grad phi at the grid nodes: shape (..., Nx) in 1D, (..., 3, Nz, Ny, Nx)
+    in 3D (components x, y, z)."""
+    c = _coeffs(conf, slices)
     comps = []
-    for axis_der, d_inv in ((2, conf.dx_inv), (1, conf.dy_inv), (0, conf.dz_inv)):
+    for der_axis, _, d_inv in _axes(conf):
         g = c
-        for axis, n in ((2, conf.Nx), (1, conf.Ny), (0, conf.Nz)):
-            g = _stencil(g, axis, _DER if axis == axis_der else _VAL, n)
+        for axis, n, _ in _axes(conf):
+            g = _stencil(g, axis, _DER if axis == der_axis else _VAL, n)
         comps.append(-g * d_inv)
-    return np.stack(comps)
+    return comps[0] if conf.dim == 1 else np.stack(comps, axis=-4)
 
 
 def _k2(conf: Config) -> np.ndarray:
@@ -83,10 +82,18 @@ def _k2(conf: Config) -> np.ndarray:
     return kx * kx + ky * ky + kz * kz
 
 
-def node_density(conf: Config, slice_) -> np.ndarray:
-    """The mean-free rho of the step whose Poisson solve and fit made this slice."""
-    phi = node_potential(conf, slice_)
-    return np.real(np.fft.ifftn(_k2(conf) * np.fft.fftn(phi)))
+def node_density(conf: Config, slices) -> np.ndarray:
+    """The mean-free rho of the step whose Poisson solve and fit made each slice."""
+    phi = node_potential(conf, slices)
+    axes = (-1,) if conf.dim == 1 else (-3, -2, -1)
+    return np.real(np.fft.ifftn(_k2(conf) * np.fft.fftn(phi, axes=axes), axes=axes))
+
+
+def _per_step_rel(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """||a_n - b_n|| / ||b_n|| per leading index n (0 where b_n and a_n vanish)."""
+    d = np.sqrt(np.sum((a - b).reshape(a.shape[0], -1) ** 2, axis=1))
+    r = np.sqrt(np.sum(b.reshape(b.shape[0], -1) ** 2, axis=1))
+    return np.where(r > 0, d / np.where(r > 0, r, 1.0), (d > 0).astype(float))
 
 
 def run_errors(conf: Config, hist, energies, ref_hist, ref_energies) -> dict:
@@ -95,23 +102,18 @@ def run_errors(conf: Config, hist, energies, ref_hist, ref_energies) -> dict:
     mean-free rho, of E at the nodes and of the electric energy, with the step
     each maximum is reached at (all steps 0..Nt of both histories)."""
     st = conf.stride_t()
-    n_slices = min(len(energies), len(ref_energies), np.size(hist) // st, np.size(ref_hist) // st)
-    worst = {"rho": (0.0, 0), "E": (0.0, 0), "energy": (0.0, 0)}
-    for n in range(n_slices):
-        a, b = hist[n * st:(n + 1) * st], ref_hist[n * st:(n + 1) * st]
-        for key, fn in (("rho", node_density), ("E", node_field)):
-            x, y = fn(conf, a), fn(conf, b)
-            den = np.linalg.norm(y)
-            e = float(np.linalg.norm(x - y) / den) if den > 0 else float(np.linalg.norm(x - y) > 0)
-            if e > worst[key][0]:
-                worst[key] = (e, n)
-        e = abs(energies[n] - ref_energies[n]) / abs(ref_energies[n]) if ref_energies[n] != 0 else 0.0
-        if e > worst["energy"][0]:
-            worst["energy"] = (float(e), n)
-    return {"rho_max_rel_err": worst["rho"][0], "rho_err_at_step": worst["rho"][1],
-            "E_max_rel_err": worst["E"][0], "E_err_at_step": worst["E"][1],
-            "energy_max_rel_err": worst["energy"][0], "energy_err_at_step": worst["energy"][1],
-            "steps_compared": n_slices}
+    n = min(len(energies), len(ref_energies), np.size(hist) // st, np.size(ref_hist) // st)
+    a = np.asarray(hist, dtype=np.float64)[: n * st].reshape(n, st)
+    b = np.asarray(ref_hist, dtype=np.float64)[: n * st].reshape(n, st)
+    er = _per_step_rel(node_density(conf, a), node_density(conf, b))
+    eE = _per_step_rel(node_field(conf, a), node_field(conf, b))
+    e, r = np.asarray(energies[:n], dtype=np.float64), np.asarray(ref_energies[:n], dtype=np.float64)
+    ee = np.where(r != 0, np.abs(e - r) / np.where(r != 0, np.abs(r), 1.0), 0.0)
+    return {"rho_max_rel_err": float(er.max()), "rho_err_at_step": int(er.argmax()),
+            "E_max_rel_err": float(eE.max()), "E_err_at_step": int(eE.argmax()),
+            "energy_max_rel_err": float(ee.max()), "energy_err_at_step": int(ee.argmax()),
+            "energy_normwise_rel_err": float(np.linalg.norm(e - r) / np.linalg.norm(r)),
+            "steps_compared": int(n)}
 
 
 __all__ = ["node_potential", "node_field", "node_density", "run_errors"]

@@ -104,7 +104,12 @@ class GpuScheduler:
         return metrics
 
     # ---- device-resident loop --------------------------------------------
-    def step(self, n: int) -> float:
+    def step(self, n: int, wait: bool = True) -> float | None:
+        """Step n on the device; returns its energy (synchronising), or queues the
+        step and returns None with wait=False."""
+        if not wait:
+            self._check(self._lib.nufi_step(self._ctx, n, None))
+            return None
         e = np.zeros(1)
         self._check(self._lib.nufi_step(self._ctx, n, _ptr(e)))
         return float(e[0])
