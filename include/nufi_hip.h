@@ -228,7 +228,10 @@ int nufi_trace_timing(nufi_ctx *ctx, uint64_t *launches, double *kernel_ms,
  * poisson.cpp:73-88) with respect to the parameters of f0: a_1..a_n_modes for
  * NUFI_F0_MULTIMODE_1D, alpha for the Landau and two-stream families.
  * dJ_da receives n_par >= that count values; *J_out (may be NULL) receives J.
- * 1D, order 4, FP64 contexts; the history is overwritten by the forward run.
+ * 1D, order 4; the history is overwritten by the forward run.  An FP32C
+ * context differentiates its own forward model, the FP32-compensated trace
+ * (the reverse sweep undoes its steps on the same hi + lo pairs; the adjoint
+ * variables, the slice adjoints and the field adjoint are FP64).
  */
 int nufi_adjoint_grad(nufi_ctx *ctx, double *dJ_da, int n_par, double *J_out);
 

@@ -19,6 +19,14 @@
 // lambda_m accumulates in LDS per workgroup (all nodes of a workgroup visit the
 // same slice at the same time) and is flushed with global atomics once per
 // chunk of slices; its summation order therefore varies at the rounding level.
+//
+// FP32C contexts differentiate the FP32-compensated trace: the forward sweep
+// repeats the FP32C kernel's steps (the shared fp32c_* helpers of step1d.hpp,
+// so the foot points are those of the forward run), the reverse sweep undoes
+// them on the same pairs (u_b = u_a - h q(s_a), s_b = s_a + cd u_b, both with
+// TwoSum), and the adjoints sbar, ubar, lambda and the parameter gradients are
+// FP64, as is the field adjoint.  The FP32 tables hold kick * (Q0, Q1, Q2), so
+// pbar = h kick ubar and sbar += h ubar dq/dt.
 #include "device_common.hpp"
 #include "fft_lds.hpp"
 #include "internal.hpp"
@@ -182,6 +190,117 @@ void adjoint1d_kernel( adjoint1_params p )
     }
 }
 
+// adjoints of q0 = D0 + D1, q1 = 2 (D1 - D0), q2 = D0 - 2 D1 + D2 folded into
+// the slice adjoints of cell i (periodic, Nx points)
+__device__ __forceinline__ void scatter_poly_adjoint( double *L, int i, int Nx, double pbar, double t )
+{
+    const double q0 = pbar, q1 = pbar * t, q2 = pbar * ( t*t );
+    const double d0 = q0 - 2.0*q1 + q2, d1 = q0 + 2.0*q1 - 2.0*q2, d2 = q2;
+    const int i0 = ( i + Nx ) % Nx;
+    atomicAdd( L + i0, -d0 );
+    atomicAdd( L + ( i0 + 1 ) % Nx, d0 - d1 );
+    atomicAdd( L + ( i0 + 2 ) % Nx, d1 - d2 );
+    atomicAdd( L + ( i0 + 3 ) % Nx, d2 );
+}
+
+template <int BLOCK, bool POW2>
+__global__ __launch_bounds__( BLOCK )
+void adjoint1d_fp32c_kernel( adjoint1_params p )
+{
+    extern __shared__ double lds[];
+    float  *Ql  = reinterpret_cast<float*>( lds );                       // chunk of FP32 tables
+    double *lam = lds + ( size_t( p.chunk ) * p.stride_q + 1 ) / 2;      // chunk of folded slice adjoints
+    constexpr int WPB = BLOCK / WAVE;
+    const uint32_t lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    const uint32_t g = blockIdx.x * WPB + wave;
+    const bool task = g < p.n_tasks;
+    const uint32_t ix = task ? g / p.wpx : 0, j = task ? g % p.wpx : 0;
+    const uint32_t iu = j * WAVE + lane;
+    const bool active = task && iu < p.Nu;
+
+    const int Nx = int( p.Nx ), mask = Nx - 1;
+    const float Nxf = float( p.Nx ), invNx = 1.0f / Nxf;
+    const double cd = p.dt * p.dx_inv, kick = -0.5 * p.dt * p.dx_inv;
+    const float cd_hi = float( cd ), cd_lo = float( cd - double( cd_hi ) );
+    const uint32_t sq = p.stride_q, q2 = p.q2_off;
+
+    // the FP32C trace kernel's start state (trace1d_fp32c_body)
+    const double u0 = p.u_min + iu*p.du + p.du/2;
+    float sh = float( ix ), sl = 0.0f;
+    float uh = float( u0 ), ul = float( u0 - double( uh ) );
+
+    // forward: slices n-1 .. 0
+    for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
+    {
+        const int64_t mlo = mhi - int64_t( p.chunk ) + 1 > 0 ? mhi - int64_t( p.chunk ) + 1 : 0;
+        const int cnt = int( mhi - mlo + 1 );
+        __syncthreads();
+        dma_copy16<BLOCK>( Ql, p.qhist32 + size_t( mlo ) * sq, uint32_t( cnt ) * sq / 4u );
+        dma_publish();
+        for ( int jj = cnt - 1; jj >= 0; --jj )
+        {
+            const int64_t m = mlo + jj;
+            if ( POW2 && poly_wrap_due( m ) ) fp32c_wrap( sh, sl, Nxf, invNx );
+            fp32c_drift( sh, sl, uh, ul, cd_hi, cd_lo );
+            if ( !POW2 ) fp32c_wrap( sh, sl, Nxf, invNx );
+            float t;
+            const int i = fp32c_locate<POW2>( sh, sl, mask, t );
+            fp32c_kick( uh, ul, ( m == 0 ? 0.5f : 1.0f ) * fp32c_poly( Ql + jj * sq, q2, i, t ) );
+        }
+    }
+
+    // seed at the foot
+    const double fbar = active ? -p.du * p.g[ix] : 0.0;
+    double fx, fu, fa[NUFI_MAX_MODES];
+    f0_grad_1d( p.f0, p.x_min + ( double( sh ) + double( sl ) ) * p.dx, double( uh ) + double( ul ), fx, fu, fa, p.npar );
+    double sbar = fbar * fx * p.dx;     // d/ds = dx d/dx
+    double ubar = fbar * fu;
+    #pragma unroll
+    for ( int m = 0; m < NUFI_MAX_MODES; ++m )
+    {
+        if ( m >= int( p.npar ) ) continue;
+        const csum v = wave_csum( csum { fbar * fa[m], 0.0 } );
+        if ( task && lane == 0 ) p.grad_part[ size_t( g ) * NUFI_MAX_MODES + m ] += v.s + v.c;
+    }
+
+    // reverse: slices 0 .. n-1, undoing each step on the pairs
+    for ( int64_t mlo = 0; mlo < int64_t( p.n ); mlo += p.chunk )
+    {
+        const int64_t mhi = ( mlo + p.chunk - 1 < int64_t( p.n ) - 1 ) ? mlo + p.chunk - 1 : int64_t( p.n ) - 1;
+        const int cnt = int( mhi - mlo + 1 );
+        __syncthreads();
+        dma_copy16<BLOCK>( Ql, p.qhist32 + size_t( mlo ) * sq, uint32_t( cnt ) * sq / 4u );
+        for ( uint32_t q = threadIdx.x; q < uint32_t( cnt ) * p.Nx; q += BLOCK ) lam[q] = 0.0;
+        dma_publish();
+        for ( int jj = 0; jj < cnt; ++jj )
+        {
+            const int64_t m = mlo + jj;
+            const float hf = m == 0 ? 0.5f : 1.0f;
+            const float *Q = Ql + jj * sq;
+            float t;
+            const int i = fp32c_locate<POW2>( sh, sl, mask, t );
+            const float q = fp32c_poly( Q, q2, i, t );
+            // dq/dt of the scaled table, in FP64
+            const float2 q01 = reinterpret_cast<const float2*>( Q )[ i + 1 ];
+            const double dqdt = fma( 2.0 * double( Q[ q2 + ( i + 1 ) ] ), double( t ), double( q01.y ) );
+            const double pbar = double( hf ) * kick * ubar;
+            sbar = fma( double( hf ) * ubar, dqdt, sbar );
+            if ( active ) scatter_poly_adjoint( lam + jj * p.Nx, i, Nx, pbar, double( t ) );
+            // undo the kick, then the drift
+            fp32c_kick( uh, ul, -( hf * q ) );
+            ubar = fma( -cd, sbar, ubar );
+            fp32c_drift( sh, sl, -uh, -ul, cd_hi, cd_lo );
+            if ( !POW2 || poly_wrap_due( m ) ) fp32c_wrap( sh, sl, Nxf, invNx );
+        }
+        __syncthreads();
+        for ( uint32_t q = threadIdx.x; q < uint32_t( cnt ) * p.Nx; q += BLOCK )
+        {
+            const double v = lam[q];
+            if ( v != 0.0 ) atomicAdd( p.lambda + size_t( mlo ) * p.Nx + q, v );
+        }
+    }
+}
+
 // g = 2 dt Re F^H( w . F rho ) + Re F^H( conj(m) . F lambda )   (one workgroup)
 __global__ __launch_bounds__( FIELD_BLOCK )
 void adjoint_field_kernel( adjoint_field_params p )
@@ -236,6 +355,7 @@ void reduce_grad_kernel( const double *part, uint32_t n_tasks, uint32_t npar, do
 
 size_t adjoint1d_lds_bytes( const adjoint1_params &p )
 {
+    if ( p.qhist32 ) return ( size_t( p.chunk ) * p.stride_q + 1 ) / 2 * sizeof(double) + size_t( p.chunk ) * p.Nx * sizeof(double);
     return size_t( p.chunk ) * ( p.stride_q + p.Nx ) * sizeof(double);
 }
 
@@ -246,15 +366,20 @@ hipError_t launch_adjoint1d( const adjoint1_params &p, hipStream_t st )
     const uint32_t grid = ( p.n_tasks + BLOCK / WAVE - 1 ) / ( BLOCK / WAVE );
     const size_t lds = adjoint1d_lds_bytes( p );
     const bool pow2 = ( p.Nx & ( p.Nx - 1 ) ) == 0;
-    const void *k = pow2 ? reinterpret_cast<const void*>( adjoint1d_kernel<BLOCK,true> )
-                         : reinterpret_cast<const void*>( adjoint1d_kernel<BLOCK,false> );
+    const bool f32 = p.qhist32 != nullptr;
+    const void *k = f32 ? ( pow2 ? reinterpret_cast<const void*>( adjoint1d_fp32c_kernel<BLOCK,true> )
+                                 : reinterpret_cast<const void*>( adjoint1d_fp32c_kernel<BLOCK,false> ) )
+                        : ( pow2 ? reinterpret_cast<const void*>( adjoint1d_kernel<BLOCK,true> )
+                                 : reinterpret_cast<const void*>( adjoint1d_kernel<BLOCK,false> ) );
     if ( lds > 65536 )
     {
         hipError_t e = hipFuncSetAttribute( k, hipFuncAttributeMaxDynamicSharedMemorySize, int( lds ) );
         if ( e != hipSuccess ) return e;
     }
-    if ( pow2 ) hipLaunchKernelGGL( ( adjoint1d_kernel<BLOCK,true> ), dim3( grid ), dim3( BLOCK ), lds, st, p );
-    else        hipLaunchKernelGGL( ( adjoint1d_kernel<BLOCK,false> ), dim3( grid ), dim3( BLOCK ), lds, st, p );
+    if ( f32 && pow2 )  hipLaunchKernelGGL( ( adjoint1d_fp32c_kernel<BLOCK,true> ), dim3( grid ), dim3( BLOCK ), lds, st, p );
+    else if ( f32 )     hipLaunchKernelGGL( ( adjoint1d_fp32c_kernel<BLOCK,false> ), dim3( grid ), dim3( BLOCK ), lds, st, p );
+    else if ( pow2 )    hipLaunchKernelGGL( ( adjoint1d_kernel<BLOCK,true> ), dim3( grid ), dim3( BLOCK ), lds, st, p );
+    else                hipLaunchKernelGGL( ( adjoint1d_kernel<BLOCK,false> ), dim3( grid ), dim3( BLOCK ), lds, st, p );
     return hipGetLastError();
 }
 

@@ -753,8 +753,8 @@ int nufi_run( nufi_ctx *c, uint64_t n_begin, uint64_t n_end, double *energies )
 int nufi_adjoint_grad( nufi_ctx *c, double *dJ_da, int n_par, double *J_out )
 {
     if ( c == nullptr ) return NUFI_ERR_INVALID;
-    if ( c->dim != 1 || c->order != 4 || c->precision != NUFI_FP64 )
-        return fail( c, NUFI_ERR_UNSUPPORTED, "adjoint_grad: built for 1D, order 4, FP64" );
+    if ( c->dim != 1 || c->order != 4 )
+        return fail( c, NUFI_ERR_UNSUPPORTED, "adjoint_grad: built for 1D, order 4" );
     const int npar = adjoint_npar( c->conf.f0 );
     if ( npar == 0 ) return fail( c, NUFI_ERR_UNSUPPORTED, "adjoint_grad: no parameters for this f0 family" );
     if ( n_par < npar || dJ_da == nullptr ) return fail( c, NUFI_ERR_INVALID, "adjoint_grad: output too short" );
@@ -771,12 +771,12 @@ int nufi_adjoint_grad( nufi_ctx *c, double *dJ_da, int n_par, double *J_out )
         if ( ( rc = dev_alloc( c, &c->grad_part, size_t( n_tasks ) * NUFI_MAX_MODES, "adjoint partials" ) ) ) return rc;
         if ( ( rc = dev_alloc( c, &c->grad, NUFI_MAX_MODES, "adjoint gradient" ) ) ) return rc;
     }
-    // forward run (the FP64 fast trace, as nufi_run), keeping rho_n; a STRICT
-    // context's launch shape is not one the fast kernel takes, so the fast
-    // default shape runs it and both are restored afterwards
+    // forward run (the FP64 fast or the FP32C trace, as nufi_run), keeping
+    // rho_n; a STRICT context's launch shape is not one the fast kernel takes,
+    // so the fast default shape runs it and both are restored afterwards
     const int saved_variant = c->variant;
     const launch_shape saved_shape = c->shape;
-    if ( saved_variant != TRACE_FAST )
+    if ( saved_variant == TRACE_STRICT )
     {
         if ( slice_lds_bytes( &c->conf, c->order, TRACE_FAST ) > LDS_LIMIT )
             return fail( c, NUFI_ERR_UNSUPPORTED, "adjoint_grad: grid too large for the fast trace" );
@@ -794,15 +794,17 @@ int nufi_adjoint_grad( nufi_ctx *c, double *dJ_da, int n_par, double *J_out )
     af.N = uint32_t( Nx ); af.log2x = log2_or_none( Nx );
     af.g = c->gvec; af.tw = c->tw_x; af.mult = c->mult_step; af.ewt = c->ewt; af.dt = c->conf.dt;
     adjoint1_params ap {};
-    ap.qhist = c->qhist; ap.g = c->gvec; ap.lambda = c->lambda; ap.grad_part = c->grad_part;
+    const bool f32 = c->precision == NUFI_FP32C;
+    ap.qhist = f32 ? nullptr : c->qhist; ap.qhist32 = f32 ? c->qhist32 : nullptr;
+    ap.g = c->gvec; ap.lambda = c->lambda; ap.grad_part = c->grad_part;
     ap.Nx = uint32_t( Nx ); ap.Nu = uint32_t( c->conf.Nu ); ap.stride_d = c->sd;
     ap.stride_q = c->sq; ap.q2_off = poly_q2( uint32_t( Nx ) );
     ap.wpx = c->wpx;
     ap.n_tasks = n_tasks; ap.npar = uint32_t( npar );
     ap.x_min = c->conf.x_min; ap.dx = c->conf.dx; ap.dx_inv = c->conf.dx_inv; ap.dt = c->conf.dt;
     ap.u_min = c->conf.u_min; ap.du = c->conf.du; ap.f0 = c->conf.f0;
-    ap.chunk = uint32_t( std::max<size_t>( 1, std::min<size_t>( 32, std::min<size_t>( c->lds_cap, 65536 ) /
-                                                                  ( ( size_t( c->sq ) + Nx ) * 8 ) ) ) );
+    const size_t per_slice = size_t( c->sq ) * ( f32 ? sizeof(float) : sizeof(double) ) + Nx * sizeof(double);
+    ap.chunk = uint32_t( std::max<size_t>( 1, std::min<size_t>( 32, std::min<size_t>( c->lds_cap, 65536 ) / per_slice ) ) );
     for ( uint64_t n = Nt + 1; n-- > 0; )
     {
         af.rho = c->rho_hist + n * Nx; af.lambda = c->lambda + n * Nx;

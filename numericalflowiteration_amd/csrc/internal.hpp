@@ -188,6 +188,7 @@ bool trace3d_shape_ok( int variant, int order, const launch_shape &s );
 struct adjoint1_params
 {
     const double *qhist;      // kick-polynomial tables (FP64), slice m at qhist + m*stride_q
+    const float  *qhist32;    // FP32C: the FP32 tables scaled by the kick (qhist is then null)
     const double *g;          // dJ/drho_n (Nx)
     double       *lambda;     // folded slice adjoints, slice m at lambda + m*Nx
     double       *grad_part;  // per wave-task parameter-gradient accumulators [g][NUFI_MAX_MODES]

@@ -104,6 +104,56 @@ __device__ __forceinline__ void poly_step( double &s, double &u, const double *Q
     s = x;
 }
 
+// ---- the FP32-compensated step ---------------------------------------------
+// Position and velocity as unevaluated pairs (hi + lo).  The FP32C trace kernel
+// and the FP32C adjoint share these, so both reach the same states bit for bit.
+
+// drift s -= cd*u with TwoProd on the leading product (cd = cd_hi + cd_lo)
+__device__ __forceinline__ void fp32c_drift( float &sh, float &sl, float uh, float ul, float cd_hi, float cd_lo )
+{
+    const float pr = cd_hi * uh;
+    float pe = fmaf( cd_hi, uh, -pr );
+    pe = fmaf( cd_hi, ul, fmaf( cd_lo, uh, pe ) );
+    float y, e;
+    two_sum( sh, -pr, y, e );
+    e = e + ( sl - pe );
+    fast_two_sum( y, e, sh, sl );
+}
+
+// exact periodic wrap of the pair: sh - Nx*w is exact (Sterbenz) for w >= 1
+// and w <= -2, and Fast2Sum is exact for w = -1 (Nx >= |sh|)
+__device__ __forceinline__ void fp32c_wrap( float &sh, float &sl, float Nxf, float invNx )
+{
+    const float w = floorf( sh * invNx );
+    float y2, e2;
+    fast_two_sum( -Nxf * w, sh, y2, e2 );
+    sh = y2; sl = sl + e2;
+}
+
+// the cell (in [-1, Nx]; POW2: wrapped by the mask) and t of the pair
+template <bool POW2>
+__device__ __forceinline__ int fp32c_locate( float sh, float sl, int mask, float &t )
+{
+    const float k = floorf( sh );
+    t = ( sh - k ) + sl;
+    return POW2 ? ( int( k ) & mask ) : int( k );
+}
+
+// the kick q = kick * (Q0 + Q1 t + Q2 t^2) from the FP32 table (pairs at Q, Q2 row at Q + q2)
+__device__ __forceinline__ float fp32c_poly( const float *Q, uint32_t q2, int i, float t )
+{
+    const float2 q01 = reinterpret_cast<const float2*>( Q )[ i + 1 ];
+    return fmaf( fmaf( Q[ q2 + ( i + 1 ) ], t, q01.y ), t, q01.x );
+}
+
+// u += q with TwoSum
+__device__ __forceinline__ void fp32c_kick( float &uh, float &ul, float q )
+{
+    float y, e;
+    two_sum( uh, q, y, e );
+    uh = y; ul = ul + e;
+}
+
 // ---- kick-polynomial tables ------------------------------------------------
 // Entry j (cell j-1) of a slice's table from the entries j, j+1, j+2 of the
 // slice's extended difference row (entry e holds c[e] - c[e-1], periodic):

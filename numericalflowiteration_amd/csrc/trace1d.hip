@@ -400,41 +400,19 @@ __device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint
     const float cd_hi = float( cd ), cd_lo = float( cd - double( cd_hi ) );
     const uint32_t sq = p.stride_q, q2 = p.q2_off;
 
-    // exact periodic wrap of the pair: sh - Nx*w is exact (Sterbenz) for w >= 1
-    // and w <= -2, and Fast2Sum is exact for w = -1 (Nx >= |sh|)
-    auto wrap = [&]( int r )
-    {
-        const float w = floorf( sh[r] * invNx );
-        float y2, e2;
-        fast_two_sum( -Nxf * w, sh[r], y2, e2 );
-        sh[r] = y2; sl[r] = sl[r] + e2;
-    };
+    auto wrap = [&]( int r ) { fp32c_wrap( sh[r], sl[r], Nxf, invNx ); };
     auto step = [&]( const float *Q, float hf )
     {
         #pragma unroll
         for ( int r = 0; r < NPT; ++r )
         {
-            // drift: s -= cd*u with TwoProd on the leading product
-            const float pr = cd_hi * uh[r];
-            float pe = fmaf( cd_hi, uh[r], -pr );
-            pe = fmaf( cd_hi, ul[r], fmaf( cd_lo, uh[r], pe ) );
-            float y, e;
-            two_sum( sh[r], -pr, y, e );
-            e = e + ( sl[r] - pe );
-            fast_two_sum( y, e, sh[r], sl[r] );
+            fp32c_drift( sh[r], sl[r], uh[r], ul[r], cd_hi, cd_lo );
             // exact periodic wrap into [0, Nx); POW2: the cell index wraps by
             // the mask and the position is re-wrapped every POLY_WRAP slices
             if ( !POW2 ) wrap( r );
-            // locate
-            const float k = floorf( sh[r] );
-            const float t = ( sh[r] - k ) + sl[r];
-            const int i = POW2 ? ( int( k ) & mask ) : int( k );     // in [-1, Nx]
-            const float2 q01 = reinterpret_cast<const float2*>( Q )[ i + 1 ];
-            const float kq = fmaf( fmaf( Q[ q2 + ( i + 1 ) ], t, q01.y ), t, q01.x );
-            // kick: u += h*kq with TwoSum
-            float y3, e3;
-            two_sum( uh[r], hf * kq, y3, e3 );
-            uh[r] = y3; ul[r] = ul[r] + e3;
+            float t;
+            const int i = fp32c_locate<POW2>( sh[r], sl[r], mask, t );
+            fp32c_kick( uh[r], ul[r], hf * fp32c_poly( Q, q2, i, t ) );
         }
     };
     // slices mlo+hi down to mlo+lo, in segments between re-wrap points;

@@ -100,6 +100,49 @@ def test_adjoint_of_a_strict_context():
     assert abs(conf.dt * float(np.sum(e)) - J0) <= 1e-12 * abs(J0)
 
 
+def test_fp32c_adjoint_against_the_oracle_tangent_and_fd():
+    """The FP32C adjoint differentiates the FP32-compensated forward model.  Its
+    J is the FP32C run's (against the oracle's FP64 J: <= 1e-6), its gradient is
+    checked against the oracle's forward tangent, which central differences of
+    the oracle pin (test above), at <= 1e-5 of the gradient's scale, and against
+    the FP64 device adjoint; and against central differences (h = 1e-6) of the
+    FP32C device model itself, whose rounding noise divided by h dominates that
+    comparison (measured 6.9e-6; the FP32C forward is deterministic, so the
+    figure is too) — held to the adjoint bar, 1e-5."""
+    conf = small_multimode()
+    a0 = nufi.adjoint_modes()
+    J, g = A.energy_and_grad(conf, a0, precision=nufi.FP32C)
+    J64, g64 = A.energy_and_grad(conf, a0)
+    Jt, dJ = O.tangent_run(conf)
+    scale = np.max(np.abs(dJ[:8]))
+    err_t = np.max(np.abs(g - dJ[:8])) / scale
+    err_64 = np.max(np.abs(g - g64)) / scale
+    h = 1e-6
+    fd = []
+    for m in range(8):
+        ap, am = list(a0), list(a0)
+        ap[m] += h
+        am[m] -= h
+        fd.append((A.energy_and_grad(conf, ap, precision=nufi.FP32C)[0] -
+                   A.energy_and_grad(conf, am, precision=nufi.FP32C)[0]) / (2 * h))
+    err_fd = np.max(np.abs(g - np.array(fd))) / scale
+    print(f"[adjoint FP32C] J {J:.12e} vs oracle {Jt:.12e}; grad vs tangent {err_t:.3e}, vs FP64 adjoint "
+          f"{err_64:.3e}, vs FD of the FP32C model {err_fd:.3e}", flush=True)
+    assert abs(J - Jt) <= 1e-6 * abs(Jt)
+    assert err_t <= 1e-5 and err_64 <= 1e-5
+    assert err_fd <= 1e-5
+
+
+def test_fp32c_adjoint_non_power_of_two_grid():
+    """Nx = 24 (positions wrapped every step) against the oracle's tangent."""
+    conf = nufi.multimode_1d(nufi.adjoint_modes(), Nx=24, Nu=128, T=1.5)
+    J, g = A.energy_and_grad(conf, nufi.adjoint_modes(), precision=nufi.FP32C)
+    Jt, dJ = O.tangent_run(conf)
+    err = np.max(np.abs(g - dJ[:8])) / np.max(np.abs(dJ[:8]))
+    print(f"[adjoint FP32C Nx=24] grad vs tangent {err:.3e}", flush=True)
+    assert abs(J - Jt) <= 1e-6 * abs(Jt) and err <= 1e-5
+
+
 def test_adjoint_rejects_unsupported():
     with nufi.GpuScheduler(nufi.landau_3d(N=8, V=8, T=1.0)) as s:
         with pytest.raises(nufi.NufiError):
