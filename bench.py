@@ -86,10 +86,13 @@ def sync():
 
 # ---- 3D3V Landau (headline) ---------------------------------------------------
 
-def landau3d_form(conf, precision, steps, warmup, autotune):
-    """One context: (optionally autotuned) whole run to T (time-to-T), then the
-    bench steps at n = Nt/2 with every trace launch of the context timed by HIP
-    events on its stream.  Returns (figures, energies, history, scheduler)."""
+def landau3d_form(conf, precision, steps, warmup, autotune, flops_per_cs, kernel):
+    """One context: (optionally autotuned) whole run to T (time-to-T, the
+    production path: a replayed hipGraph of the steps), then the bench steps at
+    n = Nt/2, each trace launch of which is timed by HIP events on the context
+    stream (the trace kernel's duration is affine in n, so its mean over the
+    run equals its duration at n = Nt/2).  Returns (figures, energies, history,
+    scheduler)."""
     import numericalflowiteration_amd as nufi
     n_mid = conf.Nt // 2
     s = nufi.GpuScheduler(conf, precision=precision)
@@ -97,7 +100,6 @@ def landau3d_form(conf, precision, steps, warmup, autotune):
     if autotune:
         s.run(0, n_mid + 1)
         tuned = s.autotune(n_mid, repeats=2)
-    s.set_timing(True)            # every trace launch from here on: the whole run, the bench steps
     s.synchronize()
     sync()
     t0 = time.perf_counter()
@@ -107,6 +109,7 @@ def landau3d_form(conf, precision, steps, warmup, autotune):
     history = s.history()
     for _ in range(warmup):
         s.step(n_mid, wait=False)
+    s.set_timing(True)            # the trace launches of the timed steps
     s.synchronize()
     sync()                        # device-wide synchronisation on both sides of the timed steps
     t0 = time.perf_counter()
@@ -115,12 +118,14 @@ def landau3d_form(conf, precision, steps, warmup, autotune):
     s.synchronize()
     sync()
     t_steps = time.perf_counter() - t0
+    roof = trace_roofline(s, flops_per_cs, kernel, PEAK_FP64) if steps else None
+    s.set_timing(False)
     cs_run = conf.char_steps(0, conf.Nt + 1)
     cs_step = conf.n_nodes() * n_mid
     r = {"time_to_T_s": t_run, "cs_per_s_time_to_T": cs_run / t_run, "char_steps_per_run": cs_run,
          "bench_step_n": n_mid, "char_steps_per_step": cs_step, "ms_per_step": t_steps / max(steps, 1) * 1e3,
          "cs_per_s_steps": cs_step * steps / t_steps if steps else None,
-         "launch_shape": list(s.launch_shape()),
+         "launch_shape": list(s.launch_shape()), "trace_roofline": roof,
          "autotune_probe": None if tuned is None else {"n": n_mid, "best_ms": tuned[1]}}
     return r, energies, history, s
 
@@ -188,29 +193,37 @@ def cpu_landau3d(conf, hist, s_gpu, threads, budget_points=48):
 
 # ---- 1D two-stream ---------------------------------------------------------------
 
-def timed_runs(conf, precision, runs, warmup, tune_n=None):
-    """`runs` whole runs to T after `warmup` untimed ones (autotuned at tune_n)."""
+def timed_runs(conf, precision, runs, warmup, tune_n=None, graph=True):
+    """`runs` whole runs to T (a replayed hipGraph of the steps unless graph is
+    False) after `warmup` untimed ones (autotuned at tune_n); then one more run
+    with every trace launch timed by HIP events for the kernel figures."""
     import numericalflowiteration_amd as nufi
     with nufi.GpuScheduler(conf, precision=precision) as s:
+        s.set_run_graph(graph)
         for _ in range(warmup):
             s.run()
         if tune_n is not None:
             if warmup == 0:
                 s.run(0, tune_n)
             s.autotune(tune_n, repeats=2)
-        s.set_timing(True)
+            s.run()                                  # the graph of the tuned shape
         s.synchronize()
+        sync()
         t0 = time.perf_counter()
         energies = None
         for _ in range(runs):
             energies = s.run()
         s.synchronize()
+        sync()
         wall = time.perf_counter() - t0
+        s.set_timing(True)
+        s.run()
         roof = trace_roofline(s, FLOPS_1D, "trace1d", PEAK_FP64)
+        s.set_timing(False)
         history = s.history()
         shape = s.launch_shape()
     cs_run = conf.char_steps(0, conf.Nt + 1)
-    return {"runs": runs, "ms_per_run": wall / runs * 1e3, "char_steps_per_run": cs_run,
+    return {"runs": runs, "ms_per_run": wall / runs * 1e3, "char_steps_per_run": cs_run, "graph": graph,
             "cs_per_s_time_to_T": cs_run * runs / wall, "cs_per_s_kernel": roof["cs_per_s_kernel"],
             "trace_avg_ms": roof["avg_launch_ms"], "achieved_tflops": roof["achieved"],
             "launch_shape": list(shape)}, energies, history
@@ -275,10 +288,12 @@ def landau1d(args, threads, with_cpu):
     import numericalflowiteration_amd as nufi
     from tests import oracle_lib as O
     conf = nufi.landau_1d()
-    r64, e64, _ = timed_runs(conf, nufi.FP64, max(args.runs_1d, 5), 1, None if args.no_autotune else conf.Nt)
+    tune = None if args.no_autotune else conf.Nt
+    r64, e64, _ = timed_runs(conf, nufi.FP64, max(args.runs_1d, 5), 1, tune)
+    rd, _, _ = timed_runs(conf, nufi.FP64, max(args.runs_1d, 5), 1, tune, graph=False)
     out = {"workload": "1D Landau, x in [0,4pi), (1+0.01cos(0.5x)) M(v), v in [-10,10], Nx=64 order 4, Nv=256, "
-                       "dt=1/16, T=30 (481 steps)", "fp64": r64}
-    progress(f"1D Landau FP64 {r64['ms_per_run']:.2f} ms per run")
+                       "dt=1/16, T=30 (481 steps)", "fp64": r64, "fp64_launched_per_step": rd}
+    progress(f"1D Landau FP64 {r64['ms_per_run']:.2f} ms per run (graph), {rd['ms_per_run']:.2f} ms launched per step")
     if with_cpu:
         h = O.new_history(conf, conf.Nt + 1)
         t = time.perf_counter()
@@ -662,18 +677,17 @@ def main():
     cpu = parity = None
     e64 = h64 = None
     if args.only in ("fp64", "both"):
-        r64, e64, h64, s64 = landau3d_form(c5, nufi.FP64, args.steps, args.warmup, args.autotune)
+        r64, e64, h64, s64 = landau3d_form(c5, nufi.FP64, args.steps, args.warmup, args.autotune, FLOPS_3D,
+                                           "trace3d_xm_fast_kernel")
         l3["fp64"] = r64
         progress(f"3D Landau FP64: {r64['time_to_T_s']:.2f} s to T, {r64['ms_per_step']:.1f} ms per step n=80")
         if with_cpu and args.case == "all":
             cpu, parity = cpu_landau3d(c5, h64, s64, threads)
             progress("3D Landau CPU baseline and parity done")
-        # every trace launch of this context (run to T, bench steps, parity steps) is timed
-        r64["trace_roofline"] = trace_roofline(s64, FLOPS_3D, "trace3d_xm_fast_kernel", PEAK_FP64)
         s64.close()
     if args.only in ("fp32c", "both"):
-        r32, e32, h32, s32 = landau3d_form(c5, nufi.FP32C, args.steps, args.warmup, args.autotune)
-        r32["trace_roofline"] = trace_roofline(s32, FLOPS_3D, "trace3d_fp32c_kernel", PEAK_FP64)
+        r32, e32, h32, s32 = landau3d_form(c5, nufi.FP32C, args.steps, args.warmup, args.autotune, FLOPS_3D,
+                                           "trace3d_fp32c_kernel")
         s32.close()
         if e64 is not None:
             r32.update(D.run_errors(c5, h32, e32, h64, e64))

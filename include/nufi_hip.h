@@ -164,8 +164,16 @@ int nufi_download_metrics(nufi_ctx *ctx, double *metrics_accum);
  * (poisson.cpp:73-88) — this synchronises; with NULL the step stays async.
  */
 int nufi_step(nufi_ctx *ctx, uint64_t n, double *energy_out);
-/* Steps n_begin..n_end-1; energies (may be NULL) gets n_end-n_begin values. */
+/* Steps n_begin..n_end-1; energies (may be NULL) gets n_end-n_begin values.
+ * The steps' launches are captured into a hipGraph on the first call for a
+ * step range (and launch shape) and replayed on the next ones, unless
+ * nufi_set_run_graph(ctx, 0) or the trace launches are timed
+ * (nufi_set_timing); the results are the same either way. */
 int nufi_run(nufi_ctx *ctx, uint64_t n_begin, uint64_t n_end, double *energies);
+/* No counterpart in the reference (whose driver launches each step from the
+ * host, bin/test_nufi_gpu_1d.cpp:241-253): whether nufi_run replays a graph
+ * (default 1). */
+int nufi_set_run_graph(nufi_ctx *ctx, int enabled);
 /* Host copies of history slices [n_begin, n_end). */
 int nufi_download_history(nufi_ctx *ctx, uint64_t n_begin, uint64_t n_end, double *out);
 int nufi_upload_history(nufi_ctx *ctx, uint64_t n_begin, uint64_t n_end, const double *in);

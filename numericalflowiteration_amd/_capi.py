@@ -61,6 +61,7 @@ SIGNATURES = {
     "nufi_download_metrics": (_int, [_vp, _dp]),
     "nufi_step": (_int, [_vp, _u64, _dp]),
     "nufi_run": (_int, [_vp, _u64, _u64, _dp]),
+    "nufi_set_run_graph": (_int, [_vp, _int]),
     "nufi_download_history": (_int, [_vp, _u64, _u64, _dp]),
     "nufi_upload_history": (_int, [_vp, _u64, _u64, _dp]),
     "nufi_download_energies": (_int, [_vp, _u64, _u64, _dp]),

@@ -64,6 +64,14 @@ struct nufi_ctx
     std::vector<double> host_rho;
     std::string err;
 
+    // nufi_run replays a hipGraph of its step range (captured once per range,
+    // launch shape and variant) unless disabled or the launches are timed
+    bool use_graph = true;
+    hipGraphExec_t run_graph = nullptr;
+    uint64_t graph_b = 0, graph_e = 0;
+    launch_shape graph_shape { 0, 0, 0 };
+    int graph_variant = -1;
+
     bool timing = false;
     std::vector<std::pair<hipEvent_t,hipEvent_t>> ev;
     size_t ev_used = 0;
@@ -119,6 +127,8 @@ void free_all( nufi_ctx *c )
                      c->values, c->scratch_slice, c->metrics, c->mpartials, c->energy_tmp,
                      c->rho_hist, c->lambda, c->gvec, c->grad_part, c->grad, c->tw_x, c->tw_y, c->tw_z,
                      c->mult_step, c->mult_pois, c->mult_fit, c->ewt, c->samp };
+    if ( c->run_graph ) (void)hipGraphExecDestroy( c->run_graph );
+    c->run_graph = nullptr;
     for ( void *p : ptrs ) if ( p ) (void)hipFree( p );
     for ( auto &pr : c->ev ) { (void)hipEventDestroy( pr.first ); (void)hipEventDestroy( pr.second ); }
     c->ev.clear();
@@ -732,11 +742,42 @@ int nufi_run( nufi_ctx *c, uint64_t n_begin, uint64_t n_end, double *energies )
 {
     if ( c == nullptr ) return NUFI_ERR_INVALID;
     if ( n_end < n_begin || ( n_end > 0 && n_end - 1 > c->conf.Nt ) ) return fail( c, NUFI_ERR_RANGE, "run: step range out of range" );
+    if ( n_end > n_begin && !c->field_ok )
+        return fail( c, NUFI_ERR_UNSUPPORTED, "step: grid too large for the single-workgroup field solve" );
     HIP_TRY( c, hipSetDevice( c->device ) );
-    for ( uint64_t n = n_begin; n < n_end; ++n )
+    if ( c->use_graph && !c->timing && n_end > n_begin )
     {
-        int rc = step_async( c, n );
-        if ( rc ) return rc;
+        // the trace and field launches of every step, captured once and replayed
+        // (no memset or event in the captured range: every buffer a step reads
+        // is written by a kernel of that step or an earlier one)
+        const bool same = c->run_graph && c->graph_b == n_begin && c->graph_e == n_end &&
+                          c->graph_variant == c->variant && c->graph_shape.block == c->shape.block &&
+                          c->graph_shape.npt == c->shape.npt && c->graph_shape.chunk == c->shape.chunk;
+        if ( !same )
+        {
+            if ( c->run_graph ) { HIP_TRY( c, hipGraphExecDestroy( c->run_graph ) ); c->run_graph = nullptr; }
+            HIP_TRY( c, hipStreamBeginCapture( c->stream, hipStreamCaptureModeThreadLocal ) );
+            int rc = NUFI_OK;
+            for ( uint64_t n = n_begin; n < n_end && rc == NUFI_OK; ++n ) rc = step_async( c, n );
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture( c->stream, &g );
+            if ( rc ) { if ( g ) (void)hipGraphDestroy( g ); return rc; }
+            if ( e != hipSuccess ) return hip_fail( c, e, "hipStreamEndCapture" );
+            e = hipGraphInstantiate( &c->run_graph, g, nullptr, nullptr, 0 );
+            (void)hipGraphDestroy( g );
+            if ( e != hipSuccess ) { c->run_graph = nullptr; return hip_fail( c, e, "hipGraphInstantiate" ); }
+            c->graph_b = n_begin; c->graph_e = n_end;
+            c->graph_variant = c->variant; c->graph_shape = c->shape;
+        }
+        HIP_TRY( c, hipGraphLaunch( c->run_graph, c->stream ) );
+    }
+    else
+    {
+        for ( uint64_t n = n_begin; n < n_end; ++n )
+        {
+            int rc = step_async( c, n );
+            if ( rc ) return rc;
+        }
     }
     if ( energies && n_end > n_begin )
     {
@@ -1110,6 +1151,13 @@ int nufi_autotune( nufi_ctx *c, uint64_t n_probe, int repeats, int *block, int *
     if ( npt ) *npt = c->shape.npt;
     if ( chunk ) *chunk = c->shape.chunk;
     if ( best_ms ) *best_ms = best;
+    return NUFI_OK;
+}
+
+int nufi_set_run_graph( nufi_ctx *c, int enabled )
+{
+    if ( c == nullptr ) return NUFI_ERR_INVALID;
+    c->use_graph = enabled != 0;
     return NUFI_OK;
 }
 

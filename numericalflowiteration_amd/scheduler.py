@@ -120,6 +120,10 @@ class GpuScheduler:
         self._check(self._lib.nufi_run(self._ctx, n_begin, n_end, _ptr(e) if e.size else None))
         return e
 
+    def set_run_graph(self, enabled: bool):
+        """Whether run() replays a captured hipGraph of its step range (default on)."""
+        self._check(self._lib.nufi_set_run_graph(self._ctx, int(enabled)))
+
     def history(self, n_begin: int = 0, n_end: int | None = None) -> np.ndarray:
         n_end = self.conf.Nt + 1 if n_end is None else n_end
         out = np.zeros((n_end - n_begin) * self.stride_t)
