@@ -570,7 +570,9 @@ def summarize_profiles(outdir, tag, commands):
                   "which for the 1D polynomial step is fewer FLOPs than the algorithmic 30 per char-step. "
                   "LDS bytes are 64 x SQ_INSTS_LDS_{LOAD,STORE}_BANDWIDTH; the L2 hit rate is "
                   "TCC_HIT / (TCC_HIT + TCC_MISS). VALU issue is SQ_INSTS_VALU (wave instructions) per SIMD "
-                  f"({4 * CUS} SIMDs) and clock.", "",
+                  f"({4 * CUS} SIMDs) and clock. The counter passes run the profiled workload alone, the kernel "
+                  "trace the whole command: for kernels other workloads launch too (field_kernel) the rate "
+                  "divides by a mean over all of them and is indicative only.", "",
                   "| kernel | mean us | FP64 TFLOP/s executed | FP32 TFLOP/s executed | VALU instructions per SIMD per clock | LDS load+store TB/s (frac of peak) | LDS bank-conflict / VALU-busy cycles | L2 hit rate |",
                   "|---|---|---|---|---|---|---|---|"]
         for k in sorted(set(pv) | set(pl)):

@@ -172,7 +172,9 @@ int nufi_step(nufi_ctx *ctx, uint64_t n, double *energy_out);
 int nufi_run(nufi_ctx *ctx, uint64_t n_begin, uint64_t n_end, double *energies);
 /* No counterpart in the reference (whose driver launches each step from the
  * host, bin/test_nufi_gpu_1d.cpp:241-253): whether nufi_run replays a graph
- * (default 1). */
+ * (default 1).  A context created with NUFI_RUN_GRAPH=0 in the environment
+ * never replays one, whatever this is set to (graph launches are not traced
+ * by the profiler runs of this build). */
 int nufi_set_run_graph(nufi_ctx *ctx, int enabled);
 /* Host copies of history slices [n_begin, n_end). */
 int nufi_download_history(nufi_ctx *ctx, uint64_t n_begin, uint64_t n_end, double *out);
@@ -280,7 +282,9 @@ int nufi_batch_create(const nufi_config *confs, uint64_t n_runs, int order, int 
                       nufi_batch **out);
 int nufi_batch_destroy(nufi_batch *b);
 const char *nufi_batch_last_error(const nufi_batch *b);
-/* Steps [n_begin, n_end) of every run (asynchronous). */
+/* Steps [n_begin, n_end) of every run (asynchronous); use_graph is ignored
+ * (steps launched one by one) for a batch created with NUFI_RUN_GRAPH=0 in
+ * the environment. */
 int nufi_batch_run(nufi_batch *b, uint64_t n_begin, uint64_t n_end, int use_graph);
 int nufi_batch_synchronize(nufi_batch *b);
 int nufi_batch_energies(nufi_batch *b, uint64_t run, uint64_t n_begin, uint64_t n_end, double *out);

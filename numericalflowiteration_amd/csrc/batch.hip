@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -42,6 +43,7 @@ struct nufi_batch
     size_t trace_lds = 0, field_lds = 0;
 
     hipGraphExec_t graph = nullptr;
+    bool graph_ok = true;      // false when created with NUFI_RUN_GRAPH=0 in the environment
     uint64_t graph_b = 0, graph_e = 0;
     int graph_block = 0, graph_npt = 0, graph_chunk = 0;
     std::string err;
@@ -171,6 +173,10 @@ int nufi_batch_create( const nufi_config *confs, uint64_t n_runs, int order, int
     if ( b == nullptr ) return bfail( nullptr, NUFI_ERR_ALLOC, "batch_create: host allocation" );
     auto bail = [&]( int code ) { std::string m = b->err; bfree_all( b ); delete b; return bfail( nullptr, code, m ); };
     b->device = device; b->order = order; b->precision = precision;
+    {
+        const char *g = std::getenv( "NUFI_RUN_GRAPH" );
+        b->graph_ok = !( g && g[0] == '0' );
+    }
     b->variant = precision == NUFI_FP32C ? TRACE_FP32C : TRACE_FAST;
     b->confs.assign( confs, confs + n_runs );
     b->n_runs = n_runs;
@@ -297,7 +303,7 @@ int nufi_batch_run( nufi_batch *b, uint64_t n_begin, uint64_t n_end, int use_gra
     if ( n_end < n_begin || ( n_end > 0 && n_end - 1 > b->Nt_max ) ) return bfail( b, NUFI_ERR_RANGE, "batch_run: step range" );
     if ( n_end == n_begin ) return NUFI_OK;
     BHIP( b, hipSetDevice( b->device ) );
-    if ( !use_graph )
+    if ( !use_graph || !b->graph_ok )
     {
         for ( uint64_t n = n_begin; n < n_end; ++n ) { int rc = launch_step( b, n ); if ( rc ) return rc; }
         return NUFI_OK;

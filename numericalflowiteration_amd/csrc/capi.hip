@@ -17,6 +17,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -67,6 +68,7 @@ struct nufi_ctx
     // nufi_run replays a hipGraph of its step range (captured once per range,
     // launch shape and variant) unless disabled or the launches are timed
     bool use_graph = true;
+    bool graph_allowed = true;  // false when created with NUFI_RUN_GRAPH=0 in the environment
     hipGraphExec_t run_graph = nullptr;
     uint64_t graph_b = 0, graph_e = 0;
     launch_shape graph_shape { 0, 0, 0 };
@@ -445,6 +447,12 @@ int nufi_create( const nufi_config *conf, const nufi_config *conf_metrics, int o
     c->conf_m = conf_metrics ? *conf_metrics : *conf;
     c->order = order; c->precision = precision; c->device = device; c->dim = conf->dim;
     c->variant = variant;
+    // NUFI_RUN_GRAPH=0 in the environment: nufi_run always launches each step
+    {
+        const char *g = std::getenv( "NUFI_RUN_GRAPH" );
+        c->graph_allowed = !( g && g[0] == '0' );
+        c->use_graph = c->graph_allowed;
+    }
 
     auto bail = [&]( int code ) { std::string m = c->err; free_all( c ); delete c; return fail( nullptr, code, m ); };
 
@@ -1157,7 +1165,7 @@ int nufi_autotune( nufi_ctx *c, uint64_t n_probe, int repeats, int *block, int *
 int nufi_set_run_graph( nufi_ctx *c, int enabled )
 {
     if ( c == nullptr ) return NUFI_ERR_INVALID;
-    c->use_graph = enabled != 0;
+    c->use_graph = enabled != 0 && c->graph_allowed;
     return NUFI_OK;
 }
 
