@@ -32,6 +32,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../nufi_hip.h"
@@ -111,10 +112,14 @@ public:
     gpu_scheduler() = delete;
     gpu_scheduler( const gpu_scheduler& ) = delete;
     gpu_scheduler& operator=( const gpu_scheduler& ) = delete;
-    gpu_scheduler( gpu_scheduler &&rhs ) noexcept: conf_ { rhs.conf_ }, ctx_ { rhs.ctx_ } { rhs.ctx_ = nullptr; }
+    gpu_scheduler( gpu_scheduler &&rhs ) noexcept: conf_ { rhs.conf_ }, ctx_ { rhs.ctx_ }, up_ { std::move( rhs.up_ ) }
+    { rhs.ctx_ = nullptr; }
     gpu_scheduler& operator=( gpu_scheduler &&rhs ) noexcept
     {
-        if ( this != &rhs ) { nufi_destroy( ctx_ ); conf_ = rhs.conf_; ctx_ = rhs.ctx_; rhs.ctx_ = nullptr; }
+        if ( this != &rhs )
+        {
+            nufi_destroy( ctx_ ); conf_ = rhs.conf_; ctx_ = rhs.ctx_; up_ = std::move( rhs.up_ ); rhs.ctx_ = nullptr;
+        }
         return *this;
     }
 
@@ -151,10 +156,13 @@ public:
     {
         if ( std::is_same<real,double>::value )
         { detail::check( nufi_upload_phi( ctx_, n, reinterpret_cast<const double*>( coeffs ) ), ctx_ ); return; }
+        // a float history crosses as doubles: slice n is widened into a
+        // history-sized buffer made once, so each call costs O(stride_t)
+        if ( n > conf_.Nt ) throw std::range_error( "upload_phi: time step out of range" );
         const size_t st = stride_t();
-        std::vector<double> tmp( ( n + 1 ) * st, 0.0 );
-        for ( size_t i = 0; i < st; ++i ) tmp[n*st + i] = double( coeffs[n*st + i] );
-        detail::check( nufi_upload_phi( ctx_, n, tmp.data() ), ctx_ );
+        if ( up_.empty() ) up_.assign( ( conf_.Nt + 1 ) * st, 0.0 );
+        for ( size_t i = 0; i < st; ++i ) up_[n*st + i] = double( coeffs[n*st + i] );
+        detail::check( nufi_upload_phi( ctx_, n, up_.data() ), ctx_ );
     }
 
     // cuda_scheduler.hpp:128-157
@@ -183,6 +191,7 @@ private:
     static int precision() noexcept { return std::is_same<real,float>::value ? NUFI_FP32C : NUFI_FP64; }
     config_t<real> conf_;
     nufi_ctx *ctx_ = nullptr;
+    std::vector<double> up_;       // float: the widened host history upload_phi hands over
 };
 
 } // namespace dim1
@@ -255,10 +264,14 @@ public:
     gpu_scheduler() = delete;
     gpu_scheduler( const gpu_scheduler& ) = delete;
     gpu_scheduler& operator=( const gpu_scheduler& ) = delete;
-    gpu_scheduler( gpu_scheduler &&rhs ) noexcept: conf_ { rhs.conf_ }, ctx_ { rhs.ctx_ } { rhs.ctx_ = nullptr; }
+    gpu_scheduler( gpu_scheduler &&rhs ) noexcept: conf_ { rhs.conf_ }, ctx_ { rhs.ctx_ }, up_ { std::move( rhs.up_ ) }
+    { rhs.ctx_ = nullptr; }
     gpu_scheduler& operator=( gpu_scheduler &&rhs ) noexcept
     {
-        if ( this != &rhs ) { nufi_destroy( ctx_ ); conf_ = rhs.conf_; ctx_ = rhs.ctx_; rhs.ctx_ = nullptr; }
+        if ( this != &rhs )
+        {
+            nufi_destroy( ctx_ ); conf_ = rhs.conf_; ctx_ = rhs.ctx_; up_ = std::move( rhs.up_ ); rhs.ctx_ = nullptr;
+        }
         return *this;
     }
 
@@ -287,10 +300,13 @@ public:
     {
         if ( std::is_same<real,double>::value )
         { detail::check( nufi_upload_phi( ctx_, n, reinterpret_cast<const double*>( coeffs ) ), ctx_ ); return; }
+        // a float history crosses as doubles: slice n is widened into a
+        // history-sized buffer made once, so each call costs O(stride_t)
+        if ( n > conf_.Nt ) throw std::range_error( "upload_phi: time step out of range" );
         const size_t st = stride_t();
-        std::vector<double> tmp( ( n + 1 ) * st, 0.0 );
-        for ( size_t i = 0; i < st; ++i ) tmp[n*st + i] = double( coeffs[n*st + i] );
-        detail::check( nufi_upload_phi( ctx_, n, tmp.data() ), ctx_ );
+        if ( up_.empty() ) up_.assign( ( conf_.Nt + 1 ) * st, 0.0 );
+        for ( size_t i = 0; i < st; ++i ) up_[n*st + i] = double( coeffs[n*st + i] );
+        detail::check( nufi_upload_phi( ctx_, n, up_.data() ), ctx_ );
     }
 
     void compute_metrics( size_t n, size_t q_begin, size_t q_end )
@@ -320,6 +336,7 @@ private:
     static int precision() noexcept { return std::is_same<real,float>::value ? NUFI_FP32C : NUFI_FP64; }
     config_t<real> conf_;
     nufi_ctx *ctx_ = nullptr;
+    std::vector<double> up_;       // float: the widened host history upload_phi hands over
 };
 
 } // namespace dim3

@@ -1,13 +1,14 @@
 """The C++ drop-in header (include/nufi/gpu_scheduler.hpp) compiles with the
-host compiler alone, and its driver (tests/cpp/drop_in_1d.cpp: the reference
-driver's step loop with a host round trip) agrees with the device loop."""
+host compiler alone, and its drivers (tests/cpp/drop_in_1d.cpp and
+drop_in_3d.cpp: the reference drivers' step loops with a host round trip)
+agree with the device loop."""
 import os
 import subprocess
 
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-BIN = os.path.join(ROOT, "tests", "cpp", "_build", "drop_in_1d")
+BUILD = os.path.join(ROOT, "tests", "cpp", "_build")
 
 
 def test_header_compiles_with_gxx(tmp_path):
@@ -21,10 +22,12 @@ def test_header_compiles_with_gxx(tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("dim", ["1d", "3d"])
 @pytest.mark.parametrize("real", ["double", "float"])
-def test_drop_in_driver_runs(real):
-    assert os.path.exists(BIN), "build() compiles tests/cpp first"
-    r = subprocess.run([BIN] + (["float"] if real == "float" else []), capture_output=True, text=True,
+def test_drop_in_driver_runs(dim, real):
+    binary = os.path.join(BUILD, f"drop_in_{dim}")
+    assert os.path.exists(binary), "build() compiles tests/cpp first"
+    r = subprocess.run([binary] + (["float"] if real == "float" else []), capture_output=True, text=True,
                        timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
