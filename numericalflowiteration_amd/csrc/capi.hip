@@ -41,7 +41,9 @@ struct nufi_ctx
     uint32_t wpx = 0;          // wave-task partials per spatial point (strict 1D, 3D)
     uint32_t ppx = 0;          // x-major partials per spatial point (fast; 1D FP32C)
     uint32_t vgroup = 0;       // 3D x-major: velocities per partial
-    int field_ok = 0;
+    int field_ok = 0;          // 1: one-workgroup field solve; 2: the multi-workgroup one; 0: none
+    double2 *lw0 = nullptr, *lw1 = nullptr;   // multi-workgroup solve: two complex buffers
+    double *lpart = nullptr;                  // and its energy partials
     int n_cu = 256;
     size_t lds_cap = 65536;
     launch_shape shape { 256, 1, 1 };
@@ -128,7 +130,7 @@ void free_all( nufi_ctx *c )
     void *ptrs[] = { c->hist, c->dhist, c->qhist, c->qhist32, c->hist32, c->partials, c->rho, c->energies,
                      c->values, c->scratch_slice, c->metrics, c->mpartials, c->energy_tmp,
                      c->rho_hist, c->lambda, c->gvec, c->grad_part, c->grad, c->tw_x, c->tw_y, c->tw_z,
-                     c->mult_step, c->mult_pois, c->mult_fit, c->ewt, c->samp };
+                     c->mult_step, c->mult_pois, c->mult_fit, c->ewt, c->samp, c->lw0, c->lw1, c->lpart };
     if ( c->run_graph ) (void)hipGraphExecDestroy( c->run_graph );
     c->run_graph = nullptr;
     for ( void *p : ptrs ) if ( p ) (void)hipFree( p );
@@ -369,6 +371,14 @@ field_params base_field( nufi_ctx *c )
     return f;
 }
 
+// The field solve of f on the context's stream: one workgroup, or the
+// multi-workgroup passes for grids one workgroup's LDS cannot hold.
+hipError_t run_field( nufi_ctx *c, const field_params &f )
+{
+    if ( c->field_ok == 2 ) return launch_field_large( f, c->lw0, c->lw1, c->lpart, c->stream );
+    return launch_field( f, c->stream );
+}
+
 } // namespace
 
 int nufi_impl::validate_config( const nufi_config *c, int order ) { return validate( c, order ); }
@@ -497,6 +507,14 @@ int nufi_create( const nufi_config *conf, const nufi_config *conf_metrics, int o
     }
     c->field_ok = field_supported( c->dim, uint32_t( conf->Nx ), c->dim == 3 ? uint32_t( conf->Ny ) : 1u,
                                    c->dim == 3 ? uint32_t( conf->Nz ) : 1u );
+    if ( !c->field_ok && ( c->dim == 3 || conf->Nx <= FIELD_LARGE_MAX_1D ) && c->n_space < ( uint64_t( 1 ) << 31 ) )
+    {
+        c->field_ok = 2;
+        if ( ( rc = dev_alloc( c, &c->lw0, c->n_space, "field buffers" ) ) ) return bail( rc );
+        if ( ( rc = dev_alloc( c, &c->lw1, c->n_space, "field buffers" ) ) ) return bail( rc );
+        if ( ( rc = dev_alloc( c, &c->lpart, field_large_partials( uint32_t( c->n_space ) ), "field partials" ) ) )
+            return bail( rc );
+    }
     const uint64_t slices = conf->Nt + 1;
     if ( ( rc = dev_alloc( c, &c->hist, slices * c->st + 8, "history" ) ) ) return bail( rc );
     if ( c->dim == 1 && ( rc = dev_alloc( c, &c->dhist, slices * c->sd + 8, "difference history" ) ) ) return bail( rc );
@@ -727,7 +745,7 @@ static int step_async( nufi_ctx *c, uint64_t n, double *rho_out )
     f.slice32 = c->hist32 ? c->hist32 + n * c->st32 : nullptr;
     f.energy = c->energies + n;
     f.rho_out = rho_out ? rho_out : c->rho;
-    hipError_t e = launch_field( f, c->stream );
+    hipError_t e = run_field( c, f );
     if ( e != hipSuccess ) return hip_fail( c, e, "field kernel launch" );
     return NUFI_OK;
 }
@@ -807,7 +825,11 @@ int nufi_adjoint_grad( nufi_ctx *c, double *dJ_da, int n_par, double *J_out )
     const int npar = adjoint_npar( c->conf.f0 );
     if ( npar == 0 ) return fail( c, NUFI_ERR_UNSUPPORTED, "adjoint_grad: no parameters for this f0 family" );
     if ( n_par < npar || dJ_da == nullptr ) return fail( c, NUFI_ERR_INVALID, "adjoint_grad: output too short" );
-    if ( !c->field_ok || c->conf.Nx > 2048 ) return fail( c, NUFI_ERR_UNSUPPORTED, "adjoint_grad: grid too large" );
+    if ( !c->field_ok ) return fail( c, NUFI_ERR_UNSUPPORTED, "adjoint_grad: grid too large for the field solve" );
+    // one slice's kick-polynomial table and its slice adjoints in LDS
+    if ( size_t( c->sq ) * ( c->precision == NUFI_FP32C ? sizeof(float) : sizeof(double) ) +
+         c->conf.Nx * sizeof(double) + 16 > LDS_LIMIT )
+        return fail( c, NUFI_ERR_UNSUPPORTED, "adjoint_grad: one slice's tables and adjoints exceed the LDS" );
     HIP_TRY( c, hipSetDevice( c->device ) );
     const uint64_t Nt = c->conf.Nt, Nx = c->conf.Nx;
     const uint32_t n_tasks = uint32_t( c->n_space * c->wpx );
@@ -854,10 +876,13 @@ int nufi_adjoint_grad( nufi_ctx *c, double *dJ_da, int n_par, double *J_out )
     ap.u_min = c->conf.u_min; ap.du = c->conf.du; ap.f0 = c->conf.f0;
     const size_t per_slice = size_t( c->sq ) * ( f32 ? sizeof(float) : sizeof(double) ) + Nx * sizeof(double);
     ap.chunk = uint32_t( std::max<size_t>( 1, std::min<size_t>( 32, std::min<size_t>( c->lds_cap, 65536 ) / per_slice ) ) );
+    if ( adjoint1d_lds_bytes( ap ) > LDS_LIMIT )
+        return fail( c, NUFI_ERR_UNSUPPORTED, "adjoint_grad: one slice's tables and adjoints exceed the LDS" );
     for ( uint64_t n = Nt + 1; n-- > 0; )
     {
         af.rho = c->rho_hist + n * Nx; af.lambda = c->lambda + n * Nx;
-        hipError_t e = launch_adjoint_field( af, c->stream );
+        hipError_t e = c->field_ok == 2 ? launch_adjoint_field_large( af, c->lw0, c->lw1, c->stream )
+                                        : launch_adjoint_field( af, c->stream );
         if ( e != hipSuccess ) return hip_fail( c, e, "adjoint field launch" );
         ap.n = n;
         e = launch_adjoint1d( ap, c->stream );
@@ -1030,7 +1055,7 @@ int nufi_poisson_solve( nufi_ctx *c, double *data, double *energy )
     HIP_TRY( c, hipMemcpyAsync( c->values, data, c->n_space * sizeof(double), hipMemcpyHostToDevice, c->stream ) );
     field_params f = base_field( c );
     f.mode = FIELD_POISSON; f.values = c->values; f.mult = c->mult_pois; f.energy = c->energy_tmp;
-    hipError_t e = launch_field( f, c->stream );
+    hipError_t e = run_field( c, f );
     if ( e != hipSuccess ) return hip_fail( c, e, "field kernel launch" );
     HIP_TRY( c, hipMemcpyAsync( data, c->values, c->n_space * sizeof(double), hipMemcpyDeviceToHost, c->stream ) );
     double en = 0;
@@ -1049,7 +1074,7 @@ int nufi_interpolate( nufi_ctx *c, double *slice, const double *values )
     HIP_TRY( c, hipMemcpyAsync( c->values, values, c->n_space * sizeof(double), hipMemcpyHostToDevice, c->stream ) );
     field_params f = base_field( c );
     f.mode = FIELD_FIT; f.values = c->values; f.mult = c->mult_fit; f.slice = c->scratch_slice;
-    hipError_t e = launch_field( f, c->stream );
+    hipError_t e = run_field( c, f );
     if ( e != hipSuccess ) return hip_fail( c, e, "field kernel launch" );
     HIP_TRY( c, hipMemcpyAsync( slice, c->scratch_slice, c->st * sizeof(double), hipMemcpyDeviceToHost, c->stream ) );
     HIP_TRY( c, hipStreamSynchronize( c->stream ) );

@@ -211,6 +211,7 @@ struct adjoint_field_params
 hipError_t launch_adjoint1d( const adjoint1_params &p, hipStream_t st );
 size_t adjoint1d_lds_bytes( const adjoint1_params &p );
 hipError_t launch_adjoint_field( const adjoint_field_params &p, hipStream_t st );
+hipError_t launch_adjoint_field_large( const adjoint_field_params &p, double2 *W0, double2 *W1, hipStream_t st );
 hipError_t launch_reduce_grad( const double *part, uint32_t n_tasks, uint32_t npar, double *grad, hipStream_t st );
 
 // ---- diagnostics samplers (sample1d.hip) -------------------------------------
@@ -270,6 +271,13 @@ struct field_params
 };
 
 hipError_t launch_field( const field_params &p, hipStream_t st );
+// The same solve over many workgroups for grids field_supported refuses
+// (field_large.hip): W0, W1 hold N complex values each, epart
+// field_large_partials(N) reals.
+hipError_t launch_field_large( const field_params &p, double2 *W0, double2 *W1, double *epart, hipStream_t st );
+size_t field_large_partials( uint32_t N );
+// 1D lines up to this length take the multi-workgroup solve (a direct DFT per line)
+constexpr uint64_t FIELD_LARGE_MAX_1D = 32768;
 
 // One run of a batched sweep for the field kernel: f holds the bases of the
 // run's history (slice, dslice at slice 0; energy at step 0).
