@@ -235,7 +235,7 @@ int effective_chunk( const nufi_ctx *c, const launch_shape &s, size_t *lds = nul
 
 // Default launch shapes (the autotuner refines them): 1D x-major 1024 threads
 // x 1 node (16 velocity rows per workgroup), the largest chunk two table
-// buffers in 160 KiB of LDS hold; 3D 512 threads x 4 nodes.
+// buffers in 160 KiB of LDS hold; 3D FP64 1024 threads x 2 nodes, FP32C 512 x 4.
 void default_shape( nufi_ctx *c )
 {
     const uint64_t waves = c->n_space * c->wpx;
@@ -265,9 +265,10 @@ void default_shape( nufi_ctx *c )
         const size_t per = size_t( c->st ) * elem;
         int chunk = int( std::min<size_t>( c->lds_cap, 64 * 1024 ) / per );
         chunk = std::max( 1, std::min( chunk, 1 ) );
+        // the autotuner's picks on 16^3 x 32^3 (two 1-slice buffers each)
         if ( c->variant == TRACE_STRICT ) c->shape = launch_shape { 256, 1, chunk };
-        else if ( c->variant == TRACE_FP32C ) c->shape = launch_shape { 512, 4, 1 };   // two 1-slice buffers
-        else c->shape = launch_shape { 512, 4, chunk };
+        else if ( c->variant == TRACE_FP32C ) c->shape = launch_shape { 512, 4, 1 };
+        else c->shape = launch_shape { 1024, 2, chunk };
     }
 }
 
@@ -1138,7 +1139,7 @@ int nufi_autotune( nufi_ctx *c, uint64_t n_probe, int repeats, int *block, int *
     else
     {
         if ( c->variant == TRACE_STRICT ) cand.push_back( { 256, 1, 1 } );
-        else for ( int b : { 256, 512 } ) for ( int np : npts ) for ( int ch : { 1, 2 } ) cand.push_back( { b, np, ch } );
+        else for ( int b : blocks1 ) for ( int np : npts ) for ( int ch : { 1, 2 } ) cand.push_back( { b, np, ch } );
     }
     const launch_shape saved = c->shape;
     const bool saved_timing = c->timing;

@@ -301,11 +301,13 @@ hipError_t launch_reduce_metrics( const double *partials, uint32_t n_tasks, doub
 hipError_t launch_reduce_partials( const double *partials, uint32_t wpx, double weight,
                                    uint32_t first, uint32_t count, uint32_t n_space,
                                    double *rho, hipStream_t st );
-// The 3D FP32C trace reads the slice as float( c * fp32c3_scale ): the kick
-// -dt/dx over 72 = 2*6*6, the factor of one derivative basis (2 D) and two
-// value bases (6 B) in their integer-coefficient form, so the trace's kick is
-// the gradient itself and no rounded constant scales every node's kick alike.
-inline double fp32c3_scale( double dt, double dx_inv ) { return -dt * dx_inv / 72.0; }
+// The 3D FP32C trace carries its velocities in cells per step (w = dt dx_inv
+// u) and reads the slice as float( c * fp32c3_scale ): the x kick -dt dx_inv
+// times dt dx_inv (velocity to cells per step) over 72 = 2*6*6, the factor of
+// one derivative basis (2 D) and two value bases (6 B) in their
+// integer-coefficient form, so the gradient the trace evaluates is the change
+// of w itself and no rounded constant scales every node's kick alike.
+inline double fp32c3_scale( double dt, double dx_inv ) { return -( dt * dx_inv ) * ( dt * dx_inv ) / 72.0; }
 
 // Differences and the FP64 / FP32 kick-polynomial tables (1D) and the FP32
 // copy (3D FP32C) of an uploaded slice.

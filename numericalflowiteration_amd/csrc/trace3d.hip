@@ -30,10 +30,11 @@
 //              P_k = sum_j A_jk By_j, Q_k = sum_j A_jk Dy_j,
 //              R_k = sum_j X_jk By_j                                  (48 FMA)
 //              Ex ~ sum_k R_k Bz_k, Ey ~ sum_k Q_k Bz_k, Ez ~ sum_k P_k Dz_k (12)
-//   fp32c  : as fast in FP32 from an FP32 copy of the slice, positions and
-//            velocities as hi + lo pairs with TwoSum/TwoProd updates; f0 and
-//            the rho sums in FP64.  The copy is scaled in FP64 by the kick
-//            over 72 before its one rounding and the bases are 6B and 2D, so
+//   fp32c  : as fast in FP32 from an FP32 copy of the slice, positions (cell
+//            units) and velocities (cells per step, w = dt dx_inv u) as hi + lo
+//            pairs with TwoSum updates; f0 and the rho sums in FP64.  The copy
+//            is scaled in FP64 by the kick in cells per step over 72 before
+//            its one rounding and the bases are 6B and 2D, so
 //            no FP32-rounded constant (1/6, -dt/dx) multiplies the kicks: such
 //            a constant biases every node's kick by the same relative error,
 //            which the sum over velocities does not average away (for 3D
@@ -437,8 +438,10 @@ struct xm3_nodes
     }
 };
 
+// 1024-thread workgroups run four waves per SIMD (128 VGPRs: the f0 evaluation
+// after each velocity pass spills, the step loop does not); 512 threads, two
 template <int BLOCK, int NPT, bool MASK>
-__global__ __launch_bounds__( BLOCK )
+__global__ __launch_bounds__( BLOCK ) __attribute__(( amdgpu_waves_per_eu( BLOCK >= 1024 ? 4 : 1 ) ))
 void trace3d_xm_fast_kernel( trace3_params p )
 {
     extern __shared__ double lds[];
@@ -518,17 +521,15 @@ __device__ __forceinline__ void rewrap32( float &sh, float &sl, float N, float i
     sh = y2; sl = sl + e2;
 }
 
-// drift with TwoProd/TwoSum, wrap (unless MASK, see advance_axis), locate
+// drift s -= w (cells per step) with TwoSum, wrap (unless MASK, see
+// advance_axis), locate
 template <bool MASK>
-__device__ __forceinline__ int advance_axis32( float &sh, float &sl, float uh, float ul,
-                                               float cdh, float cdl, float N, float invN, int Ni, float &t )
+__device__ __forceinline__ int advance_axis32( float &sh, float &sl, float wh, float wl,
+                                               float N, float invN, int Ni, float &t )
 {
-    const float pr = cdh * uh;
-    float pe = fmaf( cdh, uh, -pr );
-    pe = fmaf( cdh, ul, fmaf( cdl, uh, pe ) );
     float y, e;
-    two_sum( sh, -pr, y, e );
-    e = e + ( sl - pe );
+    two_sum( sh, -wh, y, e );
+    e = e + ( sl - wl );
     fast_two_sum( y, e, sh, sl );
     if ( !MASK ) rewrap32( sh, sl, N, invN );
     const float k = floorf( sh );
@@ -561,13 +562,15 @@ void trace3d_fp32c_kernel( trace3_params p )
 
     node3 id[NPT];
     float xh[NPT], xl[NPT], yh[NPT], yl[NPT], zh[NPT], zl[NPT];
-    float uh[NPT], ul[NPT], vh[NPT], vl[NPT], wh[NPT], wl[NPT];
+    float uh[NPT], ul[NPT], vh[NPT], vl[NPT], wh[NPT], wl[NPT];     // velocities in cells per step
+    const double cdx = p.dt * p.dx_inv, cdy = p.dt * p.dy_inv, cdz = p.dt * p.dz_inv;
     #pragma unroll
     for ( int r = 0; r < NPT; ++r )
     {
         id[r] = make_node3<WPB,NPT>( p, r );
         uint32_t ix, iy, iz; double u, v, w;
         node_coords( p, id[r], ix, iy, iz, u, v, w );
+        u *= cdx; v *= cdy; w *= cdz;
         xh[r] = float( ix ); xl[r] = 0.0f;
         yh[r] = float( iy ); yl[r] = 0.0f;
         zh[r] = float( iz ); zl[r] = 0.0f;
@@ -578,13 +581,10 @@ void trace3d_fp32c_kernel( trace3_params p )
     const float Nxf = p.Nx, Nyf = p.Ny, Nzf = p.Nz;
     const float iNx = 1.0f / Nxf, iNy = 1.0f / Nyf, iNz = 1.0f / Nzf;
     const int Nxi = int( p.Nx ), Nyi = int( p.Ny ), Nzi = int( p.Nz );
-    const double cdx = p.dt * p.dx_inv, cdy = p.dt * p.dy_inv, cdz = p.dt * p.dz_inv;
-    const float cxh = float( cdx ), cxl = float( cdx - double( cxh ) );
-    const float cyh = float( cdy ), cyl = float( cdy - double( cyh ) );
-    const float czh = float( cdz ), czl = float( cdz - double( czh ) );
-    // the slices carry the x kick (fp32c3_scale); y and z scale it by
-    // dy_inv/dx_inv and dz_inv/dx_inv, split hi + lo (exactly 1 + 0 on cubic cells)
-    const double ry = p.dy_inv / p.dx_inv, rz = p.dz_inv / p.dx_inv;
+    // the slices carry the x kick in cells per step (fp32c3_scale); y and z
+    // scale it by (dy_inv/dx_inv)^2 and (dz_inv/dx_inv)^2, split hi + lo
+    // (exactly 1 + 0 on cubic cells)
+    const double ry = ( p.dy_inv / p.dx_inv ) * ( p.dy_inv / p.dx_inv ), rz = ( p.dz_inv / p.dx_inv ) * ( p.dz_inv / p.dx_inv );
     const float ryh = float( ry ), ryl = float( ry - double( ryh ) );
     const float rzh = float( rz ), rzl = float( rz - double( rzh ) );
     const uint32_t st = p.st32, sy = p.sy, sz = p.sz, C = p.chunk;
@@ -641,9 +641,9 @@ void trace3d_fp32c_kernel( trace3_params p )
             for ( int r = 0; r < NPT; ++r )
             {
                 float tx, ty, tz;
-                const int i = advance_axis32<MASK>( xh[r], xl[r], uh[r], ul[r], cxh, cxl, Nxf, iNx, Nxi, tx );
-                const int j = advance_axis32<MASK>( yh[r], yl[r], vh[r], vl[r], cyh, cyl, Nyf, iNy, Nyi, ty );
-                const int k = advance_axis32<MASK>( zh[r], zl[r], wh[r], wl[r], czh, czl, Nzf, iNz, Nzi, tz );
+                const int i = advance_axis32<MASK>( xh[r], xl[r], uh[r], ul[r], Nxf, iNx, Nxi, tx );
+                const int j = advance_axis32<MASK>( yh[r], yl[r], vh[r], vl[r], Nyf, iNy, Nyi, ty );
+                const int k = advance_axis32<MASK>( zh[r], zl[r], wh[r], wl[r], Nzf, iNz, Nzi, tz );
                 float Bx[4], Dx[4], By[4], Dy[4], Bz[4], Dz[4];
                 cubic_bases_x6( tx, Bx, Dx );
                 cubic_bases_x6( ty, By, Dy );
@@ -667,8 +667,8 @@ void trace3d_fp32c_kernel( trace3_params p )
             const double x = p.x_min + ( double( xh[r] ) + double( xl[r] ) ) * p.dx;
             const double y = p.y_min + ( double( yh[r] ) + double( yl[r] ) ) * p.dy;
             const double z = p.z_min + ( double( zh[r] ) + double( zl[r] ) ) * p.dz;
-            f = f0_3d( p.f0, x, y, z, double( uh[r] ) + double( ul[r] ), double( vh[r] ) + double( vl[r] ),
-                       double( wh[r] ) + double( wl[r] ) );
+            f = f0_3d( p.f0, x, y, z, ( double( uh[r] ) + double( ul[r] ) ) / cdx,
+                       ( double( vh[r] ) + double( vl[r] ) ) / cdy, ( double( wh[r] ) + double( wl[r] ) ) / cdz );
         }
         write_partial3( p, id[r], f );
     }
@@ -872,9 +872,7 @@ bool trace3d_shape_ok( int variant, int order, const launch_shape &s )
     if ( order != 4 || s.chunk < 1 ) return false;
     if ( variant == TRACE_STRICT )
         return s.npt == 1 && ( s.block == 64 || s.block == 256 );
-    // 1024-thread workgroups cap a wave at 128 VGPRs, and these kernels then
-    // spill to scratch: not built
-    return ( s.block == 256 || s.block == 512 ) && ( s.npt == 1 || s.npt == 2 || s.npt == 4 );
+    return ( s.block == 256 || s.block == 512 || s.block == 1024 ) && ( s.npt == 1 || s.npt == 2 || s.npt == 4 );
 }
 
 static uint64_t visited3( const trace3_params &p )
@@ -935,6 +933,7 @@ hipError_t launch_trace3d( const trace3_params &p_in, int variant, const launch_
     {
     case 256:  return launch3_npt<256> ( p, variant, s.npt, grid, lds, st );
     case 512:  return launch3_npt<512> ( p, variant, s.npt, grid, lds, st );
+    case 1024: return launch3_npt<1024>( p, variant, s.npt, grid, lds, st );
     default:   return hipErrorInvalidValue;
     }
 }
