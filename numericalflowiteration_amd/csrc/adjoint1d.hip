@@ -20,13 +20,14 @@
 // same slice at the same time) and is flushed with global atomics once per
 // chunk of slices; its summation order therefore varies at the rounding level.
 //
-// FP32C contexts differentiate the FP32-compensated trace: the forward sweep
-// repeats the FP32C kernel's steps (the shared fp32c_* helpers of step1d.hpp,
-// so the foot points are those of the forward run), the reverse sweep undoes
-// them on the same pairs (u_b = u_a - h q(s_a), s_b = s_a + cd u_b, both with
-// TwoSum), and the adjoints sbar, ubar, lambda and the parameter gradients are
-// FP64, as is the field adjoint.  The FP32 tables hold kick * (Q0, Q1, Q2), so
-// pbar = h kick ubar and sbar += h ubar dq/dt.
+// FP32C contexts differentiate the FP32-compensated trace, whose velocity w is
+// in cells per step (w = cd u): the forward sweep repeats the FP32C kernel's
+// steps (the shared fp32c_* helpers of step1d.hpp, so the foot points are
+// those of the forward run), the reverse sweep undoes them on the same pairs
+// (w_b = w_a - h q(s_a), s_b = s_a + w_b, both with TwoSum), and the adjoints
+// sbar, wbar, lambda and the parameter gradients are FP64, as is the field
+// adjoint.  The FP32 tables hold kick cd (Q0, Q1, Q2), so pbar = h kick cd wbar,
+// sbar += h wbar dq/dt, wbar -= sbar, and the seed is wbar = fbar df0/du / cd.
 #include "device_common.hpp"
 #include "fft_lds.hpp"
 #include "internal.hpp"
@@ -221,13 +222,12 @@ void adjoint1d_fp32c_kernel( adjoint1_params p )
     const int Nx = int( p.Nx ), mask = Nx - 1;
     const float Nxf = float( p.Nx ), invNx = 1.0f / Nxf;
     const double cd = p.dt * p.dx_inv, kick = -0.5 * p.dt * p.dx_inv;
-    const float cd_hi = float( cd ), cd_lo = float( cd - double( cd_hi ) );
     const uint32_t sq = p.stride_q, q2 = p.q2_off;
 
     // the FP32C trace kernel's start state (trace1d_fp32c_body)
-    const double u0 = p.u_min + iu*p.du + p.du/2;
+    const double w0 = cd * ( p.u_min + iu*p.du + p.du/2 );
     float sh = float( ix ), sl = 0.0f;
-    float uh = float( u0 ), ul = float( u0 - double( uh ) );
+    float uh = float( w0 ), ul = float( w0 - double( uh ) );
 
     // forward: slices n-1 .. 0
     for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
@@ -241,7 +241,7 @@ void adjoint1d_fp32c_kernel( adjoint1_params p )
         {
             const int64_t m = mlo + jj;
             if ( POW2 && poly_wrap_due( m ) ) fp32c_wrap( sh, sl, Nxf, invNx );
-            fp32c_drift( sh, sl, uh, ul, cd_hi, cd_lo );
+            fp32c_move( sh, sl, uh, ul );
             if ( !POW2 ) fp32c_wrap( sh, sl, Nxf, invNx );
             float t;
             const int i = fp32c_locate<POW2>( sh, sl, mask, t );
@@ -252,9 +252,10 @@ void adjoint1d_fp32c_kernel( adjoint1_params p )
     // seed at the foot
     const double fbar = active ? -p.du * p.g[ix] : 0.0;
     double fx, fu, fa[NUFI_MAX_MODES];
-    f0_grad_1d( p.f0, p.x_min + ( double( sh ) + double( sl ) ) * p.dx, double( uh ) + double( ul ), fx, fu, fa, p.npar );
+    f0_grad_1d( p.f0, p.x_min + ( double( sh ) + double( sl ) ) * p.dx, ( double( uh ) + double( ul ) ) / cd,
+                fx, fu, fa, p.npar );
     double sbar = fbar * fx * p.dx;     // d/ds = dx d/dx
-    double ubar = fbar * fu;
+    double ubar = fbar * fu / cd;       // d/dw = d/du / cd
     #pragma unroll
     for ( int m = 0; m < NUFI_MAX_MODES; ++m )
     {
@@ -283,13 +284,13 @@ void adjoint1d_fp32c_kernel( adjoint1_params p )
             // dq/dt of the scaled table, in FP64
             const float2 q01 = reinterpret_cast<const float2*>( Q )[ i + 1 ];
             const double dqdt = fma( 2.0 * double( Q[ q2 + ( i + 1 ) ] ), double( t ), double( q01.y ) );
-            const double pbar = double( hf ) * kick * ubar;
+            const double pbar = double( hf ) * kick * cd * ubar;
             sbar = fma( double( hf ) * ubar, dqdt, sbar );
             if ( active ) scatter_poly_adjoint( lam + jj * p.Nx, i, Nx, pbar, double( t ) );
             // undo the kick, then the drift
             fp32c_kick( uh, ul, -( hf * q ) );
-            ubar = fma( -cd, sbar, ubar );
-            fp32c_drift( sh, sl, -uh, -ul, cd_hi, cd_lo );
+            ubar = ubar - sbar;
+            fp32c_move( sh, sl, -uh, -ul );
             if ( !POW2 || poly_wrap_due( m ) ) fp32c_wrap( sh, sl, Nxf, invNx );
         }
         __syncthreads();

@@ -30,7 +30,7 @@ struct nufi_batch
 
     double *hist = nullptr, *dhist = nullptr, *partials = nullptr, *energies = nullptr;
     double *qhist = nullptr;       // FP64: every run's ready kick-polynomial tables (stride sq)
-    float *qhist32 = nullptr;      // FP32C: the same in FP32, scaled by the run's kick
+    float *qhist32 = nullptr;      // FP32C: the same in FP32, scaled by the run's fp32c1_scale
     double2 *tw = nullptr, *mult = nullptr;
     double *ewt = nullptr;
     batch_run *d_runs = nullptr;
@@ -263,7 +263,7 @@ int nufi_batch_create( const nufi_config *confs, uint64_t n_runs, int order, int
         f.dslice = b->dhist + b->off_dhist[r];
         f.qslice = b->qhist ? b->qhist + b->off_q[r] : nullptr;
         f.qslice32 = b->qhist32 ? b->qhist32 + b->off_q[r] : nullptr;
-        f.q2_off = poly_q2( uint32_t( c.Nx ) ); f.kick = -0.5 * c.dt * c.dx_inv;
+        f.q2_off = poly_q2( uint32_t( c.Nx ) ); f.kick = fp32c1_scale( c.dt, c.dx_inv );
         f.energy = b->energies + b->off_en[r];
         f.sy = b->st[r]; f.sz = b->st[r]; f.stride_d = b->sd[r];
         fields[r].Nt = c.Nt; fields[r].st = b->st[r]; fields[r].sd = b->sd[r]; fields[r].sq = b->sq[r];

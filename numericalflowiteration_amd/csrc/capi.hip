@@ -340,11 +340,11 @@ int launch_trace( nufi_ctx *c, uint64_t n, uint64_t q_begin, uint64_t q_end, uin
     return NUFI_OK;
 }
 
-// The scale the field and slice-table kernels apply to the per-slice tables:
-// 1D the kick polynomials' -dt/(2 dx); 3D the FP32 slice copy's fp32c3_scale.
+// The scale the field and slice-table kernels apply to the FP32 per-slice
+// tables (FP64 tables are not scaled): 1D fp32c1_scale, 3D fp32c3_scale.
 double table_kick( const nufi_ctx *c )
 {
-    if ( c->dim == 1 ) return -0.5 * c->conf.dt * c->conf.dx_inv;
+    if ( c->dim == 1 ) return fp32c1_scale( c->conf.dt, c->conf.dx_inv );
     return fp32c3_scale( c->conf.dt, c->conf.dx_inv );
 }
 

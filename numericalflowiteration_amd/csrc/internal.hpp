@@ -61,7 +61,7 @@ struct trace1_params
 {
     const double *dhist;     // extended differences of slice m at dhist + m*stride_d (FP64)
     const double *qhist;     // FP64 rho: kick-polynomial tables, slice m at qhist + m*stride_q, LDS layout
-    const float  *qhist32;   // FP32C rho: the same, FP32 tables scaled by the kick
+    const float  *qhist32;   // FP32C rho: the same, FP32 tables scaled by fp32c1_scale
     uint32_t stride_q, q2_off;        // LDS kick-polynomial table: slice stride, Q2 row (poly_stride, poly_q2)
     double   kick;                    // -dt/(2 dx), the scale of the kick polynomials
     uint32_t pipe;                    // x-major: tables double-buffered (poly_ready_pipeline)
@@ -264,7 +264,7 @@ struct field_params
     double  *qslice;             // 1D FP64 kick-polynomial table of slice n (poly_stride layout) or null
     float   *qslice32;           // 1D FP32 table of slice n, scaled by kick, or null
     uint32_t q2_off;             // their Q2 row (poly_q2)
-    double   kick;               // 1D: -dt/(2 dx); 3D: the FP32 slice scale fp32c3_scale
+    double   kick;               // the FP32 tables' scale: 1D fp32c1_scale, 3D fp32c3_scale
     float   *slice32;            // FP32 coefficients times kick (3D FP32C) or null
     double  *energy;             // energies[n]
     uint32_t sy, sz, stride_d;
@@ -301,6 +301,11 @@ hipError_t launch_reduce_metrics( const double *partials, uint32_t n_tasks, doub
 hipError_t launch_reduce_partials( const double *partials, uint32_t wpx, double weight,
                                    uint32_t first, uint32_t count, uint32_t n_space,
                                    double *rho, hipStream_t st );
+// The 1D FP32C trace carries its velocities in cells per step (w = dt dx_inv
+// u): its FP32 kick-polynomial tables are scaled in FP64 by the kick
+// -dt dx_inv / 2 times dt dx_inv before their one rounding.
+inline double fp32c1_scale( double dt, double dx_inv ) { return -0.5 * ( dt * dx_inv ) * ( dt * dx_inv ); }
+
 // The 3D FP32C trace carries its velocities in cells per step (w = dt dx_inv
 // u) and reads the slice as float( c * fp32c3_scale ): the x kick -dt dx_inv
 // times dt dx_inv (velocity to cells per step) over 72 = 2*6*6, the factor of

@@ -105,18 +105,17 @@ __device__ __forceinline__ void poly_step( double &s, double &u, const double *Q
 }
 
 // ---- the FP32-compensated step ---------------------------------------------
-// Position and velocity as unevaluated pairs (hi + lo).  The FP32C trace kernel
-// and the FP32C adjoint share these, so both reach the same states bit for bit.
+// Position (cell units) and velocity (cells per step, w = dt dx_inv u) as
+// unevaluated pairs (hi + lo); the FP32 tables are scaled to give the change
+// of w.  The FP32C trace kernel and the FP32C adjoint share these, so both
+// reach the same states bit for bit.
 
-// drift s -= cd*u with TwoProd on the leading product (cd = cd_hi + cd_lo)
-__device__ __forceinline__ void fp32c_drift( float &sh, float &sl, float uh, float ul, float cd_hi, float cd_lo )
+// drift s -= w: a TwoSum of the leading parts, the trailing parts added to its error
+__device__ __forceinline__ void fp32c_move( float &sh, float &sl, float wh, float wl )
 {
-    const float pr = cd_hi * uh;
-    float pe = fmaf( cd_hi, uh, -pr );
-    pe = fmaf( cd_hi, ul, fmaf( cd_lo, uh, pe ) );
     float y, e;
-    two_sum( sh, -pr, y, e );
-    e = e + ( sl - pe );
+    two_sum( sh, -wh, y, e );
+    e = e + ( sl - wl );
     fast_two_sum( y, e, sh, sl );
 }
 
@@ -139,14 +138,15 @@ __device__ __forceinline__ int fp32c_locate( float sh, float sl, int mask, float
     return POW2 ? ( int( k ) & mask ) : int( k );
 }
 
-// the kick q = kick * (Q0 + Q1 t + Q2 t^2) from the FP32 table (pairs at Q, Q2 row at Q + q2)
+// the kick q = kick dt dx_inv (Q0 + Q1 t + Q2 t^2) in cells per step from the
+// FP32 table (pairs at Q, Q2 row at Q + q2)
 __device__ __forceinline__ float fp32c_poly( const float *Q, uint32_t q2, int i, float t )
 {
     const float2 q01 = reinterpret_cast<const float2*>( Q )[ i + 1 ];
     return fmaf( fmaf( Q[ q2 + ( i + 1 ) ], t, q01.y ), t, q01.x );
 }
 
-// u += q with TwoSum
+// w += q with TwoSum
 __device__ __forceinline__ void fp32c_kick( float &uh, float &ul, float q )
 {
     float y, e;

@@ -383,21 +383,20 @@ __device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint
     constexpr int WPB = BLOCK / WAVE;
     const xm_tile tile = make_tile( p, wg );
     xm_node id[NPT];
-    float sh[NPT], sl[NPT], uh[NPT], ul[NPT];
+    float sh[NPT], sl[NPT], uh[NPT], ul[NPT];       // position (cells), velocity (cells per step)
+    const double cd = p.dt * p.dx_inv;
     #pragma unroll
     for ( int r = 0; r < NPT; ++r )
     {
         id[r] = make_xm_node<WPB,NPT>( p, tile, r );
-        const double u0 = p.nu_min + id[r].iu*p.ndu + p.ndu/2;
+        const double w0 = cd * ( p.nu_min + id[r].iu*p.ndu + p.ndu/2 );
         sh[r] = float( id[r].ix ); sl[r] = 0.0f;
-        uh[r] = float( u0 );       ul[r] = float( u0 - double( uh[r] ) );
+        uh[r] = float( w0 );       ul[r] = float( w0 - double( uh[r] ) );
     }
 
     const float Nxf   = float( p.Nx );
     const float invNx = 1.0f / Nxf;
     const int mask    = int( p.Nx ) - 1;
-    const double cd   = p.dt * p.dx_inv;
-    const float cd_hi = float( cd ), cd_lo = float( cd - double( cd_hi ) );
     const uint32_t sq = p.stride_q, q2 = p.q2_off;
 
     auto wrap = [&]( int r ) { fp32c_wrap( sh[r], sl[r], Nxf, invNx ); };
@@ -406,7 +405,7 @@ __device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint
         #pragma unroll
         for ( int r = 0; r < NPT; ++r )
         {
-            fp32c_drift( sh[r], sl[r], uh[r], ul[r], cd_hi, cd_lo );
+            fp32c_move( sh[r], sl[r], uh[r], ul[r] );
             // exact periodic wrap into [0, Nx); POW2: the cell index wraps by
             // the mask and the position is re-wrapped every POLY_WRAP slices
             if ( !POW2 ) wrap( r );
@@ -443,7 +442,7 @@ __device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint
     for ( int r = 0; r < NPT; ++r )
     {
         const double s = double( sh[r] ) + double( sl[r] );
-        const double u = double( uh[r] ) + double( ul[r] );
+        const double u = ( double( uh[r] ) + double( ul[r] ) ) / cd;
         f[r] = id[r].active ? f0_1d( p.f0, p.x_min + s*p.dx, u ) : 0.0;
     }
     xm_reduce<BLOCK,NPT>( p, tile, id, f, reinterpret_cast<double*>( ldsf ) );
