@@ -240,7 +240,8 @@ def twostream(args, threads, with_cpu):
     r32.update(D.run_errors(conf, h32, e32, h64, e64))
     r32["speedup_vs_fp64"] = r32["cs_per_s_time_to_T"] / r64["cs_per_s_time_to_T"]
     out = {"workload": "1D two-stream, x in [0,10pi), (1+0.001cos(0.2x)) v^2 M(v), v in [-8,8], Nx=256 order 4, "
-                       "Nv=1024, dt=1/32, T=50 (1601 steps)", "fp64": r64, "fp32c": r32}
+                       "Nv=1024, dt=1/32, T=50 (1601 steps)", "fp64": r64, "fp32c": r32,
+           "host_round_trip": host_round_trip(conf, e64)}
     progress(f"two-stream FP64 {r64['ms_per_run']:.1f} ms, FP32C {r32['ms_per_run']:.1f} ms per run")
     if with_cpu:
         from tests import oracle_lib as O
@@ -279,6 +280,34 @@ def twostream(args, threads, with_cpu):
                                    "note": "mean-free rho, GPU FP64 vs the oracle on the same history"}
         progress("two-stream CPU baseline done")
     return out
+
+
+def host_round_trip(conf, e_dev):
+    """The reference driver's step loop through the C ABI (bin/test_nufi_gpu_1d.cpp:
+    241-253): compute_rho -> download_rho (PCIe) -> Poisson and fit on host
+    buffers (each a copy in and out) -> upload_phi (PCIe), driven from Python;
+    the whole run to T, its energies against the device-resident run."""
+    import numpy as np
+    import numericalflowiteration_amd as nufi
+    st = conf.stride_t()
+    coeffs = np.zeros((conf.Nt + 1) * st)
+    e = np.zeros(conf.Nt + 1)
+    with nufi.GpuScheduler(conf) as s:
+        rho = np.zeros(conf.n_space())
+        t0 = time.perf_counter()
+        for n in range(conf.Nt + 1):
+            rho[:] = 0.0
+            s.compute_rho(n)
+            s.download_rho(rho)
+            e[n] = s.poisson_solve(rho)
+            coeffs[n * st:(n + 1) * st] = s.interpolate(rho)
+            s.upload_phi(n, coeffs)
+        wall = time.perf_counter() - t0
+    cs = conf.char_steps(0, conf.Nt + 1)
+    return {"ms_per_run": wall * 1e3, "cs_per_s_time_to_T": cs / wall,
+            "energy_max_rel_err_vs_device_loop": float(np.max(np.abs(e - e_dev) / np.abs(e_dev))),
+            "note": "host buffers cross PCIe every step (rho down, slice up) and each call synchronises; "
+                    "never the headline value"}
 
 
 # ---- 1D Landau ----------------------------------------------------------------------
