@@ -268,7 +268,7 @@ void default_shape( nufi_ctx *c )
         // the autotuner's picks on 16^3 x 32^3 (two 1-slice buffers each)
         if ( c->variant == TRACE_STRICT ) c->shape = launch_shape { 256, 1, chunk };
         else if ( c->variant == TRACE_FP32C ) c->shape = launch_shape { 512, 4, 1 };
-        else c->shape = launch_shape { 1024, 2, chunk };
+        else c->shape = launch_shape { 512, 4, chunk };
     }
 }
 
@@ -677,14 +677,7 @@ int nufi_compute_metrics( nufi_ctx *c, uint64_t n, uint64_t q_begin, uint64_t q_
             p.Nu = uint32_t( g.Nu ); p.Nv = uint32_t( g.Nv ); p.Nw = uint32_t( g.Nw );
             p.sy = c->sy; p.sz = c->sz; p.st = c->st; p.wpx = c->m_wpx;
             p.l_first = uint32_t( l0 ); p.n_tasks = uint32_t( tasks ); p.order = uint32_t( c->order );
-            p.x_min = g.x_min; p.y_min = g.y_min; p.z_min = g.z_min;
-            p.dx = g.dx; p.dy = g.dy; p.dz = g.dz;
-            p.dx_inv = g.dx_inv; p.dy_inv = g.dy_inv; p.dz_inv = g.dz_inv;
-            p.Lx = g.Lx; p.Ly = g.Ly; p.Lz = g.Lz;
-            p.Lx_inv = g.Lx_inv; p.Ly_inv = g.Ly_inv; p.Lz_inv = g.Lz_inv;
-            p.u_min = g.u_min; p.v_min = g.v_min; p.w_min = g.w_min;
-            p.du = g.du; p.dv = g.dv; p.dw = g.dw; p.dt = g.dt;
-            p.f0 = g.f0;
+            fill_trace3_geometry( p, g );
             e = launch_trace3d( p, variant, launch_shape { 256, 1, 1 }, c->stream );
         }
         if ( e != hipSuccess ) return hip_fail( c, e, "metrics trace launch" );

@@ -118,6 +118,7 @@ struct trace3_params
     double x_min, y_min, z_min, dx, dy, dz, dx_inv, dy_inv, dz_inv;
     double Lx, Ly, Lz, Lx_inv, Ly_inv, Lz_inv;
     double u_min, v_min, w_min, du, dv, dw, dt;
+    double kick72[3], kick72h[3]; // fast FP64: -dt d_inv / 72 per axis (72-scaled gradient), and half of it
     nufi_f0 f0;
 };
 
@@ -131,6 +132,12 @@ inline void fill_trace3_geometry( trace3_params &p, const nufi_config &g )
     p.Lx_inv = g.Lx_inv; p.Ly_inv = g.Ly_inv; p.Lz_inv = g.Lz_inv;
     p.u_min = g.u_min; p.v_min = g.v_min; p.w_min = g.w_min;
     p.du = g.du; p.dv = g.dv; p.dw = g.dw; p.dt = g.dt;
+    const double dinv[3] = { g.dx_inv, g.dy_inv, g.dz_inv };
+    for ( int a = 0; a < 3; ++a )
+    {
+        p.kick72[a]  = -g.dt * dinv[a] / 72.0;
+        p.kick72h[a] = 0.5 * p.kick72[a];
+    }
     p.f0 = g.f0;
 }
 

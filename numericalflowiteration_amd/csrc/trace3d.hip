@@ -156,24 +156,33 @@ __device__ __forceinline__ void node_coords( const trace3_params &p, const node3
 // ---- closed-form cubic B-spline bases on [0,1) ------------------------------
 // values  B = ((1-t)^3, 3t^3-6t^2+4, -3t^3+3t^2+3t+1, t^3)/6
 // slopes  D = (-(1-t)^2, 3t^2-4t, -3t^2+2t+1, t^2)/2
-template <typename T>
-__device__ __forceinline__ void cubic_bases( T t, T B[4], T D[4] )
+// Both kernels evaluate them times 6 (values) and 2 (slopes): integer-
+// coefficient polynomials, with -(1-t)^2 and t^2 left to the contraction as a
+// negation and a reuse.  The gradient from these bases is 72 times the
+// gradient in reference-cell units: the FP64 kick folds 1/72 into its FP64
+// constant; the FP32C kernel reads slices pre-scaled by fp32c3_scale, so its
+// gradient is the kick itself.
+//
+// FP64 form: every constant is one the FP64 VALU takes inline (1, 2, 4, -4),
+// so none occupies registers; the second value and slope come from the first
+// by the symmetry B_2(t) = B_1(1-t), D_2(t) = -D_1(1-t):
+//   B_1 = 3t^2 (t-2) + 4,  D_1 = 3t^2 - 4t,  3t^2 = 2t^2 + t^2.
+__device__ __forceinline__ void cubic_bases_x6( double t, double B[4], double D[4] )
 {
-    const T a = T(1) - t, t2 = t*t, a2 = a*a;
-    const T sixth = T(1) / T(6);
-    B[0] = a2 * a * sixth;
-    B[3] = t2 * t * sixth;
-    B[1] = fma( t2, fma( T(0.5), t, T(-1) ), T(2) / T(3) );   // 2/3 - t^2 + t^3/2
-    B[2] = ( ( T(1) - B[0] ) - B[1] ) - B[3];
-    D[0] = T(-0.5) * a2;
-    D[3] = T(0.5) * t2;
-    D[1] = t * fma( T(1.5), t, T(-2) );                         // 1.5 t^2 - 2 t
-    D[2] = ( -D[0] - D[1] ) - D[3];
+    const double a = 1.0 - t, t2 = t*t, a2 = a*a;
+    const double t23 = fma( 2.0, t2, t2 ), a23 = fma( 2.0, a2, a2 );
+    B[0] = a2 * a;
+    B[3] = t2 * t;
+    B[1] = fma( t23, t - 2.0, 4.0 );
+    B[2] = fma( a23, a - 2.0, 4.0 );
+    D[0] = -a2;
+    D[3] = t2;
+    D[1] = fma( -4.0, t, t23 );
+    D[2] = fma( 4.0, a, -a23 );
 }
 
-// The same bases times 6 (values) and 2 (slopes): integer-coefficient
-// polynomials whose constants are exact in FP32.  The FP32C trace uses them on
-// slices pre-scaled by fp32c3_scale, so its gradient is the kick itself.
+// FP32 form (32-bit literals are free): the third value and slope close the
+// partition of unity and the zero sum of the slopes.
 __device__ __forceinline__ void cubic_bases_x6( float t, float B[4], float D[4] )
 {
     const float a = 1.0f - t, t2 = t*t, a2 = a*a;
@@ -184,34 +193,67 @@ __device__ __forceinline__ void cubic_bases_x6( float t, float B[4], float D[4] 
     D[0] = -a2;
     D[3] = t2;
     D[1] = t * fmaf( 3.0f, t, -4.0f );                           // 3t^2 - 4t
-    D[2] = ( -D[0] - D[1] ) - D[3];
+    D[2] = ( a2 - D[1] ) - t2;
 }
 
-// Shared-basis gradient in reference-cell units: g = (dphi/dtx, dphi/dty, dphi/dtz).
+// Shared-basis gradient of the 4x4x4 window at c (72 times the gradient in
+// reference-cell units with the bases above): per row (j,k)
+//   A_jk = sum_i c_ijk Bx_i,  X_jk = sum_i c_ijk Dx_i,
+// per plane k  P_k = sum_j A_jk By_j, Q_k = sum_j A_jk Dy_j, R_k = sum_j X_jk By_j,
+// and g = (sum_k R_k Bz_k, sum_k Q_k Bz_k, sum_k P_k Dz_k).
+//
+// The window gather is software-pipelined (FP64 and FP32C kernels): the 16 rows
+// (4 values each) of the 4x4x4 window are read PF rows ahead of the FMAs that
+// use them, and the first PF rows before the bases are computed, so each row's
+// LDS latency runs under earlier work.  The scheduling barriers keep the
+// compiler from sinking the reads back next to their uses (it does so to save
+// registers, and the waves a SIMD holds then cannot cover the latency).
+// Measured at 512 x 4 on 3D Landau (FP64): PF = 1 is the fastest, 2 and 3 no
+// faster, 4 slower.
+constexpr int GATHER_PF = 1;
+
 template <typename T>
-__device__ __forceinline__ void grad_window( const T *__restrict__ c, uint32_t sy, uint32_t sz,
-                                             const T Bx[4], const T Dx[4], const T By[4], const T Dy[4],
-                                             const T Bz[4], const T Dz[4], T &gx, T &gy, T &gz )
+__device__ __forceinline__ void load_row( const T *__restrict__ c, int idx, uint32_t sy, uint32_t sz, T r[4] )
 {
+    const T *row = c + ( idx >> 2 ) * sz + ( idx & 3 ) * sy;
+    r[0] = row[0]; r[1] = row[1]; r[2] = row[2]; r[3] = row[3];
+}
+
+// the first PF rows of the window at c, before the bases are computed
+template <int PF, typename T>
+__device__ __forceinline__ void grad_prefetch( const T *__restrict__ c, uint32_t sy, uint32_t sz, T buf[PF + 1][4] )
+{
+    #pragma unroll
+    for ( int d = 0; d < PF; ++d ) load_row( c, d, sy, sz, buf[d] );
+    __builtin_amdgcn_sched_barrier( 0 );
+}
+
+template <int PF, typename T>
+__device__ __forceinline__ void grad_rows( const T *__restrict__ c, uint32_t sy, uint32_t sz, T buf[PF + 1][4],
+                                           const T Bx[4], const T Dx[4], const T By[4],
+                                           const T Dy[4], const T Bz[4], const T Dz[4],
+                                           T &gx, T &gy, T &gz )
+{
+    T P = T(0), Q = T(0), R = T(0);
     gx = T(0); gy = T(0); gz = T(0);
     #pragma unroll
-    for ( int k = 0; k < 4; ++k )
+    for ( int idx = 0; idx < 16; ++idx )
     {
-        T P = T(0), Q = T(0), R = T(0);
-        #pragma unroll
-        for ( int j = 0; j < 4; ++j )
+        if ( idx + PF < 16 ) load_row( c, idx + PF, sy, sz, buf[ ( idx + PF ) % ( PF + 1 ) ] );
+        __builtin_amdgcn_sched_barrier( 0 );
+        const T *r = buf[ idx % ( PF + 1 ) ];
+        const int j = idx & 3, k = idx >> 2;
+        const T A = fma( r[3], Bx[3], fma( r[2], Bx[2], fma( r[1], Bx[1], r[0]*Bx[0] ) ) );
+        const T X = fma( r[3], Dx[3], fma( r[2], Dx[2], fma( r[1], Dx[1], r[0]*Dx[0] ) ) );
+        P = j == 0 ? A * By[0] : fma( A, By[j], P );
+        Q = j == 0 ? A * Dy[0] : fma( A, Dy[j], Q );
+        R = j == 0 ? X * By[0] : fma( X, By[j], R );
+        if ( j == 3 )
         {
-            const T *row = c + k*sz + j*sy;
-            const T c0 = row[0], c1 = row[1], c2 = row[2], c3 = row[3];
-            const T A = fma( c3, Bx[3], fma( c2, Bx[2], fma( c1, Bx[1], c0*Bx[0] ) ) );
-            const T X = fma( c3, Dx[3], fma( c2, Dx[2], fma( c1, Dx[1], c0*Dx[0] ) ) );
-            P = fma( A, By[j], P );
-            Q = fma( A, Dy[j], Q );
-            R = fma( X, By[j], R );
+            gx = fma( R, Bz[k], gx );
+            gy = fma( Q, Bz[k], gy );
+            gz = fma( P, Dz[k], gz );
         }
-        gx = fma( R, Bz[k], gx );
-        gy = fma( Q, Bz[k], gy );
-        gz = fma( P, Dz[k], gz );
     }
 }
 
@@ -249,9 +291,11 @@ __device__ __forceinline__ void rewrap3( double &s, double N, double invN )
 
 // one fast FP64 characteristic step of one node with slice C: drift (unless
 // `still`), exact wrap, E = -grad phi from the shared-basis window, kick by hf*dt
+// (`half`: hf = 1/2, the kick constants are selected per slice, not multiplied)
 struct fast3_consts
 {
-    double Nxd, Nyd, Nzd, iNx, iNy, iNz, cdx, cdy, cdz, kx, ky, kz;
+    double Nxd, Nyd, Nzd, iNx, iNy, iNz, cdx, cdy, cdz;
+    double kx, ky, kz, kxh, kyh, kzh;     // -dt d_inv / 72 (times 72-scaled gradients), and half of it
     int Nxi, Nyi, Nzi;
     uint32_t sy, sz;
 };
@@ -263,28 +307,32 @@ __device__ __forceinline__ fast3_consts make_fast3( const trace3_params &p )
     k.iNx = 1.0 / k.Nxd; k.iNy = 1.0 / k.Nyd; k.iNz = 1.0 / k.Nzd;
     k.Nxi = int( p.Nx ); k.Nyi = int( p.Ny ); k.Nzi = int( p.Nz );
     k.cdx = p.dt * p.dx_inv; k.cdy = p.dt * p.dy_inv; k.cdz = p.dt * p.dz_inv;
-    k.kx = -p.dt * p.dx_inv; k.ky = -p.dt * p.dy_inv; k.kz = -p.dt * p.dz_inv;
+    k.kx = p.kick72[0]; k.ky = p.kick72[1]; k.kz = p.kick72[2];           // kernel arguments: scalar registers
+    k.kxh = p.kick72h[0]; k.kyh = p.kick72h[1]; k.kzh = p.kick72h[2];
     k.sy = p.sy; k.sz = p.sz;
     return k;
 }
 
-template <bool MASK = false>
-__device__ __forceinline__ void step3_fast( const fast3_consts &k, const double *C, double hf, bool still,
+template <bool MASK = false, int PF = GATHER_PF>
+__device__ __forceinline__ void step3_fast( const fast3_consts &k, const double *C, bool half, bool still,
                                             double &sx, double &sy_, double &sz_, double &vu, double &vv, double &vw )
 {
     double tx, ty, tz;
     const int i = advance_axis<MASK>( sx,  vu, k.cdx, k.Nxd, k.iNx, k.Nxi, tx, still );
     const int j = advance_axis<MASK>( sy_, vv, k.cdy, k.Nyd, k.iNy, k.Nyi, ty, still );
     const int l = advance_axis<MASK>( sz_, vw, k.cdz, k.Nzd, k.iNz, k.Nzi, tz, still );
+    const double *W = C + l*k.sz + j*k.sy + i;
+    double buf[PF + 1][4];
+    grad_prefetch<PF>( W, k.sy, k.sz, buf );
     double Bx[4], Dx[4], By[4], Dy[4], Bz[4], Dz[4];
-    cubic_bases( tx, Bx, Dx );
-    cubic_bases( ty, By, Dy );
-    cubic_bases( tz, Bz, Dz );
+    cubic_bases_x6( tx, Bx, Dx );
+    cubic_bases_x6( ty, By, Dy );
+    cubic_bases_x6( tz, Bz, Dz );
     double gx, gy, gz;
-    grad_window( C + l*k.sz + j*k.sy + i, k.sy, k.sz, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
-    vu = fma( hf * k.kx, gx, vu );
-    vv = fma( hf * k.ky, gy, vv );
-    vw = fma( hf * k.kz, gz, vw );
+    grad_rows<PF>( W, k.sy, k.sz, buf, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
+    vu = fma( half ? k.kxh : k.kx, gx, vu );
+    vv = fma( half ? k.kyh : k.ky, gy, vv );
+    vw = fma( half ? k.kzh : k.kz, gz, vw );
 }
 
 template <int BLOCK, int NPT, int MODE>
@@ -319,10 +367,10 @@ void trace3d_fast_kernel( trace3_params p )
             const double *C = lds + size_t( jj ) * st;
             const uint64_t m = uint64_t( mlo + jj );
             const bool initial = ( MODE == MODE_METRICS ) && m == p.n;     // rho.hpp:386-394
-            const double hf = ( m == 0 || initial ) ? 0.5 : 1.0;
+            const bool half = m == 0 || initial;
             #pragma unroll
             for ( int r = 0; r < NPT; ++r )
-                step3_fast( k3, C, hf, initial, sx[r], sy_[r], sz_[r], vu[r], vv[r], vw[r] );
+                step3_fast( k3, C, half, initial, sx[r], sy_[r], sz_[r], vu[r], vv[r], vw[r] );
         }
     }
 
@@ -403,13 +451,14 @@ struct xm3_nodes
         }
     }
 
-    // slices mlo+cnt-1 .. mlo with the chunk's slices at Cb + jj*st
+    // slices mlo+cnt-1 .. mlo with the chunk's slices at Cb + jj*st; slice 0
+    // (the half kick) is peeled off the loop
     __device__ __forceinline__ void steps( const fast3_consts &k3, const double *Cb, uint32_t st, int64_t mlo, int cnt )
     {
-        for ( int jj = cnt - 1; jj >= 0; --jj )
+        const int jlast = mlo == 0 ? 1 : 0;
+        for ( int jj = cnt - 1; jj >= jlast; --jj )
         {
             const double *C = Cb + size_t( jj ) * st;
-            const double hf = ( mlo + jj == 0 ) ? 0.5 : 1.0;
             if ( MASK && wrap3_due( mlo + jj ) )
             {
                 #pragma unroll
@@ -420,7 +469,13 @@ struct xm3_nodes
             }
             #pragma unroll
             for ( int r = 0; r < NPT; ++r )
-                step3_fast<MASK>( k3, C, hf, false, sx[r], sy[r], sz[r], vu[r], vv[r], vw[r] );
+                step3_fast<MASK>( k3, C, false, false, sx[r], sy[r], sz[r], vu[r], vv[r], vw[r] );
+        }
+        if ( mlo == 0 )                  // slice 0: wrap3_due(0) is false
+        {
+            #pragma unroll
+            for ( int r = 0; r < NPT; ++r )
+                step3_fast<MASK>( k3, Cb, true, false, sx[r], sy[r], sz[r], vu[r], vv[r], vw[r] );
         }
     }
 
@@ -440,6 +495,7 @@ struct xm3_nodes
 
 // 1024-thread workgroups run four waves per SIMD (128 VGPRs: the f0 evaluation
 // after each velocity pass spills, the step loop does not); 512 threads, two
+// (about 230 VGPRs, no spills; with four nodes a lane, the default shape)
 template <int BLOCK, int NPT, bool MASK>
 __global__ __launch_bounds__( BLOCK ) __attribute__(( amdgpu_waves_per_eu( BLOCK >= 1024 ? 4 : 1 ) ))
 void trace3d_xm_fast_kernel( trace3_params p )
@@ -644,12 +700,15 @@ void trace3d_fp32c_kernel( trace3_params p )
                 const int i = advance_axis32<MASK>( xh[r], xl[r], uh[r], ul[r], Nxf, iNx, Nxi, tx );
                 const int j = advance_axis32<MASK>( yh[r], yl[r], vh[r], vl[r], Nyf, iNy, Nyi, ty );
                 const int k = advance_axis32<MASK>( zh[r], zl[r], wh[r], wl[r], Nzf, iNz, Nzi, tz );
+                const float *W = Cs + k*sz + j*sy + i;
+                float buf[GATHER_PF + 1][4];
+                grad_prefetch<GATHER_PF>( W, sy, sz, buf );
                 float Bx[4], Dx[4], By[4], Dy[4], Bz[4], Dz[4];
                 cubic_bases_x6( tx, Bx, Dx );
                 cubic_bases_x6( ty, By, Dy );
                 cubic_bases_x6( tz, Bz, Dz );
                 float gx, gy, gz;
-                grad_window( Cs + k*sz + j*sy + i, sy, sz, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
+                grad_rows<GATHER_PF>( W, sy, sz, buf, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
                 kick32( uh[r], ul[r], hf * gx );
                 kick32( vh[r], vl[r], fmaf( gy, hyh, gy * hyl ) );
                 kick32( wh[r], wl[r], fmaf( gz, hzh, gz * hzl ) );
