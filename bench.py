@@ -41,6 +41,7 @@ FLOPS_3D = 475.0     # 3D
 PEAK_FP64 = 78.6     # TFLOP/s, MI355X FP64 vector
 PEAK_FP32 = 157.3    # TFLOP/s, MI355X FP32 vector
 CUS, CLOCK_GHZ = 256, 2.4
+SHADER_ENGINES = 32  # 8 XCDs x 4; SQ_BUSY_CYCLES counts per shader engine
 LDS_PEAK_TBS = 128 * CUS * CLOCK_GHZ * 1e9 / 1e12   # 128 B per clock per CU
 PROFILE_KERNEL = "trace"                              # summarize_profiles: the kernel whose traffic is kept
 
@@ -598,12 +599,14 @@ def summarize_profiles(outdir, tag, commands):
                   "rate is 64 times the counter (a wave has 64 lanes); it counts what the kernel executes, "
                   "which for the 1D polynomial step is fewer FLOPs than the algorithmic 30 per char-step. "
                   "LDS bytes are 64 x SQ_INSTS_LDS_{LOAD,STORE}_BANDWIDTH; the L2 hit rate is "
-                  "TCC_HIT / (TCC_HIT + TCC_MISS). VALU issue is SQ_INSTS_VALU (wave instructions) per SIMD "
-                  f"({4 * CUS} SIMDs) and clock. The counter passes run the profiled workload alone, the kernel "
+                  "TCC_HIT / (TCC_HIT + TCC_MISS). The shader clock is SQ_BUSY_CYCLES per shader engine "
+                  f"({SHADER_ENGINES}) over the mean duration (the box's clock while the kernel ran; the peaks "
+                  f"assume {CLOCK_GHZ} GHz); VALU issue is SQ_INSTS_VALU (wave instructions) per SIMD "
+                  f"({4 * CUS} SIMDs) and shader clock. The counter passes run the profiled workload alone, the kernel "
                   "trace the whole command: for kernels other workloads launch too (field_kernel) the rate "
                   "divides by a mean over all of them and is indicative only.", "",
-                  "| kernel | mean us | FP64 TFLOP/s executed | FP32 TFLOP/s executed | VALU instructions per SIMD per clock | LDS load+store TB/s (frac of peak) | LDS bank-conflict / VALU-busy cycles | L2 hit rate |",
-                  "|---|---|---|---|---|---|---|---|"]
+                  "| kernel | mean us | shader clock GHz | FP64 TFLOP/s executed | FP32 TFLOP/s executed | VALU instructions per SIMD per clock | LDS load+store TB/s (frac of peak) | LDS bank-conflict / VALU-busy cycles | L2 hit rate |",
+                  "|---|---|---|---|---|---|---|---|---|"]
         for k in sorted(set(pv) | set(pl)):
             if "nufi" not in k or k not in dur:
                 continue
@@ -613,15 +616,17 @@ def summarize_profiles(outdir, tag, commands):
             f32 = 64.0 * c.get("SQ_INSTS_VALU_FLOPS_FP32", 0.0) / ( us * 1e-6 ) / 1e12
             lds_b = 64.0 * ( c.get("SQ_INSTS_LDS_LOAD_BANDWIDTH", 0.0) + c.get("SQ_INSTS_LDS_STORE_BANDWIDTH", 0.0) )
             lds = lds_b / ( us * 1e-6 ) / 1e12
-            ipc = c.get("SQ_INSTS_VALU", 0.0) / ( 4 * CUS * us * 1e-6 * CLOCK_GHZ * 1e9 )
+            ghz = c.get("SQ_BUSY_CYCLES", 0.0) / SHADER_ENGINES / ( us * 1e-6 ) / 1e9
+            ipc = c.get("SQ_INSTS_VALU", 0.0) / ( 4 * CUS * us * 1e-6 * ( ghz if ghz > 0 else CLOCK_GHZ ) * 1e9 )
             busy = c.get("SQ_ACTIVE_INST_VALU", 0.0)
             conf = c.get("SQ_LDS_BANK_CONFLICT", 0.0) / busy if busy else float("nan")
             h = pl.get(k, {})
             hit, miss = h.get("TCC_HIT_sum", 0.0), h.get("TCC_MISS_sum", 0.0)
             hr = hit / ( hit + miss ) if hit + miss else float("nan")
-            lines.append(f"| `{k[:70]}` | {us:.1f} | {f64:.2f} | {f32:.2f} | {ipc:.3f} | "
+            lines.append(f"| `{k[:70]}` | {us:.1f} | {ghz:.2f} | {f64:.2f} | {f32:.2f} | {ipc:.3f} | "
                          f"{lds:.2f} ({lds / LDS_PEAK_TBS:.3f}) | {conf:.3f} | {hr:.4f} |")
-            result.setdefault("counters", {})[k] = {"mean_us": us, "fp64_tflops": f64, "fp32_tflops": f32,
+            result.setdefault("counters", {})[k] = {"mean_us": us, "shader_clock_ghz": ghz,
+                                                    "fp64_tflops": f64, "fp32_tflops": f32,
                                                     "valu_insts_per_simd_clock": ipc,
                                                     "lds_tbs": lds, "lds_frac": lds / LDS_PEAK_TBS,
                                                     "l2_hit_rate": hr}
