@@ -181,19 +181,30 @@ __device__ __forceinline__ void cubic_bases_x6( double t, double B[4], double D[
     D[2] = fma( 4.0, a, -a23 );
 }
 
-// FP32 form (32-bit literals are free): the third value and slope close the
-// partition of unity and the zero sum of the slopes.
+// FP32 form.  The four slopes must sum to exactly zero as the FP32 numbers
+// they are: a gradient contraction sum_i c_i D_i then carries no term in the
+// level of the coefficients, only in their differences.  Rounded slopes leave
+// a residual r(t) = sum_i D_i of about 2^-24 that multiplies the local level
+// of phi, not its gradient; r depends on t alone, so it does not average out
+// over cells or velocities, and on 3D Landau it gave 2e-6 relative energy
+// error where the field passes near zero.  So (1-t)^2, t^2 and 3t^2 - 4t are
+// each rounded once onto multiples of 2^-23 (an FMA that adds 1 or -1, which
+// puts the result at magnitude [1,2] or a little above, then that 1 taken back
+// exactly), and the third slope (a^2 - t^2) - D_1, whose partial results stay
+// below 2 in magnitude, is exact.
 __device__ __forceinline__ void cubic_bases_x6( float t, float B[4], float D[4] )
 {
-    const float a = 1.0f - t, t2 = t*t, a2 = a*a;
+    const float a = 1.0f - t;
+    const float a2 = fmaf( a, a, 1.0f ) - 1.0f;                  // (1-t)^2 on the 2^-23 grid
+    const float t2 = fmaf( t, t, 1.0f ) - 1.0f;                  // t^2 on the 2^-23 grid
     B[0] = a2 * a;
     B[3] = t2 * t;
     B[1] = fmaf( t2, fmaf( 3.0f, t, -6.0f ), 4.0f );            // 3t^3 - 6t^2 + 4
     B[2] = ( ( 6.0f - B[0] ) - B[1] ) - B[3];
     D[0] = -a2;
     D[3] = t2;
-    D[1] = t * fmaf( 3.0f, t, -4.0f );                           // 3t^2 - 4t
-    D[2] = ( a2 - D[1] ) - t2;
+    D[1] = fmaf( t, fmaf( 3.0f, t, -4.0f ), -1.0f ) + 1.0f;     // 3t^2 - 4t on the 2^-23 grid
+    D[2] = ( a2 - t2 ) - D[1];                                   // exact
 }
 
 // Shared-basis gradient of the 4x4x4 window at c (72 times the gradient in
