@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include <string>
 #include <vector>
@@ -119,6 +120,7 @@ struct trace3_params
     double Lx, Ly, Lz, Lx_inv, Ly_inv, Lz_inv;
     double u_min, v_min, w_min, du, dv, dw, dt;
     double kick72[3], kick72h[3]; // fast FP64: -dt d_inv / 72 per axis (72-scaled gradient), and half of it
+    float f32_p0[3], f32_w0[3];   // FP32C: offsets of the position and velocity hi parts per axis
     nufi_f0 f0;
 };
 
@@ -133,10 +135,25 @@ inline void fill_trace3_geometry( trace3_params &p, const nufi_config &g )
     p.u_min = g.u_min; p.v_min = g.v_min; p.w_min = g.w_min;
     p.du = g.du; p.dv = g.dv; p.dw = g.dw; p.dt = g.dt;
     const double dinv[3] = { g.dx_inv, g.dy_inv, g.dz_inv };
+    const double vlo[3] = { g.u_min, g.v_min, g.w_min }, dv[3] = { g.du, g.dv, g.dw };
+    const double nv[3] = { double( g.Nu ), double( g.Nv ), double( g.Nw ) }, nx[3] = { double( g.Nx ), double( g.Ny ), double( g.Nz ) };
     for ( int a = 0; a < 3; ++a )
     {
         p.kick72[a]  = -g.dt * dinv[a] / 72.0;
         p.kick72h[a] = 0.5 * p.kick72[a];
+        // FP32C frames: W0 = 1.5 * 2^j and P0 = 1.5 * 2^k with 2^(j-1) >= wcap (a
+        // bound on |velocity| in cells per step, twice the grid's with one cell
+        // to spare for the kicks) and 2^(k-1) >= N + 17 wcap + 2 (a wrapped
+        // position plus the drift of the 16 slices between re-wraps): W0 + w and
+        // P0 + s then stay in one binade, [2^j, 2^(j+1)) and [2^k, 2^(k+1)), and
+        // P0 is a multiple of N when N is a power of two
+        const double vmax = std::max( std::fabs( vlo[a] ), std::fabs( vlo[a] + nv[a] * dv[a] ) );
+        const double wcap = 2.0 * g.dt * dinv[a] * vmax + 1.0;
+        int j = 1, k = 1;
+        while ( std::ldexp( 1.0, j - 1 ) < wcap ) ++j;
+        while ( std::ldexp( 1.0, k - 1 ) < nx[a] + 17.0 * wcap + 2.0 ) ++k;
+        p.f32_w0[a] = float( 1.5 * std::ldexp( 1.0, j ) );
+        p.f32_p0[a] = float( 1.5 * std::ldexp( 1.0, k ) );
     }
     p.f0 = g.f0;
 }

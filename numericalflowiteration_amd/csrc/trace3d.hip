@@ -578,44 +578,61 @@ void trace3d_xm_fast_kernel( trace3_params p )
 }
 
 // ---- FP32 compensated --------------------------------------------------------
-// Exact periodic wrap of the pair sh + sl: sh - N*w is exact (Sterbenz) for
-// w >= 1 and for w <= -2, and Fast2Sum is exact for w = -1 (|N| >= |sh|).
-__device__ __forceinline__ void rewrap32( float &sh, float &sl, float N, float invN )
+// Positions (cells) and velocities (cells per step) are unevaluated pairs
+// hi + lo whose hi parts carry a per-axis offset, P0 + s and W0 + w, chosen on
+// the host so that every hi value of the run stays inside one binade.  Then the
+// offsets come off exactly, the hi parts dominate every increment, so the drift
+// and the kick are ordered Fast2Sums (3 operations, exact) rather than TwoSums
+// (6), and a wrap by a multiple of N is exact.  The drift leaves its rounding in
+// the lo part unnormalised; both pairs are renormalised every WRAP3 slices.
+struct frame32
 {
-    const float w = floorf( sh * invN );
-    float y2, e2;
-    fast_two_sum( -N * w, sh, y2, e2 );
-    sh = y2; sl = sl + e2;
-}
+    float P0, W0, N, invN;
+    int Ni, P0i;
+};
 
-// drift s -= w (cells per step) with TwoSum, wrap (unless MASK, see
-// advance_axis), locate
+// drift s -= w, wrap into [P0, P0 + N) (unless MASK: P0 is then a multiple of
+// N, the cell index wraps by the mask and frame32_rewrap wraps the position
+// every WRAP3 slices), locate
 template <bool MASK>
-__device__ __forceinline__ int advance_axis32( float &sh, float &sl, float wh, float wl,
-                                               float N, float invN, int Ni, float &t )
+__device__ __forceinline__ int drift_locate32( const frame32 &f, float &sh, float &sl, float wh, float wl, float &t )
 {
-    float y, e;
-    two_sum( sh, -wh, y, e );
-    e = e + ( sl - wl );
-    fast_two_sum( y, e, sh, sl );
-    if ( !MASK ) rewrap32( sh, sl, N, invN );
+    const float d = wh - f.W0;                       // exact: wh and W0 share a binade
+    const float y = sh - d;                          // Fast2Sum, |sh| > |d|
+    const float e = ( sh - y ) - d;
+    sl = ( sl - wl ) + e;
+    sh = y;
+    if ( !MASK ) sh = fmaf( -f.N, floorf( ( sh - f.P0 ) * f.invN ), sh );   // exact
     const float k = floorf( sh );
     t = ( sh - k ) + sl;
     int i = int( k );
-    if ( MASK ) i &= Ni - 1;
+    if ( MASK ) i &= f.Ni - 1;
     else
     {
-        i = ( i < 0 ) ? i + Ni : i;
-        i = ( i >= Ni ) ? i - Ni : i;
+        i -= f.P0i;
+        i = ( i < 0 ) ? i + f.Ni : i;
+        i = ( i >= f.Ni ) ? i - f.Ni : i;
     }
     return i;
 }
 
-__device__ __forceinline__ void kick32( float &uh, float &ul, float q )
+// every WRAP3 slices: wrap (MASK) and renormalise the position and velocity pairs
+template <bool MASK>
+__device__ __forceinline__ void frame32_rewrap( const frame32 &f, float &sh, float &sl, float &wh, float &wl )
 {
-    float y, e;
-    two_sum( uh, q, y, e );
-    uh = y; ul = ul + e;
+    if ( MASK ) sh = fmaf( -f.N, floorf( ( sh - f.P0 ) * f.invN ), sh );      // exact
+    float s = sh + sl;
+    sl = sl - ( s - sh ); sh = s;
+    s = wh + wl;
+    wl = wl - ( s - wh ); wh = s;
+}
+
+// w += q: Fast2Sum, |wh| >= W0 / 2 > |q|
+__device__ __forceinline__ void kick32( float &wh, float &wl, float q )
+{
+    const float s = wh + q;
+    wl = wl + ( q - ( s - wh ) );
+    wh = s;
 }
 
 // velocity-major: node r of a lane is make_node3's (wave task g, lane), the
@@ -628,26 +645,35 @@ void trace3d_fp32c_kernel( trace3_params p )
     constexpr int WPB = BLOCK / WAVE;
 
     node3 id[NPT];
-    float xh[NPT], xl[NPT], yh[NPT], yl[NPT], zh[NPT], zl[NPT];
-    float uh[NPT], ul[NPT], vh[NPT], vl[NPT], wh[NPT], wl[NPT];     // velocities in cells per step
+    float xh[NPT], xl[NPT], yh[NPT], yl[NPT], zh[NPT], zl[NPT];     // positions (cells) + P0
+    float uh[NPT], ul[NPT], vh[NPT], vl[NPT], wh[NPT], wl[NPT];     // velocities (cells per step) + W0
     const double cdx = p.dt * p.dx_inv, cdy = p.dt * p.dy_inv, cdz = p.dt * p.dz_inv;
+    frame32 fx, fy, fz;
+    {
+        const uint32_t Ns[3] = { p.Nx, p.Ny, p.Nz };
+        frame32 *fs[3] = { &fx, &fy, &fz };
+        #pragma unroll
+        for ( int a = 0; a < 3; ++a )
+        {
+            fs[a]->P0 = p.f32_p0[a]; fs[a]->W0 = p.f32_w0[a];
+            fs[a]->N = float( Ns[a] ); fs[a]->invN = 1.0f / fs[a]->N;
+            fs[a]->Ni = int( Ns[a] ); fs[a]->P0i = int( p.f32_p0[a] );
+        }
+    }
     #pragma unroll
     for ( int r = 0; r < NPT; ++r )
     {
         id[r] = make_node3<WPB,NPT>( p, r );
         uint32_t ix, iy, iz; double u, v, w;
         node_coords( p, id[r], ix, iy, iz, u, v, w );
-        u *= cdx; v *= cdy; w *= cdz;
-        xh[r] = float( ix ); xl[r] = 0.0f;
-        yh[r] = float( iy ); yl[r] = 0.0f;
-        zh[r] = float( iz ); zl[r] = 0.0f;
+        u = double( fx.W0 ) + u * cdx; v = double( fy.W0 ) + v * cdy; w = double( fz.W0 ) + w * cdz;
+        xh[r] = fx.P0 + float( ix ); xl[r] = 0.0f;                   // exact
+        yh[r] = fy.P0 + float( iy ); yl[r] = 0.0f;
+        zh[r] = fz.P0 + float( iz ); zl[r] = 0.0f;
         uh[r] = float( u ); ul[r] = float( u - double( uh[r] ) );
         vh[r] = float( v ); vl[r] = float( v - double( vh[r] ) );
         wh[r] = float( w ); wl[r] = float( w - double( wh[r] ) );
     }
-    const float Nxf = p.Nx, Nyf = p.Ny, Nzf = p.Nz;
-    const float iNx = 1.0f / Nxf, iNy = 1.0f / Nyf, iNz = 1.0f / Nzf;
-    const int Nxi = int( p.Nx ), Nyi = int( p.Ny ), Nzi = int( p.Nz );
     // the slices carry the x kick in cells per step (fp32c3_scale); y and z
     // scale it by (dy_inv/dx_inv)^2 and (dz_inv/dx_inv)^2, split hi + lo
     // (exactly 1 + 0 on cubic cells)
@@ -696,21 +722,23 @@ void trace3d_fp32c_kernel( trace3_params p )
             const float *Cs = cur + size_t( jj ) * st;
             const float hf = ( mlo + jj == 0 ) ? 0.5f : 1.0f;          // a power of two: hf * q is exact
             const float hyh = hf * ryh, hyl = hf * ryl, hzh = hf * rzh, hzl = hf * rzl;
-            if ( MASK && wrap3_due( mlo + jj ) )
+            if ( wrap3_due( mlo + jj ) )
             {
                 #pragma unroll
                 for ( int r = 0; r < NPT; ++r )
                 {
-                    rewrap32( xh[r], xl[r], Nxf, iNx ); rewrap32( yh[r], yl[r], Nyf, iNy ); rewrap32( zh[r], zl[r], Nzf, iNz );
+                    frame32_rewrap<MASK>( fx, xh[r], xl[r], uh[r], ul[r] );
+                    frame32_rewrap<MASK>( fy, yh[r], yl[r], vh[r], vl[r] );
+                    frame32_rewrap<MASK>( fz, zh[r], zl[r], wh[r], wl[r] );
                 }
             }
             #pragma unroll
             for ( int r = 0; r < NPT; ++r )
             {
                 float tx, ty, tz;
-                const int i = advance_axis32<MASK>( xh[r], xl[r], uh[r], ul[r], Nxf, iNx, Nxi, tx );
-                const int j = advance_axis32<MASK>( yh[r], yl[r], vh[r], vl[r], Nyf, iNy, Nyi, ty );
-                const int k = advance_axis32<MASK>( zh[r], zl[r], wh[r], wl[r], Nzf, iNz, Nzi, tz );
+                const int i = drift_locate32<MASK>( fx, xh[r], xl[r], uh[r], ul[r], tx );
+                const int j = drift_locate32<MASK>( fy, yh[r], yl[r], vh[r], vl[r], ty );
+                const int k = drift_locate32<MASK>( fz, zh[r], zl[r], wh[r], wl[r], tz );
                 const float *W = Cs + k*sz + j*sy + i;
                 float buf[GATHER_PF + 1][4];
                 grad_prefetch<GATHER_PF>( W, sy, sz, buf );
@@ -734,11 +762,11 @@ void trace3d_fp32c_kernel( trace3_params p )
         double f = 0.0;
         if ( id[r].active )
         {
-            const double x = p.x_min + ( double( xh[r] ) + double( xl[r] ) ) * p.dx;
-            const double y = p.y_min + ( double( yh[r] ) + double( yl[r] ) ) * p.dy;
-            const double z = p.z_min + ( double( zh[r] ) + double( zl[r] ) ) * p.dz;
-            f = f0_3d( p.f0, x, y, z, ( double( uh[r] ) + double( ul[r] ) ) / cdx,
-                       ( double( vh[r] ) + double( vl[r] ) ) / cdy, ( double( wh[r] ) + double( wl[r] ) ) / cdz );
+            const double x = p.x_min + ( double( xh[r] - fx.P0 ) + double( xl[r] ) ) * p.dx;   // hi - P0 exact
+            const double y = p.y_min + ( double( yh[r] - fy.P0 ) + double( yl[r] ) ) * p.dy;
+            const double z = p.z_min + ( double( zh[r] - fz.P0 ) + double( zl[r] ) ) * p.dz;
+            f = f0_3d( p.f0, x, y, z, ( double( uh[r] - fx.W0 ) + double( ul[r] ) ) / cdx,
+                       ( double( vh[r] - fy.W0 ) + double( vl[r] ) ) / cdy, ( double( wh[r] - fz.W0 ) + double( wl[r] ) ) / cdz );
         }
         write_partial3( p, id[r], f );
     }
