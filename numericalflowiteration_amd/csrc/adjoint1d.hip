@@ -220,14 +220,13 @@ void adjoint1d_fp32c_kernel( adjoint1_params p )
     const bool active = task && iu < p.Nu;
 
     const int Nx = int( p.Nx ), mask = Nx - 1;
-    const float Nxf = float( p.Nx ), invNx = 1.0f / Nxf;
     const double cd = p.dt * p.dx_inv, kick = -0.5 * p.dt * p.dx_inv;
     const uint32_t sq = p.stride_q, q2 = p.q2_off;
+    const frame1 fr = make_frame1( p.f32_p0, p.f32_w0, p.Nx );
 
     // the FP32C trace kernel's start state (trace1d_fp32c_body)
-    const double w0 = cd * ( p.u_min + iu*p.du + p.du/2 );
-    float sh = float( ix ), sl = 0.0f;
-    float uh = float( w0 ), ul = float( w0 - double( uh ) );
+    float sh, sl, uh, ul;
+    fp32c_start( fr, ix, cd * ( p.u_min + iu*p.du + p.du/2 ), sh, sl, uh, ul );
 
     // forward: slices n-1 .. 0
     for ( int64_t mhi = int64_t( p.n ) - 1; mhi >= 0; mhi -= p.chunk )
@@ -240,11 +239,11 @@ void adjoint1d_fp32c_kernel( adjoint1_params p )
         for ( int jj = cnt - 1; jj >= 0; --jj )
         {
             const int64_t m = mlo + jj;
-            if ( POW2 && poly_wrap_due( m ) ) fp32c_wrap( sh, sl, Nxf, invNx );
-            fp32c_move( sh, sl, uh, ul );
-            if ( !POW2 ) fp32c_wrap( sh, sl, Nxf, invNx );
+            if ( POW2 && poly_wrap_due( m ) ) fp32c_rewrap( fr, sh, sl, uh, ul );
+            fp32c_move( fr, sh, sl, uh, ul );
+            if ( !POW2 ) fp32c_rewrap( fr, sh, sl, uh, ul );
             float t;
-            const int i = fp32c_locate<POW2>( sh, sl, mask, t );
+            const int i = fp32c_locate<POW2>( fr, sh, sl, mask, t );
             fp32c_kick( uh, ul, ( m == 0 ? 0.5f : 1.0f ) * fp32c_poly( Ql + jj * sq, q2, i, t ) );
         }
     }
@@ -252,7 +251,7 @@ void adjoint1d_fp32c_kernel( adjoint1_params p )
     // seed at the foot
     const double fbar = active ? -p.du * p.g[ix] : 0.0;
     double fx, fu, fa[NUFI_MAX_MODES];
-    f0_grad_1d( p.f0, p.x_min + ( double( sh ) + double( sl ) ) * p.dx, ( double( uh ) + double( ul ) ) / cd,
+    f0_grad_1d( p.f0, p.x_min + fp32c_position( fr, sh, sl ) * p.dx, fp32c_velocity( fr, uh, ul ) / cd,
                 fx, fu, fa, p.npar );
     double sbar = fbar * fx * p.dx;     // d/ds = dx d/dx
     double ubar = fbar * fu / cd;       // d/dw = d/du / cd
@@ -279,7 +278,7 @@ void adjoint1d_fp32c_kernel( adjoint1_params p )
             const float hf = m == 0 ? 0.5f : 1.0f;
             const float *Q = Ql + jj * sq;
             float t;
-            const int i = fp32c_locate<POW2>( sh, sl, mask, t );
+            const int i = fp32c_locate<POW2>( fr, sh, sl, mask, t );
             const float q = fp32c_poly( Q, q2, i, t );
             // dq/dt of the scaled table, in FP64
             const float2 q01 = reinterpret_cast<const float2*>( Q )[ i + 1 ];
@@ -290,8 +289,8 @@ void adjoint1d_fp32c_kernel( adjoint1_params p )
             // undo the kick, then the drift
             fp32c_kick( uh, ul, -( hf * q ) );
             ubar = ubar - sbar;
-            fp32c_move( sh, sl, -uh, -ul );
-            if ( !POW2 || poly_wrap_due( m ) ) fp32c_wrap( sh, sl, Nxf, invNx );
+            fp32c_move<true>( fr, sh, sl, uh, ul );
+            if ( !POW2 || poly_wrap_due( m ) ) fp32c_rewrap( fr, sh, sl, uh, ul );
         }
         __syncthreads();
         for ( uint32_t q = threadIdx.x; q < uint32_t( cnt ) * p.Nx; q += BLOCK )

@@ -246,6 +246,7 @@ int nufi_batch_create( const nufi_config *confs, uint64_t n_runs, int order, int
         trace1d_xm_setup( p );
         p.x_min = c.x_min; p.dx = c.dx; p.dx_inv = c.dx_inv; p.Lx = c.Lx; p.Lx_inv = c.Lx_inv; p.dt = c.dt;
         p.nx_min = c.x_min; p.ndx = c.dx; p.nu_min = c.u_min; p.ndu = c.du;
+        fp32c_frame( double( c.Nx ), c.dt * c.dx_inv, c.u_min, double( c.Nu ), c.du, p.f32_p0, p.f32_w0 );
         p.f0 = c.f0;
         b->h_runs[r].Nt = c.Nt;
 

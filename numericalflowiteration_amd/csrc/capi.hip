@@ -315,6 +315,8 @@ int launch_trace( nufi_ctx *c, uint64_t n, uint64_t q_begin, uint64_t q_end, uin
         p.x_min = c->conf.x_min; p.dx = c->conf.dx; p.dx_inv = c->conf.dx_inv;
         p.Lx = c->conf.Lx; p.Lx_inv = c->conf.Lx_inv; p.dt = c->conf.dt;
         p.nx_min = c->conf.x_min; p.ndx = c->conf.dx; p.nu_min = c->conf.u_min; p.ndu = c->conf.du;
+        fp32c_frame( double( c->conf.Nx ), c->conf.dt * c->conf.dx_inv, c->conf.u_min, double( c->conf.Nu ),
+                     c->conf.du, p.f32_p0, p.f32_w0 );
         p.f0 = c->conf.f0;
         e = launch_trace1d( p, c->variant, c->shape, c->stream );
     }
@@ -868,6 +870,8 @@ int nufi_adjoint_grad( nufi_ctx *c, double *dJ_da, int n_par, double *J_out )
     ap.n_tasks = n_tasks; ap.npar = uint32_t( npar );
     ap.x_min = c->conf.x_min; ap.dx = c->conf.dx; ap.dx_inv = c->conf.dx_inv; ap.dt = c->conf.dt;
     ap.u_min = c->conf.u_min; ap.du = c->conf.du; ap.f0 = c->conf.f0;
+    fp32c_frame( double( Nx ), c->conf.dt * c->conf.dx_inv, c->conf.u_min, double( c->conf.Nu ), c->conf.du,
+                 ap.f32_p0, ap.f32_w0 );
     const size_t per_slice = size_t( c->sq ) * ( f32 ? sizeof(float) : sizeof(double) ) + Nx * sizeof(double);
     ap.chunk = uint32_t( std::max<size_t>( 1, std::min<size_t>( 32, std::min<size_t>( c->lds_cap, 65536 ) / per_slice ) ) );
     if ( adjoint1d_lds_bytes( ap ) > LDS_LIMIT )

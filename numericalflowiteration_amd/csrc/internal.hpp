@@ -79,6 +79,7 @@ struct trace1_params
     double x_min, dx, dx_inv, Lx, Lx_inv, dt;    // the field grid
     double nx_min, ndx, nu_min, ndu;             // the node grid (= field grid for rho)
     double weight;           // MODE_METRICS quadrature weight
+    float f32_p0, f32_w0;    // FP32C: offsets of the position and velocity hi parts (fp32c_frame)
     nufi_f0 f0;
 };
 
@@ -124,6 +125,25 @@ struct trace3_params
     nufi_f0 f0;
 };
 
+// FP32C frames of one axis (period N cells, velocity nodes v_lo + (i + 1/2) dv,
+// i < nv, dt * d_inv cells per step per unit velocity): W0 = 1.5 * 2^j and
+// P0 = 1.5 * 2^k with 2^(j-1) >= wcap (a bound on |velocity| in cells per
+// step, twice the grid's with one cell to spare for the kicks) and
+// 2^(k-1) >= N + 17 wcap + 2 (a wrapped position plus the drift of the 16
+// slices between re-wraps): W0 + w and P0 + s then stay in one binade each,
+// [2^j, 2^(j+1)) and [2^k, 2^(k+1)), and P0 is a multiple of N when N is a
+// power of two.
+inline void fp32c_frame( double N, double cd, double v_lo, double nv, double dv, float &p0, float &w0 )
+{
+    const double vmax = std::max( std::fabs( v_lo ), std::fabs( v_lo + nv * dv ) );
+    const double wcap = 2.0 * cd * vmax + 1.0;
+    int j = 1, k = 1;
+    while ( std::ldexp( 1.0, j - 1 ) < wcap ) ++j;
+    while ( std::ldexp( 1.0, k - 1 ) < N + 17.0 * wcap + 2.0 ) ++k;
+    w0 = float( 1.5 * std::ldexp( 1.0, j ) );
+    p0 = float( 1.5 * std::ldexp( 1.0, k ) );
+}
+
 // The field and velocity grid of a 3D trace launch from a config.
 inline void fill_trace3_geometry( trace3_params &p, const nufi_config &g )
 {
@@ -141,19 +161,7 @@ inline void fill_trace3_geometry( trace3_params &p, const nufi_config &g )
     {
         p.kick72[a]  = -g.dt * dinv[a] / 72.0;
         p.kick72h[a] = 0.5 * p.kick72[a];
-        // FP32C frames: W0 = 1.5 * 2^j and P0 = 1.5 * 2^k with 2^(j-1) >= wcap (a
-        // bound on |velocity| in cells per step, twice the grid's with one cell
-        // to spare for the kicks) and 2^(k-1) >= N + 17 wcap + 2 (a wrapped
-        // position plus the drift of the 16 slices between re-wraps): W0 + w and
-        // P0 + s then stay in one binade, [2^j, 2^(j+1)) and [2^k, 2^(k+1)), and
-        // P0 is a multiple of N when N is a power of two
-        const double vmax = std::max( std::fabs( vlo[a] ), std::fabs( vlo[a] + nv[a] * dv[a] ) );
-        const double wcap = 2.0 * g.dt * dinv[a] * vmax + 1.0;
-        int j = 1, k = 1;
-        while ( std::ldexp( 1.0, j - 1 ) < wcap ) ++j;
-        while ( std::ldexp( 1.0, k - 1 ) < nx[a] + 17.0 * wcap + 2.0 ) ++k;
-        p.f32_w0[a] = float( 1.5 * std::ldexp( 1.0, j ) );
-        p.f32_p0[a] = float( 1.5 * std::ldexp( 1.0, k ) );
+        fp32c_frame( nx[a], g.dt * dinv[a], vlo[a], nv[a], dv[a], p.f32_p0[a], p.f32_w0[a] );
     }
     p.f0 = g.f0;
 }
@@ -219,6 +227,7 @@ struct adjoint1_params
     uint64_t n;
     uint32_t Nx, Nu, stride_d, stride_q, q2_off, wpx, n_tasks, chunk, npar;
     double x_min, dx, dx_inv, dt, u_min, du;
+    float f32_p0, f32_w0;     // FP32C: the trace kernel's frames (fp32c_frame)
     nufi_f0 f0;
 };
 

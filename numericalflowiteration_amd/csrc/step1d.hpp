@@ -106,36 +106,74 @@ __device__ __forceinline__ void poly_step( double &s, double &u, const double *Q
 
 // ---- the FP32-compensated step ---------------------------------------------
 // Position (cell units) and velocity (cells per step, w = dt dx_inv u) as
-// unevaluated pairs (hi + lo); the FP32 tables are scaled to give the change
-// of w.  The FP32C trace kernel and the FP32C adjoint share these, so both
-// reach the same states bit for bit.
-
-// drift s -= w: a TwoSum of the leading parts, the trailing parts added to its error
-__device__ __forceinline__ void fp32c_move( float &sh, float &sl, float wh, float wl )
+// unevaluated pairs (hi + lo) whose hi parts carry the offsets P0 and W0 of
+// fp32c_frame: every hi value of a run stays in one binade, so the offsets
+// come off exactly, the drift and the kick are ordered Fast2Sums and a wrap
+// by a multiple of Nx is exact.  The FP32 tables are scaled to give the
+// change of w.  The FP32C trace kernels and the FP32C adjoint share these, so
+// both reach the same states bit for bit.
+struct frame1
 {
-    float y, e;
-    two_sum( sh, -wh, y, e );
-    e = e + ( sl - wl );
-    fast_two_sum( y, e, sh, sl );
+    float P0, W0, N, invN;
+    int P0i;
+};
+
+__device__ __forceinline__ frame1 make_frame1( float p0, float w0, uint32_t Nx )
+{
+    frame1 f;
+    f.P0 = p0; f.W0 = w0; f.N = float( Nx ); f.invN = 1.0f / f.N; f.P0i = int( p0 );
+    return f;
 }
 
-// exact periodic wrap of the pair: sh - Nx*w is exact (Sterbenz) for w >= 1
-// and w <= -2, and Fast2Sum is exact for w = -1 (Nx >= |sh|)
-__device__ __forceinline__ void fp32c_wrap( float &sh, float &sl, float Nxf, float invNx )
+// drift s -= w (UNDO: s += w, the inverse step of the adjoint): a Fast2Sum of
+// the hi parts (|sh| > |d|), the lo parts added to its error, left unnormalised
+template <bool UNDO = false>
+__device__ __forceinline__ void fp32c_move( const frame1 &f, float &sh, float &sl, float wh, float wl )
 {
-    const float w = floorf( sh * invNx );
-    float y2, e2;
-    fast_two_sum( -Nxf * w, sh, y2, e2 );
-    sh = y2; sl = sl + e2;
+    const float d = UNDO ? f.W0 - wh : wh - f.W0;                  // exact
+    const float y = sh - d;
+    const float e = ( sh - y ) - d;
+    sl = ( UNDO ? sl + wl : sl - wl ) + e;
+    sh = y;
 }
 
-// the cell (in [-1, Nx]; POW2: wrapped by the mask) and t of the pair
+// wrap the position into [P0, P0 + Nx) (exact) and renormalise both pairs
+__device__ __forceinline__ void fp32c_rewrap( const frame1 &f, float &sh, float &sl, float &wh, float &wl )
+{
+    sh = fmaf( -f.N, floorf( ( sh - f.P0 ) * f.invN ), sh );
+    float s = sh + sl;
+    sl = sl - ( s - sh ); sh = s;
+    s = wh + wl;
+    wl = wl - ( s - wh ); wh = s;
+}
+
+// the cell (in [-1, Nx]; POW2: wrapped by the mask, P0 being a multiple of
+// Nx) and t of the pair
 template <bool POW2>
-__device__ __forceinline__ int fp32c_locate( float sh, float sl, int mask, float &t )
+__device__ __forceinline__ int fp32c_locate( const frame1 &f, float sh, float sl, int mask, float &t )
 {
     const float k = floorf( sh );
     t = ( sh - k ) + sl;
-    return POW2 ? ( int( k ) & mask ) : int( k );
+    return POW2 ? ( int( k ) & mask ) : int( k ) - f.P0i;
+}
+
+// position (cells) and velocity (cells per step) of the pairs, in FP64
+__device__ __forceinline__ double fp32c_position( const frame1 &f, float sh, float sl )
+{
+    return double( sh - f.P0 ) + double( sl );
+}
+__device__ __forceinline__ double fp32c_velocity( const frame1 &f, float wh, float wl )
+{
+    return double( wh - f.W0 ) + double( wl );
+}
+
+// the start state of node (ix, w0 cells per step)
+__device__ __forceinline__ void fp32c_start( const frame1 &f, uint32_t ix, double w0, float &sh, float &sl,
+                                             float &wh, float &wl )
+{
+    sh = f.P0 + float( ix ); sl = 0.0f;                              // exact
+    const double w = double( f.W0 ) + w0;
+    wh = float( w ); wl = float( w - double( wh ) );
 }
 
 // the kick q = kick dt dx_inv (Q0 + Q1 t + Q2 t^2) in cells per step from the
@@ -146,12 +184,12 @@ __device__ __forceinline__ float fp32c_poly( const float *Q, uint32_t q2, int i,
     return fmaf( fmaf( Q[ q2 + ( i + 1 ) ], t, q01.y ), t, q01.x );
 }
 
-// w += q with TwoSum
-__device__ __forceinline__ void fp32c_kick( float &uh, float &ul, float q )
+// w += q: a Fast2Sum, |wh| >= W0 / 2 > |q|
+__device__ __forceinline__ void fp32c_kick( float &wh, float &wl, float q )
 {
-    float y, e;
-    two_sum( uh, q, y, e );
-    uh = y; ul = ul + e;
+    const float s = wh + q;
+    wl = wl + ( q - ( s - wh ) );
+    wh = s;
 }
 
 // ---- kick-polynomial tables ------------------------------------------------

@@ -383,34 +383,32 @@ __device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint
     constexpr int WPB = BLOCK / WAVE;
     const xm_tile tile = make_tile( p, wg );
     xm_node id[NPT];
-    float sh[NPT], sl[NPT], uh[NPT], ul[NPT];       // position (cells), velocity (cells per step)
+    float sh[NPT], sl[NPT], uh[NPT], ul[NPT];       // position (cells) + P0, velocity (cells per step) + W0
     const double cd = p.dt * p.dx_inv;
+    const frame1 fr = make_frame1( p.f32_p0, p.f32_w0, p.Nx );
     #pragma unroll
     for ( int r = 0; r < NPT; ++r )
     {
         id[r] = make_xm_node<WPB,NPT>( p, tile, r );
-        const double w0 = cd * ( p.nu_min + id[r].iu*p.ndu + p.ndu/2 );
-        sh[r] = float( id[r].ix ); sl[r] = 0.0f;
-        uh[r] = float( w0 );       ul[r] = float( w0 - double( uh[r] ) );
+        fp32c_start( fr, id[r].ix, cd * ( p.nu_min + id[r].iu*p.ndu + p.ndu/2 ), sh[r], sl[r], uh[r], ul[r] );
     }
 
-    const float Nxf   = float( p.Nx );
-    const float invNx = 1.0f / Nxf;
     const int mask    = int( p.Nx ) - 1;
     const uint32_t sq = p.stride_q, q2 = p.q2_off;
 
-    auto wrap = [&]( int r ) { fp32c_wrap( sh[r], sl[r], Nxf, invNx ); };
+    auto wrap = [&]( int r ) { fp32c_rewrap( fr, sh[r], sl[r], uh[r], ul[r] ); };
     auto step = [&]( const float *Q, float hf )
     {
         #pragma unroll
         for ( int r = 0; r < NPT; ++r )
         {
-            fp32c_move( sh[r], sl[r], uh[r], ul[r] );
-            // exact periodic wrap into [0, Nx); POW2: the cell index wraps by
-            // the mask and the position is re-wrapped every POLY_WRAP slices
+            fp32c_move( fr, sh[r], sl[r], uh[r], ul[r] );
+            // exact periodic wrap into [P0, P0 + Nx) and renormalised pairs;
+            // POW2: the cell index wraps by the mask and the position is
+            // re-wrapped every POLY_WRAP slices
             if ( !POW2 ) wrap( r );
             float t;
-            const int i = fp32c_locate<POW2>( sh[r], sl[r], mask, t );
+            const int i = fp32c_locate<POW2>( fr, sh[r], sl[r], mask, t );
             fp32c_kick( uh[r], ul[r], hf * fp32c_poly( Q, q2, i, t ) );
         }
     };
@@ -441,8 +439,8 @@ __device__ __forceinline__ void trace1d_fp32c_body( const trace1_params &p, uint
     #pragma unroll
     for ( int r = 0; r < NPT; ++r )
     {
-        const double s = double( sh[r] ) + double( sl[r] );
-        const double u = ( double( uh[r] ) + double( ul[r] ) ) / cd;
+        const double s = fp32c_position( fr, sh[r], sl[r] );
+        const double u = fp32c_velocity( fr, uh[r], ul[r] ) / cd;
         f[r] = id[r].active ? f0_1d( p.f0, p.x_min + s*p.dx, u ) : 0.0;
     }
     xm_reduce<BLOCK,NPT>( p, tile, id, f, reinterpret_cast<double*>( ldsf ) );
