@@ -443,6 +443,16 @@ def adjoint_case(threads, with_cpu):
                                "sample": "the whole forward run on the CPU (the oracle)"}
         out["parity_vs_oracle"] = {"J_rel_err_fp64": abs(out["fp64"]["J"] - Jc) / abs(Jc),
                                    "J_rel_err_fp32c": abs(out["fp32c"]["J"] - Jc) / abs(Jc)}
+        # the gradient on the full grid against the oracle's forward-mode tangent
+        # (dual numbers), over the first 81 steps: the tangent to T costs minutes
+        short = nufi.multimode_1d(a0, T=5.0)
+        _, dJ = O.tangent_run(short, nthreads=threads)
+        dJ = np.asarray(dJ[:len(a0)])
+        for name, prec in (("fp64", nufi.FP64), ("fp32c", nufi.FP32C)):
+            _, g = A.energy_and_grad(short, a0, precision=prec)
+            out["parity_vs_oracle"][f"grad_rel_err_vs_oracle_tangent_{name}"] = \
+                float(np.max(np.abs(g - dJ)) / np.max(np.abs(dJ)))
+        out["parity_vs_oracle"]["tangent_sample"] = "Nx=128, Nv=512, dt=1/16, T=5 (81 steps)"
         progress("adjoint CPU baseline done")
     return out
 

@@ -143,6 +143,21 @@ def test_fp32c_adjoint_non_power_of_two_grid():
     assert abs(J - Jt) <= 1e-6 * abs(Jt) and err <= 1e-5
 
 
+@pytest.mark.parametrize("prec,tol", [(nufi.FP64, 1e-8), (nufi.FP32C, 1e-5)])
+def test_adjoint_full_grid_against_the_oracle_tangent(prec, tol):
+    """The adjoint case's own grid (Nx = 128, Nu = 512, dt = 1/16) over 81 steps
+    (the oracle's tangent to T = 20 takes minutes): FP64 to the tangent bar of
+    the small case, FP32C to the adjoint bar."""
+    conf = nufi.multimode_1d(nufi.adjoint_modes(), T=5.0)
+    J, g = A.energy_and_grad(conf, nufi.adjoint_modes(), precision=prec)
+    Jt, dJ = O.tangent_run(conf)
+    err = np.max(np.abs(g - dJ[:8])) / np.max(np.abs(dJ[:8]))
+    print(f"[adjoint full grid] precision={prec} J rel err {abs(J - Jt) / abs(Jt):.3e}, grad vs tangent {err:.3e}",
+          flush=True)
+    assert abs(J - Jt) <= (1e-11 if prec == nufi.FP64 else 1e-6) * abs(Jt)
+    assert err <= tol
+
+
 def test_adjoint_rejects_unsupported():
     with nufi.GpuScheduler(nufi.landau_3d(N=8, V=8, T=1.0)) as s:
         with pytest.raises(nufi.NufiError):
