@@ -23,7 +23,8 @@
 //   strict : the reference's arithmetic (three separate tensor evaluations,
 //            Cox-de Boor bases, de Boor in z, unwrapped positions), no FMA
 //            contraction.
-//   fast   : cell-unit positions wrapped exactly each step; the value and
+//   fast   : cell-unit positions wrapped exactly (every 16 slices on power-of-
+//            two grids, the cell index by a mask in between); the value and
 //            derivative bases of each axis in closed form; the three gradient
 //            components share one pass over the 64 coefficients:
 //              A_jk = sum_i c_ijk Bx_i,  X_jk = sum_i c_ijk Dx_i    (128 FMA)
@@ -32,14 +33,16 @@
 //              Ex ~ sum_k R_k Bz_k, Ey ~ sum_k Q_k Bz_k, Ez ~ sum_k P_k Dz_k (12)
 //   fp32c  : as fast in FP32 from an FP32 copy of the slice, positions (cell
 //            units) and velocities (cells per step, w = dt dx_inv u) as hi + lo
-//            pairs with TwoSum updates; f0 and the rho sums in FP64.  The copy
-//            is scaled in FP64 by the kick in cells per step over 72 before
-//            its one rounding and the bases are 6B and 2D, so
-//            no FP32-rounded constant (1/6, -dt/dx) multiplies the kicks: such
-//            a constant biases every node's kick by the same relative error,
-//            which the sum over velocities does not average away (for 3D
-//            Landau it shifted the phase of the damped field enough to give
-//            2e-5 relative energy error near a minimum of the energy).
+//            pairs held at offsets (one binade per hi part), updated by ordered
+//            Fast2Sums; f0 and the rho sums in FP64.  The copy is scaled in
+//            FP64 by the kick in cells per step over 72 before its one rounding
+//            and the bases are 6B and 2D, so no FP32-rounded constant (1/6,
+//            -dt/dx) multiplies the kicks: such a constant biases every node's
+//            kick by the same relative error, which the sum over velocities
+//            does not average away (for 3D Landau it shifted the phase of the
+//            damped field enough to give 2e-5 relative energy error near a
+//            minimum of the energy).  For the same reason the four FP32 slope
+//            bases sum to exactly zero (cubic_bases_x6).
 #include "device_common.hpp"
 #include "internal.hpp"
 
