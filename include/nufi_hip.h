@@ -53,8 +53,9 @@ enum nufi_status {
 /* ---- precision of the characteristic trace ----------------------------- */
 enum nufi_precision {
     NUFI_FP64  = 0, /* the plain FP64 method                                */
-    NUFI_FP32C = 1  /* FP32 trace with compensated (two-sum) position and
-                       velocity updates; Poisson, fit and rho stay FP64     */
+    NUFI_FP32C = 1  /* FP32 trace with compensated position and velocity
+                       updates (hi + lo pairs, error-free Fast2Sums);
+                       Poisson, fit and rho stay FP64                       */
 };
 
 /* ---- initial conditions (config_t::f0) --------------------------------- */
