@@ -46,6 +46,8 @@
 #include "device_common.hpp"
 #include "internal.hpp"
 
+#include <type_traits>
+
 namespace nufi_impl
 {
 
@@ -226,24 +228,58 @@ __device__ __forceinline__ void cubic_bases_x6( float t, float B[4], float D[4] 
 // faster, 4 slower.
 constexpr int GATHER_PF = 1;
 
+// The rows of a node's 4x4x4 window: row idx (j = idx & 3, k = idx >> 2) is
+// the four reals from c + k*sz + j*sy.
 template <typename T>
-__device__ __forceinline__ void load_row( const T *__restrict__ c, int idx, uint32_t sy, uint32_t sz, T r[4] )
+struct window_rt
 {
-    const T *row = c + ( idx >> 2 ) * sz + ( idx & 3 ) * sy;
-    r[0] = row[0]; r[1] = row[1]; r[2] = row[2]; r[3] = row[3];
+    const T *c;
+    uint32_t sy, sz;
+    __device__ __forceinline__ const T *row( int idx ) const { return c + ( idx >> 2 ) * sz + ( idx & 3 ) * sy; }
+};
+
+// The same rows in the slice of a G^3 grid (strides G+3 and (G+3)^2): one LDS
+// address per z plane, held in a register, and the rows of a plane at
+// immediate offsets of the reads (at most 3(G+3)+3 reals, within the 255
+// elements of ds_read2_b64 / ds_read2_b32 offsets for G <= 77).  Without the
+// opaque register the compiler folds every row into a base address of its own.
+template <int G, typename T = double>
+struct window_g
+{
+    using lds_ptr = const __attribute__(( address_space( 3 ) )) T *;
+    lds_ptr pl[4];
+    __device__ __forceinline__ explicit window_g( const T *w )
+    {
+        pl[0] = (lds_ptr)w;
+        #pragma unroll
+        for ( int k = 1; k < 4; ++k )
+        {
+            uint32_t a = uint32_t( size_t( (lds_ptr)( w + k * ( G + 3 ) * ( G + 3 ) ) ) );
+            asm volatile( "" : "+v"( a ) );
+            pl[k] = (lds_ptr)size_t( a );
+        }
+    }
+    __device__ __forceinline__ lds_ptr row( int idx ) const { return pl[ idx >> 2 ] + ( idx & 3 ) * ( G + 3 ); }
+};
+
+template <typename Win, typename T>
+__device__ __forceinline__ void load_row( const Win &w, int idx, T r[4] )
+{
+    const auto q = w.row( idx );
+    r[0] = q[0]; r[1] = q[1]; r[2] = q[2]; r[3] = q[3];
 }
 
-// the first PF rows of the window at c, before the bases are computed
-template <int PF, typename T>
-__device__ __forceinline__ void grad_prefetch( const T *__restrict__ c, uint32_t sy, uint32_t sz, T buf[PF + 1][4] )
+// the first PF rows of the window, before the bases are computed
+template <int PF, typename Win, typename T>
+__device__ __forceinline__ void grad_prefetch( const Win &w, T buf[PF + 1][4] )
 {
     #pragma unroll
-    for ( int d = 0; d < PF; ++d ) load_row( c, d, sy, sz, buf[d] );
+    for ( int d = 0; d < PF; ++d ) load_row( w, d, buf[d] );
     __builtin_amdgcn_sched_barrier( 0 );
 }
 
-template <int PF, typename T>
-__device__ __forceinline__ void grad_rows( const T *__restrict__ c, uint32_t sy, uint32_t sz, T buf[PF + 1][4],
+template <int PF, typename Win, typename T>
+__device__ __forceinline__ void grad_rows( const Win &w, T buf[PF + 1][4],
                                            const T Bx[4], const T Dx[4], const T By[4],
                                            const T Dy[4], const T Bz[4], const T Dz[4],
                                            T &gx, T &gy, T &gz )
@@ -253,7 +289,7 @@ __device__ __forceinline__ void grad_rows( const T *__restrict__ c, uint32_t sy,
     #pragma unroll
     for ( int idx = 0; idx < 16; ++idx )
     {
-        if ( idx + PF < 16 ) load_row( c, idx + PF, sy, sz, buf[ ( idx + PF ) % ( PF + 1 ) ] );
+        if ( idx + PF < 16 ) load_row( w, idx + PF, buf[ ( idx + PF ) % ( PF + 1 ) ] );
         __builtin_amdgcn_sched_barrier( 0 );
         const T *r = buf[ idx % ( PF + 1 ) ];
         const int j = idx & 3, k = idx >> 2;
@@ -327,7 +363,19 @@ __device__ __forceinline__ fast3_consts make_fast3( const trace3_params &p )
     return k;
 }
 
-template <bool MASK = false, int PF = GATHER_PF>
+template <int G, typename T>
+__device__ __forceinline__ typename std::enable_if<( G > 0 ), window_g<G, T>>::type
+make_window( const T *w, uint32_t, uint32_t ) { return window_g<G, T>( w ); }
+
+template <int G, typename T>
+__device__ __forceinline__ typename std::enable_if<( G == 0 ), window_rt<T>>::type
+make_window( const T *w, uint32_t sy, uint32_t sz ) { return window_rt<T> { w, sy, sz }; }
+
+// G > 0: a cubic G^3 grid whose slice strides (G+3, (G+3)^2) are compile-time
+// constants, so the window's row offsets are immediates of the LDS reads and
+// only the window base is computed per node-step; G = 0 takes the strides from
+// the kernel arguments (24-bit products: a slice stays far below 2^24 reals).
+template <bool MASK = false, int PF = GATHER_PF, int G = 0>
 __device__ __forceinline__ void step3_fast( const fast3_consts &k, const double *C, bool half, bool still,
                                             double &sx, double &sy_, double &sz_, double &vu, double &vv, double &vw )
 {
@@ -335,15 +383,19 @@ __device__ __forceinline__ void step3_fast( const fast3_consts &k, const double 
     const int i = advance_axis<MASK>( sx,  vu, k.cdx, k.Nxd, k.iNx, k.Nxi, tx, still );
     const int j = advance_axis<MASK>( sy_, vv, k.cdy, k.Nyd, k.iNy, k.Nyi, ty, still );
     const int l = advance_axis<MASK>( sz_, vw, k.cdz, k.Nzd, k.iNz, k.Nzi, tz, still );
-    const double *W = C + l*k.sz + j*k.sy + i;
+    constexpr uint32_t GY = G > 0 ? uint32_t( G + 3 ) : 0u, GZ = GY * GY;
+    const uint32_t sy = G > 0 ? GY : k.sy, sz = G > 0 ? GZ : k.sz;
+    const double *W = C + __umul24( uint32_t( l ), sz ) + __umul24( uint32_t( j ), sy ) + i;
+    using Win = typename std::conditional<( G > 0 ), window_g<G>, window_rt<double>>::type;
+    Win win = make_window<G>( W, sy, sz );
     double buf[PF + 1][4];
-    grad_prefetch<PF>( W, k.sy, k.sz, buf );
+    grad_prefetch<PF>( win, buf );
     double Bx[4], Dx[4], By[4], Dy[4], Bz[4], Dz[4];
     cubic_bases_x6( tx, Bx, Dx );
     cubic_bases_x6( ty, By, Dy );
     cubic_bases_x6( tz, Bz, Dz );
     double gx, gy, gz;
-    grad_rows<PF>( W, k.sy, k.sz, buf, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
+    grad_rows<PF>( win, buf, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
     vu = fma( half ? k.kxh : k.kx, gx, vu );
     vv = fma( half ? k.kyh : k.ky, gy, vv );
     vw = fma( half ? k.kzh : k.kz, gz, vw );
@@ -448,8 +500,9 @@ __device__ __forceinline__ bool xm3_velocity( const trace3_params &p, const xm3_
     return a.ok && iv < nvel && q >= p.q_begin && q < p.q_end;
 }
 
-// The NPT nodes a lane traces in one velocity pass of the x-major FP64 kernel.
-template <int NPT, bool MASK>
+// The NPT nodes a lane traces in one velocity pass of the x-major FP64 kernel
+// (G: the compile-time grid edge of step3_fast, 0 for any grid).
+template <int NPT, bool MASK, int G>
 struct xm3_nodes
 {
     double sx[NPT], sy[NPT], sz[NPT], vu[NPT], vv[NPT], vw[NPT];
@@ -483,13 +536,13 @@ struct xm3_nodes
             }
             #pragma unroll
             for ( int r = 0; r < NPT; ++r )
-                step3_fast<MASK>( k3, C, false, false, sx[r], sy[r], sz[r], vu[r], vv[r], vw[r] );
+                step3_fast<MASK, GATHER_PF, G>( k3, C, false, false, sx[r], sy[r], sz[r], vu[r], vv[r], vw[r] );
         }
         if ( mlo == 0 )                  // slice 0: wrap3_due(0) is false
         {
             #pragma unroll
             for ( int r = 0; r < NPT; ++r )
-                step3_fast<MASK>( k3, Cb, true, false, sx[r], sy[r], sz[r], vu[r], vv[r], vw[r] );
+                step3_fast<MASK, GATHER_PF, G>( k3, Cb, true, false, sx[r], sy[r], sz[r], vu[r], vv[r], vw[r] );
         }
     }
 
@@ -510,7 +563,7 @@ struct xm3_nodes
 // 1024-thread workgroups run four waves per SIMD (128 VGPRs: the f0 evaluation
 // after each velocity pass spills, the step loop does not); 512 threads, two
 // (about 230 VGPRs, no spills; with four nodes a lane, the default shape)
-template <int BLOCK, int NPT, bool MASK>
+template <int BLOCK, int NPT, bool MASK, int G = 0>
 __global__ __launch_bounds__( BLOCK ) __attribute__(( amdgpu_waves_per_eu( BLOCK >= 1024 ? 4 : 1 ) ))
 void trace3d_xm_fast_kernel( trace3_params p )
 {
@@ -520,7 +573,7 @@ void trace3d_xm_fast_kernel( trace3_params p )
     const fast3_consts k3 = make_fast3( p );
     const uint32_t st = p.st;
     csum acc { 0.0, 0.0 };
-    xm3_nodes<NPT,MASK> nd;
+    xm3_nodes<NPT,MASK,G> nd;
     const uint32_t Cn = p.chunk;
     const int64_t top = int64_t( p.n ) - 1;
     const int64_t K = ( top + int64_t( Cn ) ) / int64_t( Cn );      // chunks per pass (0 at n = 0)
@@ -640,7 +693,7 @@ __device__ __forceinline__ void kick32( float &wh, float &wl, float q )
 
 // velocity-major: node r of a lane is make_node3's (wave task g, lane), the
 // wave's 64 lanes are 64 consecutive velocities of one spatial point
-template <int BLOCK, int NPT, bool MASK>
+template <int BLOCK, int NPT, bool MASK, int G = 0>
 __global__ __launch_bounds__( BLOCK ) __attribute__(( amdgpu_waves_per_eu( 4 ) ))
 void trace3d_fp32c_kernel( trace3_params p )
 {
@@ -683,7 +736,8 @@ void trace3d_fp32c_kernel( trace3_params p )
     const double ry = ( p.dy_inv / p.dx_inv ) * ( p.dy_inv / p.dx_inv ), rz = ( p.dz_inv / p.dx_inv ) * ( p.dz_inv / p.dx_inv );
     const float ryh = float( ry ), ryl = float( ry - double( ryh ) );
     const float rzh = float( rz ), rzl = float( rz - double( rzh ) );
-    const uint32_t st = p.st32, sy = p.sy, sz = p.sz, C = p.chunk;
+    const uint32_t st = p.st32, sy = G > 0 ? uint32_t( G + 3 ) : p.sy, sz = G > 0 ? uint32_t( ( G + 3 ) * ( G + 3 ) ) : p.sz;
+    const uint32_t C = p.chunk;
 
     // chunk k: slices max(top - kC - C + 1, 0) .. top - kC.  With p.dma the
     // chunks alternate between two LDS buffers: chunk k+1 is copied straight
@@ -742,15 +796,17 @@ void trace3d_fp32c_kernel( trace3_params p )
                 const int i = drift_locate32<MASK>( fx, xh[r], xl[r], uh[r], ul[r], tx );
                 const int j = drift_locate32<MASK>( fy, yh[r], yl[r], vh[r], vl[r], ty );
                 const int k = drift_locate32<MASK>( fz, zh[r], zl[r], wh[r], wl[r], tz );
-                const float *W = Cs + k*sz + j*sy + i;
+                const float *W = Cs + __umul24( uint32_t( k ), sz ) + __umul24( uint32_t( j ), sy ) + i;
                 float buf[GATHER_PF + 1][4];
-                grad_prefetch<GATHER_PF>( W, sy, sz, buf );
+                using Win = typename std::conditional<( G > 0 ), window_g<G, float>, window_rt<float>>::type;
+                const Win win = make_window<G>( W, sy, sz );
+                grad_prefetch<GATHER_PF>( win, buf );
                 float Bx[4], Dx[4], By[4], Dy[4], Bz[4], Dz[4];
                 cubic_bases_x6( tx, Bx, Dx );
                 cubic_bases_x6( ty, By, Dy );
                 cubic_bases_x6( tz, Bz, Dz );
                 float gx, gy, gz;
-                grad_rows<GATHER_PF>( W, sy, sz, buf, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
+                grad_rows<GATHER_PF>( win, buf, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
                 kick32( uh[r], ul[r], hf * gx );
                 kick32( vh[r], vl[r], fmaf( gy, hyh, gy * hyl ) );
                 kick32( wh[r], wl[r], fmaf( gz, hzh, gz * hzl ) );
@@ -941,8 +997,10 @@ hipError_t launch3_npt( const trace3_params &p, int variant, int npt, uint32_t g
     }
     // power-of-two axes: the cell index wraps by a mask, positions every 16 slices
     const bool mask = ( p.Nx & ( p.Nx - 1 ) ) == 0 && ( p.Ny & ( p.Ny - 1 ) ) == 0 && ( p.Nz & ( p.Nz - 1 ) ) == 0;
+    const bool g16 = mask && npt == 4 && p.Nx == 16 && p.Ny == 16 && p.Nz == 16 && p.sy == 19 && p.sz == 19 * 19;
     if ( variant == TRACE_FP32C )
     {
+        if ( g16 ) return launch3( trace3d_fp32c_kernel<BLOCK,4,true,16>, grid, BLOCK, lds, p, st );
         switch ( npt * 2 + ( mask ? 1 : 0 ) )
         {
         case 2: return launch3( trace3d_fp32c_kernel<BLOCK,1,false>, grid, BLOCK, lds, p, st );
@@ -954,6 +1012,9 @@ hipError_t launch3_npt( const trace3_params &p, int variant, int npt, uint32_t g
         default: return hipErrorInvalidValue;
         }
     }
+    // the 16^3 grid with its strides compiled in, at the default shape's node count
+    if ( g16 )
+        return launch3( trace3d_xm_fast_kernel<BLOCK,4,true,16>, grid, BLOCK, lds, p, st );
     switch ( npt * 2 + ( mask ? 1 : 0 ) )
     {
     case 2: return launch3( trace3d_xm_fast_kernel<BLOCK,1,false>, grid, BLOCK, lds, p, st );
