@@ -265,10 +265,11 @@ void default_shape( nufi_ctx *c )
         const size_t per = size_t( c->st ) * elem;
         int chunk = int( std::min<size_t>( c->lds_cap, 64 * 1024 ) / per );
         chunk = std::max( 1, std::min( chunk, 1 ) );
-        // the autotuner's picks on 16^3 x 32^3 (two 1-slice buffers each)
+        // the autotuner's picks on 16^3 x 32^3 (two 1-slice buffers each); eight
+        // FP64 nodes a lane trace twice the node-steps per slice barrier of four
         if ( c->variant == TRACE_STRICT ) c->shape = launch_shape { 256, 1, chunk };
         else if ( c->variant == TRACE_FP32C ) c->shape = launch_shape { 512, 4, 1 };
-        else c->shape = launch_shape { 512, 4, chunk };
+        else c->shape = launch_shape { 512, 8, chunk };
     }
 }
 
@@ -1136,7 +1137,11 @@ int nufi_autotune( nufi_ctx *c, uint64_t n_probe, int repeats, int *block, int *
     else
     {
         if ( c->variant == TRACE_STRICT ) cand.push_back( { 256, 1, 1 } );
-        else for ( int b : blocks1 ) for ( int np : npts ) for ( int ch : { 1, 2 } ) cand.push_back( { b, np, ch } );
+        else
+        {
+            for ( int b : blocks1 ) for ( int np : npts ) for ( int ch : { 1, 2 } ) cand.push_back( { b, np, ch } );
+            if ( c->variant == TRACE_FAST ) for ( int ch : { 1, 2 } ) cand.push_back( { 512, 8, ch } );
+        }
     }
     const launch_shape saved = c->shape;
     const bool saved_timing = c->timing;

@@ -997,10 +997,11 @@ hipError_t launch3_npt( const trace3_params &p, int variant, int npt, uint32_t g
     }
     // power-of-two axes: the cell index wraps by a mask, positions every 16 slices
     const bool mask = ( p.Nx & ( p.Nx - 1 ) ) == 0 && ( p.Ny & ( p.Ny - 1 ) ) == 0 && ( p.Nz & ( p.Nz - 1 ) ) == 0;
-    const bool g16 = mask && npt == 4 && p.Nx == 16 && p.Ny == 16 && p.Nz == 16 && p.sy == 19 && p.sz == 19 * 19;
+    const bool g16 = mask && ( npt == 2 || npt == 4 || ( npt == 8 && variant == TRACE_FAST ) ) && p.Nx == 16 && p.Ny == 16 && p.Nz == 16 && p.sy == 19 && p.sz == 19 * 19;
     if ( variant == TRACE_FP32C )
     {
-        if ( g16 ) return launch3( trace3d_fp32c_kernel<BLOCK,4,true,16>, grid, BLOCK, lds, p, st );
+        if ( g16 && npt == 4 ) return launch3( trace3d_fp32c_kernel<BLOCK,4,true,16>, grid, BLOCK, lds, p, st );
+        if ( g16 ) return launch3( trace3d_fp32c_kernel<BLOCK,2,true,16>, grid, BLOCK, lds, p, st );
         switch ( npt * 2 + ( mask ? 1 : 0 ) )
         {
         case 2: return launch3( trace3d_fp32c_kernel<BLOCK,1,false>, grid, BLOCK, lds, p, st );
@@ -1012,9 +1013,13 @@ hipError_t launch3_npt( const trace3_params &p, int variant, int npt, uint32_t g
         default: return hipErrorInvalidValue;
         }
     }
-    // the 16^3 grid with its strides compiled in, at the default shape's node count
-    if ( g16 )
-        return launch3( trace3d_xm_fast_kernel<BLOCK,4,true,16>, grid, BLOCK, lds, p, st );
+    // the 16^3 grid with its strides compiled in (two and four nodes a lane)
+    if ( g16 && npt == 8 ) return launch3( trace3d_xm_fast_kernel<BLOCK,8,true,16>, grid, BLOCK, lds, p, st );
+    if ( g16 && npt == 4 ) return launch3( trace3d_xm_fast_kernel<BLOCK,4,true,16>, grid, BLOCK, lds, p, st );
+    if ( g16 ) return launch3( trace3d_xm_fast_kernel<BLOCK,2,true,16>, grid, BLOCK, lds, p, st );
+    // eight nodes a lane exist for 16^3 only; elsewhere four (the x-major grid and the
+    // partial sums do not depend on the nodes a lane holds, and 4 divides vgroup too)
+    if ( npt == 8 ) npt = 4;
     switch ( npt * 2 + ( mask ? 1 : 0 ) )
     {
     case 2: return launch3( trace3d_xm_fast_kernel<BLOCK,1,false>, grid, BLOCK, lds, p, st );
@@ -1034,7 +1039,8 @@ bool trace3d_shape_ok( int variant, int order, const launch_shape &s )
     if ( order != 4 || s.chunk < 1 ) return false;
     if ( variant == TRACE_STRICT )
         return s.npt == 1 && ( s.block == 64 || s.block == 256 );
-    return ( s.block == 256 || s.block == 512 || s.block == 1024 ) && ( s.npt == 1 || s.npt == 2 || s.npt == 4 );
+    return ( s.block == 256 || s.block == 512 || s.block == 1024 ) &&
+           ( s.npt == 1 || s.npt == 2 || s.npt == 4 || ( s.npt == 8 && variant == TRACE_FAST ) );
 }
 
 static uint64_t visited3( const trace3_params &p )

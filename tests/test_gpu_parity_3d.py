@@ -164,3 +164,34 @@ def test_rho_parity_3d_long_history(N, name, prec, variant, tol):
             err = rel(mean_free(rho), mean_free(exact))
             print(f"  n={n} rel err {err:.3e}", flush=True)
             assert err <= tol, (n, err)
+
+
+@pytest.mark.parametrize("prec,tol", [(nufi.FP64, 1e-12), (nufi.FP32C, 1e-6)])
+def test_rho_16cubed_compiled_strides_3d(prec, tol):
+    """A 16^3 grid runs kernels whose slice strides are compile-time constants
+    (two, four and, FP64, eight nodes a lane); other node counts run the
+    runtime-stride kernels.  rho at a step past the slice-15 re-wrap is the
+    same bit for bit across all of them and matches the oracle's exact sum."""
+    conf = nufi.config_3d((16, 16, 16), (8, 8, 8), 1 / 8, 3.0, (0.0, 4 * np.pi), (-6.0, 6.0),
+                          nufi.config.F0_LANDAU_3D, 0.05, 0.5)
+    n = 18
+    hist = O.new_history(conf, n + 1)
+    O.run(conf, 0, n, hist)
+    exact = O.rho(conf, n, hist, exact_sum=True)
+    shapes = [(512, 4, 1), (512, 2, 1), (1024, 2, 1), (256, 1, 1)]
+    if prec == nufi.FP64:
+        shapes = [(512, 8, 1), (512, 8, 2)] + shapes
+    out = []
+    with nufi.GpuScheduler(conf, precision=prec) as s:
+        s.upload_history(hist[: n * conf.stride_t()], 0, n)
+        for shape in shapes:
+            s.set_launch_shape(*shape)
+            rho = np.zeros(conf.n_space())
+            s.compute_rho(n)
+            s.download_rho(rho)
+            out.append(rho)
+            err = rel(mean_free(rho), mean_free(exact))
+            print(f"[3d 16^3 strides] {shape} rel err {err:.3e}", flush=True)
+            assert err <= tol, (shape, err)
+    for shape, r in zip(shapes[1:], out[1:]):
+        assert np.array_equal(r, out[0]), shape
