@@ -693,7 +693,11 @@ __device__ __forceinline__ void kick32( float &wh, float &wl, float q )
 
 // velocity-major: node r of a lane is make_node3's (wave task g, lane), the
 // wave's 64 lanes are 64 consecutive velocities of one spatial point
-template <int BLOCK, int NPT, bool MASK, int G = 0>
+// (G: compile-time grid edge as in step3_fast; CUBIC: dx = dy = dz, the kicks
+// need no axis ratio)
+constexpr int KICK_GENERAL = 0, KICK_ONE = 1, KICK_HALF = 2;
+
+template <int BLOCK, int NPT, bool MASK, int G = 0, bool CUBIC = false>
 __global__ __launch_bounds__( BLOCK ) __attribute__(( amdgpu_waves_per_eu( 4 ) ))
 void trace3d_fp32c_kernel( trace3_params p )
 {
@@ -739,6 +743,46 @@ void trace3d_fp32c_kernel( trace3_params p )
     const uint32_t st = p.st32, sy = G > 0 ? uint32_t( G + 3 ) : p.sy, sz = G > 0 ? uint32_t( ( G + 3 ) * ( G + 3 ) ) : p.sz;
     const uint32_t C = p.chunk;
 
+    // one slice of all NPT nodes of the lane: drift, locate, gradient, kick
+    auto trace_slice = [&]( const float *Cs, float hf, auto kick_form ) __attribute__(( always_inline ))
+    {
+        constexpr int KICK = decltype( kick_form )::value;
+        #pragma unroll
+        for ( int r = 0; r < NPT; ++r )
+        {
+            float tx, ty, tz;
+            const int i = drift_locate32<MASK>( fx, xh[r], xl[r], uh[r], ul[r], tx );
+            const int j = drift_locate32<MASK>( fy, yh[r], yl[r], vh[r], vl[r], ty );
+            const int k = drift_locate32<MASK>( fz, zh[r], zl[r], wh[r], wl[r], tz );
+            const float *W = Cs + __umul24( uint32_t( k ), sz ) + __umul24( uint32_t( j ), sy ) + i;
+            float buf[GATHER_PF + 1][4];
+            using Win = typename std::conditional<( G > 0 ), window_g<G, float>, window_rt<float>>::type;
+            const Win win = make_window<G>( W, sy, sz );
+            grad_prefetch<GATHER_PF>( win, buf );
+            float Bx[4], Dx[4], By[4], Dy[4], Bz[4], Dz[4];
+            cubic_bases_x6( tx, Bx, Dx );
+            cubic_bases_x6( ty, By, Dy );
+            cubic_bases_x6( tz, Bz, Dz );
+            float gx, gy, gz;
+            grad_rows<GATHER_PF>( win, buf, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
+            if ( KICK == KICK_ONE )
+            {
+                kick32( uh[r], ul[r], gx ); kick32( vh[r], vl[r], gy ); kick32( wh[r], wl[r], gz );
+            }
+            else if ( KICK == KICK_HALF )
+            {
+                kick32( uh[r], ul[r], 0.5f * gx ); kick32( vh[r], vl[r], 0.5f * gy ); kick32( wh[r], wl[r], 0.5f * gz );
+            }
+            else
+            {
+                const float hyh = hf * ryh, hyl = hf * ryl, hzh = hf * rzh, hzl = hf * rzl;
+                kick32( uh[r], ul[r], hf * gx );
+                kick32( vh[r], vl[r], fmaf( gy, hyh, gy * hyl ) );
+                kick32( wh[r], wl[r], fmaf( gz, hzh, gz * hzl ) );
+            }
+        }
+    };
+
     // chunk k: slices max(top - kC - C + 1, 0) .. top - kC.  With p.dma the
     // chunks alternate between two LDS buffers: chunk k+1 is copied straight
     // into the other buffer while chunk k is traced, and one barrier per chunk
@@ -774,11 +818,14 @@ void trace3d_fp32c_kernel( trace3_params p )
             stage3<BLOCK>( ldsf, p.hist32 + size_t( mlo ) * st, uint32_t( cnt ) * st );
             __syncthreads();
         }
-        for ( int jj = cnt - 1; jj >= 0; --jj )
+        // slice 0 takes the half kick: hf = 1/2, a power of two, so hf * q is
+        // exact.  On cubic cells ry = rz = 1 (hi 1, lo 0) and every kick is hf
+        // times the gradient exactly, so slice 0 is peeled off the loop and the
+        // slices m > 0 multiply by nothing (the same numbers as the general form).
+        const int jlast = ( CUBIC && mlo == 0 ) ? 1 : 0;
+        for ( int jj = cnt - 1; jj >= jlast; --jj )
         {
             const float *Cs = cur + size_t( jj ) * st;
-            const float hf = ( mlo + jj == 0 ) ? 0.5f : 1.0f;          // a power of two: hf * q is exact
-            const float hyh = hf * ryh, hyl = hf * ryl, hzh = hf * rzh, hzl = hf * rzl;
             if ( wrap3_due( mlo + jj ) )
             {
                 #pragma unroll
@@ -789,29 +836,10 @@ void trace3d_fp32c_kernel( trace3_params p )
                     frame32_rewrap<MASK>( fz, zh[r], zl[r], wh[r], wl[r] );
                 }
             }
-            #pragma unroll
-            for ( int r = 0; r < NPT; ++r )
-            {
-                float tx, ty, tz;
-                const int i = drift_locate32<MASK>( fx, xh[r], xl[r], uh[r], ul[r], tx );
-                const int j = drift_locate32<MASK>( fy, yh[r], yl[r], vh[r], vl[r], ty );
-                const int k = drift_locate32<MASK>( fz, zh[r], zl[r], wh[r], wl[r], tz );
-                const float *W = Cs + __umul24( uint32_t( k ), sz ) + __umul24( uint32_t( j ), sy ) + i;
-                float buf[GATHER_PF + 1][4];
-                using Win = typename std::conditional<( G > 0 ), window_g<G, float>, window_rt<float>>::type;
-                const Win win = make_window<G>( W, sy, sz );
-                grad_prefetch<GATHER_PF>( win, buf );
-                float Bx[4], Dx[4], By[4], Dy[4], Bz[4], Dz[4];
-                cubic_bases_x6( tx, Bx, Dx );
-                cubic_bases_x6( ty, By, Dy );
-                cubic_bases_x6( tz, Bz, Dz );
-                float gx, gy, gz;
-                grad_rows<GATHER_PF>( win, buf, Bx, Dx, By, Dy, Bz, Dz, gx, gy, gz );
-                kick32( uh[r], ul[r], hf * gx );
-                kick32( vh[r], vl[r], fmaf( gy, hyh, gy * hyl ) );
-                kick32( wh[r], wl[r], fmaf( gz, hzh, gz * hzl ) );
-            }
+            if ( CUBIC ) trace_slice( Cs, 1.0f, std::integral_constant<int, KICK_ONE>() );
+            else trace_slice( Cs, ( mlo + jj == 0 ) ? 0.5f : 1.0f, std::integral_constant<int, KICK_GENERAL>() );
         }
+        if ( jlast == 1 ) trace_slice( cur, 0.5f, std::integral_constant<int, KICK_HALF>() );     // wrap3_due(0) is false
         if ( p.dma ) dma_publish();
     }
 
@@ -1000,8 +1028,9 @@ hipError_t launch3_npt( const trace3_params &p, int variant, int npt, uint32_t g
     const bool g16 = mask && ( npt == 2 || npt == 4 || ( npt == 8 && variant == TRACE_FAST ) ) && p.Nx == 16 && p.Ny == 16 && p.Nz == 16 && p.sy == 19 && p.sz == 19 * 19;
     if ( variant == TRACE_FP32C )
     {
-        if ( g16 && npt == 4 ) return launch3( trace3d_fp32c_kernel<BLOCK,4,true,16>, grid, BLOCK, lds, p, st );
-        if ( g16 ) return launch3( trace3d_fp32c_kernel<BLOCK,2,true,16>, grid, BLOCK, lds, p, st );
+        const bool cubic = p.dx_inv == p.dy_inv && p.dx_inv == p.dz_inv;
+        if ( g16 && cubic && npt == 4 ) return launch3( trace3d_fp32c_kernel<BLOCK,4,true,16,true>, grid, BLOCK, lds, p, st );
+        if ( g16 && cubic ) return launch3( trace3d_fp32c_kernel<BLOCK,2,true,16,true>, grid, BLOCK, lds, p, st );
         switch ( npt * 2 + ( mask ? 1 : 0 ) )
         {
         case 2: return launch3( trace3d_fp32c_kernel<BLOCK,1,false>, grid, BLOCK, lds, p, st );
