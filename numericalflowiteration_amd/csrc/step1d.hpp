@@ -148,13 +148,13 @@ __device__ __forceinline__ void fp32c_rewrap( const frame1 &f, float &sh, float 
 }
 
 // the cell (in [-1, Nx]; POW2: wrapped by the mask, P0 being a multiple of
-// Nx) and t of the pair
+// Nx) and t of the pair.  sh >= P0 / 2 > 0, so truncation is the floor and
+// the fraction sh - floor(sh) is exact (one instruction).
 template <bool POW2>
 __device__ __forceinline__ int fp32c_locate( const frame1 &f, float sh, float sl, int mask, float &t )
 {
-    const float k = floorf( sh );
-    t = ( sh - k ) + sl;
-    return POW2 ? ( int( k ) & mask ) : int( k ) - f.P0i;
+    t = __builtin_amdgcn_fractf( sh ) + sl;
+    return POW2 ? ( int( sh ) & mask ) : int( sh ) - f.P0i;
 }
 
 // position (cells) and velocity (cells per step) of the pairs, in FP64

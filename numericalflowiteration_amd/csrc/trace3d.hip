@@ -659,9 +659,8 @@ __device__ __forceinline__ int drift_locate32( const frame32 &f, float &sh, floa
     sl = ( sl - wl ) + e;
     sh = y;
     if ( !MASK ) sh = fmaf( -f.N, floorf( ( sh - f.P0 ) * f.invN ), sh );   // exact
-    const float k = floorf( sh );
-    t = ( sh - k ) + sl;
-    int i = int( k );
+    t = __builtin_amdgcn_fractf( sh ) + sl;          // sh > 0: the fraction is exact and
+    int i = int( sh );                               // truncation is the floor
     if ( MASK ) i &= f.Ni - 1;
     else
     {
