@@ -166,14 +166,20 @@ def test_rho_parity_3d_long_history(N, name, prec, variant, tol):
             assert err <= tol, (n, err)
 
 
+@pytest.mark.parametrize("cubic", [True, False])
 @pytest.mark.parametrize("prec,tol", [(nufi.FP64, 1e-12), (nufi.FP32C, 1e-6)])
-def test_rho_16cubed_compiled_strides_3d(prec, tol):
+def test_rho_16cubed_compiled_strides_3d(prec, tol, cubic):
     """A 16^3 grid runs kernels whose slice strides are compile-time constants
     (two, four and, FP64, eight nodes a lane); other node counts run the
-    runtime-stride kernels.  rho at a step past the slice-15 re-wrap is the
-    same bit for bit across all of them and matches the oracle's exact sum."""
+    runtime-stride kernels.  On cubic cells the FP32C kernel takes its kicks as
+    the gradient itself; with y in [0, 8pi) (dy = 2 dx) it takes the general
+    form.  rho at a step past the slice-15 re-wrap is the same bit for bit
+    across all of them and matches the oracle's exact sum."""
     conf = nufi.config_3d((16, 16, 16), (8, 8, 8), 1 / 8, 3.0, (0.0, 4 * np.pi), (-6.0, 6.0),
                           nufi.config.F0_LANDAU_3D, 0.05, 0.5)
+    if not cubic:
+        conf.y_max = 8 * np.pi              # cos(0.5 y) stays periodic
+        conf = conf.derive()
     n = 18
     hist = O.new_history(conf, n + 1)
     O.run(conf, 0, n, hist)
@@ -191,7 +197,32 @@ def test_rho_16cubed_compiled_strides_3d(prec, tol):
             s.download_rho(rho)
             out.append(rho)
             err = rel(mean_free(rho), mean_free(exact))
-            print(f"[3d 16^3 strides] {shape} rel err {err:.3e}", flush=True)
+            print(f"[3d 16^3 strides] cubic={cubic} {shape} rel err {err:.3e}", flush=True)
             assert err <= tol, (shape, err)
     for shape, r in zip(shapes[1:], out[1:]):
         assert np.array_equal(r, out[0]), shape
+
+
+@pytest.mark.parametrize("prec", [nufi.FP64, nufi.FP32C])
+def test_autotune_3d_keeps_rho(prec):
+    """The 3D autotuner times every built shape (FP64 also eight nodes a lane)
+    on a 16^3 grid, keeps one the library accepts, and rho at the probe step
+    is the same bit for bit before and after."""
+    conf = nufi.config_3d((16, 16, 16), (8, 8, 8), 1 / 8, 1.0, (0.0, 4 * np.pi), (-6.0, 6.0),
+                          nufi.config.F0_LANDAU_3D, 0.05, 0.5)
+    n = 6
+    hist = O.new_history(conf, n + 1)
+    O.run(conf, 0, n, hist)
+    with nufi.GpuScheduler(conf, precision=prec) as s:
+        s.upload_history(hist[: n * conf.stride_t()], 0, n)
+        before = np.zeros(conf.n_space())
+        s.compute_rho(n)
+        s.download_rho(before)
+        shape, ms = s.autotune(n_probe=n, repeats=1)
+        print(f"[3d autotune] {'fp64' if prec == nufi.FP64 else 'fp32c'} picked {shape} ({ms:.3f} ms)", flush=True)
+        assert ms > 0
+        assert s.launch_shape() == shape
+        after = np.zeros(conf.n_space())
+        s.compute_rho(n)
+        s.download_rho(after)
+    assert np.array_equal(after, before)
