@@ -33,8 +33,9 @@
 //              Ex ~ sum_k R_k Bz_k, Ey ~ sum_k Q_k Bz_k, Ez ~ sum_k P_k Dz_k (12)
 //   fp32c  : as fast in FP32 from an FP32 copy of the slice, positions (cell
 //            units) and velocities (cells per step, w = dt dx_inv u) as hi + lo
-//            pairs held at offsets (one binade per hi part), updated by ordered
-//            Fast2Sums; f0 and the rho sums in FP64.  The copy is scaled in
+//            pairs held at one offset (one binade for every hi part): exact
+//            drifts of the hi parts, ordered Fast2Sum kicks; f0 and the rho
+//            sums in FP64.  The copy is scaled in
 //            FP64 by the kick in cells per step over 72 before its one rounding
 //            and the bases are 6B and 2D, so no FP32-rounded constant (1/6,
 //            -dt/dx) multiplies the kicks: such a constant biases every node's
@@ -635,12 +636,13 @@ void trace3d_xm_fast_kernel( trace3_params p )
 
 // ---- FP32 compensated --------------------------------------------------------
 // Positions (cells) and velocities (cells per step) are unevaluated pairs
-// hi + lo whose hi parts carry a per-axis offset, P0 + s and W0 + w, chosen on
-// the host so that every hi value of the run stays inside one binade.  Then the
-// offsets come off exactly, the hi parts dominate every increment, so the drift
-// and the kick are ordered Fast2Sums (3 operations, exact) rather than TwoSums
-// (6), and a wrap by a multiple of N is exact.  The drift leaves its rounding in
-// the lo part unnormalised; both pairs are renormalised every WRAP3 slices.
+// hi + lo whose hi parts carry a per-axis offset, P0 + s and W0 + w (W0 = P0,
+// fp32c_frame's shared form), chosen on the host so that every hi value of the
+// run stays inside one binade, the same for both.  Then the offsets come off exactly, all hi parts are
+// multiples of one ulp, so the drift of the hi parts is exact (sh - d stays in
+// the binade), the kick is an ordered Fast2Sum (3 operations, exact) rather
+// than a TwoSum (6), and a wrap by a multiple of N is exact.  The lo parts move
+// unnormalised; both pairs are renormalised every WRAP3 slices.
 struct frame32
 {
     float P0, W0, N, invN;
@@ -654,10 +656,8 @@ template <bool MASK>
 __device__ __forceinline__ int drift_locate32( const frame32 &f, float &sh, float &sl, float wh, float wl, float &t )
 {
     const float d = wh - f.W0;                       // exact: wh and W0 share a binade
-    const float y = sh - d;                          // Fast2Sum, |sh| > |d|
-    const float e = ( sh - y ) - d;
-    sl = ( sl - wl ) + e;
-    sh = y;
+    sh = sh - d;                                     // exact: sh, d on the frames' one ulp grid
+    sl = sl - wl;
     if ( !MASK ) sh = fmaf( -f.N, floorf( ( sh - f.P0 ) * f.invN ), sh );   // exact
     t = __builtin_amdgcn_fractf( sh ) + sl;          // sh > 0: the fraction is exact and
     int i = int( sh );                               // truncation is the floor

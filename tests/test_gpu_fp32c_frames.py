@@ -1,6 +1,7 @@
 """The FP32C kernels where their offset frames are tightest (the hi parts of
-positions and velocities held at per-axis offsets so that drift and kick are
-ordered Fast2Sums, csrc/step1d.hpp and csrc/trace3d.hip): many cells per
+positions and velocities held at one per-axis offset so that the drift of the
+hi parts is exact and the kick an ordered Fast2Sum, csrc/step1d.hpp and
+csrc/trace3d.hip): many cells per
 step, velocities reversed by a nonlinear field, grids that are not powers of
 two (a wrap every step) and non-cubic 3D cells.  Per-step mean-free rho on
 the oracle's own history against the oracle with the exact v-sum
