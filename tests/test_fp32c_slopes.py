@@ -6,8 +6,9 @@ level of the coefficients, so the four FP32 slope values have to sum to
 exactly zero; plain FP32 rounding of the four slopes leaves a residual of
 about 2^-24 that multiplies the local level of phi (it gave 2e-6 relative
 energy error on 3D Landau).  Checked here: the exact zero sum for t over
-[-1e-3, 1 + 1e-3] (the lo part of a position pair can push t slightly out of
-[0, 1)), the slopes and values against the closed forms in FP64
+[-1e-2, 1 + 1e-2] (the lo part of a position pair, unnormalised for up to 16
+slices, can push t out of [0, 1) by at most 128 ulp of the frame offset:
+0.004 cells on 3D Landau), the slopes and values against the closed forms in FP64
 (nufi/splines.hpp:39-110 at order 4, times 2 and 6), and that the naive FP32
 slopes do not sum to zero (so the test can fail)."""
 from fractions import Fraction
@@ -46,9 +47,9 @@ def closed_form(t):
 
 def sample_t():
     rng = np.random.default_rng(7)
-    t = rng.uniform(-1e-3, 1 + 1e-3, 20000).astype(np.float32)
+    t = rng.uniform(-1e-2, 1 + 1e-2, 20000).astype(np.float32)
     edges = np.array([0.0, 2.0**-24, -2.0**-24, 1e-8, -1e-8, 1 / 3, 0.5, 2 / 3, 1 - 2.0**-24, 1.0,
-                      1 + 2.0**-23, -1e-3, 1 + 1e-3], dtype=np.float32)
+                      1 + 2.0**-23, -1e-3, 1 + 1e-3, -1e-2, 1 + 1e-2], dtype=np.float32)
     return np.concatenate([t, edges])
 
 
