@@ -536,6 +536,18 @@ def read_traffic(name):
         return None
 
 
+def read_executed_flops(name, kernel, char_steps_per_launch):
+    """FLOPs the kernel executes per char-step (64 x SQ_INSTS_VALU_FLOPS_FP64 over its mean duration, times
+    that duration, over the char-steps of a launch) from the committed rocprofv3 --pmc summary, if any."""
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
+            counters = json.load(f)["counters"]
+        c = next(v for k, v in counters.items() if kernel in k)
+        return c["fp64_tflops"] * 1e12 * c["mean_us"] * 1e-6 / char_steps_per_launch
+    except Exception:
+        return None
+
+
 def summarize_profiles(outdir, tag, commands):
     """Turn rocprofv3 result databases (kernel trace + FETCH_SIZE / WRITE_SIZE
     passes + the copy-kernel calibration passes) into the committed summaries
@@ -767,6 +779,15 @@ def main():
         "peak_measured": read_measured_peak("fp64_fma"),
         "peak_measured_note": "FMA-chain probe on the box (profiles/peaks_mi355x.md); peak stays the published figure",
     })
+    if prim_prec == "fp64":
+        ex = read_executed_flops("trace3d_landau3d_pmc.json", "trace3d_xm_fast_kernel", prim["char_steps_per_step"])
+        if ex is not None:
+            roofline["executed_flops_per_char_step"] = ex
+            roofline["achieved_executed"] = ex * prim["char_steps_per_step"] / (roofline["avg_launch_ms"] * 1e-3) / 1e12
+            roofline["executed_note"] = ("FP64 FLOPs the kernel executes per char-step (64 x SQ_INSTS_VALU_FLOPS_FP64 "
+                                         "from the committed --pmc summary) times this run's char-step rate: "
+                                         "the closed-form 6B/2D bases and folded kick constants execute fewer "
+                                         "FLOPs than the algorithmic 475; frac stays algorithmic")
     if cpu is not None:
         cpu["cores_physical_lscpu"] = phys
         cpu["cpus_in_affinity_mask"] = avail
