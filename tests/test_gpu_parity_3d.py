@@ -129,8 +129,9 @@ def test_rho_bit_identical_across_launch_shapes_3d(prec):
     out = []
     with nufi.GpuScheduler(conf, precision=prec) as s:
         s.upload_history(hist[: 5 * st], 0, 5)
-        extra = (512, 4, 16) if prec == nufi.FP32C else (512, 4, 8)      # the LDS holds 8 FP64 slices
-        for shape in [(256, 1, 1), (512, 4, 2), (256, 2, 1), (1024, 2, 1), (1024, 1, 1), extra]:
+        # FP64: eight slices of this grid fit the LDS, and eight nodes a lane run as four off 16^3
+        extra = [(512, 4, 16)] if prec == nufi.FP32C else [(512, 4, 8), (512, 8, 1)]
+        for shape in [(256, 1, 1), (512, 4, 2), (256, 2, 1), (1024, 2, 1), (1024, 1, 1)] + extra:
             s.set_launch_shape(*shape)
             rho = np.zeros(conf.n_space())
             s.compute_rho(4)
@@ -138,6 +139,9 @@ def test_rho_bit_identical_across_launch_shapes_3d(prec):
             out.append(rho)
         with pytest.raises(nufi.NufiError):
             s.set_launch_shape(512, 3, 1)          # not built
+        if prec == nufi.FP32C:
+            with pytest.raises(nufi.NufiError):
+                s.set_launch_shape(512, 8, 1)      # FP64 only
     for r in out[1:]:
         assert np.array_equal(r, out[0])
 
