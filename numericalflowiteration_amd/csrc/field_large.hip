@@ -1,5 +1,5 @@
 // field_large.hip — the field solve for grids whose two complex buffers do not
-// fit one workgroup's LDS (more than 5120 points): the same operator as
+// fit one workgroup's LDS (more than 5104 points): the same operator as
 // field.hip (rho -> spectral energy -> Poisson x fit multiplier -> slice; or
 // the Poisson-only / fit-only multipliers) with the transforms done as one
 // direct DFT pass per axis between two global complex buffers, one thread per
