@@ -8,7 +8,8 @@
   pattern).
 * 3D Landau, 16^3 x 32^3 nodes: nine device steps, then rho of steps 1, 4 and
   8 against the oracle at a sample of 48 spatial points (the oracle's full 3D
-  step at this size takes minutes).
+  step at this size takes minutes); and the whole FP32C run against the whole
+  FP64 run (the FP32C form's energy target, 1e-6 at every step).
 
 Tolerances: FP64 <= 1e-12 relative (mean-free rho, slice, energy); FP32C
 <= 1e-6.
@@ -99,3 +100,32 @@ def test_landau3d_fullsize_isolated_rho(prec, tol):
             print(f"[full size] 3D Landau precision={prec} n={n}: rho rel err at {pts.size} points {err:.3e}",
                   flush=True)
             assert err <= tol, (n, err)
+
+
+def test_landau3d_fp32c_run_meets_the_energy_target():
+    """The accuracy target of the FP32-compensated form on the 3D3V config:
+    over the whole 3D Landau run (16^3 x 32^3, dt = 1/8, T = 20, 161 steps) the
+    electric energy of the FP32C run stays within 1e-6 relative of the FP64
+    run's at every step (the maximum over n), and so does E at the nodes.  The
+    FP64 run is the reference here; it matches the oracle to 1e-12 (the
+    isolated-step tests above).  rho of the mean-free perturbation is reported
+    and held to 5e-6: at the minima of the damped energy the perturbation is
+    ~1e-5 of its n = 0 size, and there the FP32C rho error reaches ~1.5e-6
+    relative to it.  The kernels are deterministic, so these figures do not
+    depend on the box."""
+    from numericalflowiteration_amd import diagnostics as D
+    conf = nufi.landau_3d()
+    runs = {}
+    for prec, name in ((nufi.FP64, "fp64"), (nufi.FP32C, "fp32c")):
+        print(f"[full size] 3D Landau run to T, {name}", flush=True)
+        with nufi.GpuScheduler(conf, precision=prec) as s:
+            e = s.run()
+            runs[name] = (s.history(), e)
+    errs = D.run_errors(conf, runs["fp32c"][0], runs["fp32c"][1], runs["fp64"][0], runs["fp64"][1])
+    print(f"[full size] 3D Landau FP32C vs FP64 over {errs['steps_compared']} steps: energy "
+          f"{errs['energy_max_rel_err']:.3e} (step {errs['energy_err_at_step']}), E {errs['E_max_rel_err']:.3e}, "
+          f"rho {errs['rho_max_rel_err']:.3e}", flush=True)
+    assert errs["steps_compared"] == conf.Nt + 1
+    assert errs["energy_max_rel_err"] <= 1e-6
+    assert errs["E_max_rel_err"] <= 1e-6
+    assert errs["rho_max_rel_err"] <= 5e-6
