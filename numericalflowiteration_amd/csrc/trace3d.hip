@@ -197,7 +197,8 @@ __device__ __forceinline__ void cubic_bases_x6( double t, double B[4], double D[
 // each rounded once onto multiples of 2^-23 (an FMA that adds 1 or -1, which
 // puts the result at magnitude [1,2] or a little above, then that 1 taken back
 // exactly), and the third slope (a^2 - t^2) - D_1, whose partial results stay
-// below 2 in magnitude, is exact.
+// below 2 in magnitude, is exact.  The values need no such care (the slopes
+// of the other axes cancel their level): B_2 is B_1 at 1 - t.
 __device__ __forceinline__ void cubic_bases_x6( float t, float B[4], float D[4] )
 {
     const float a = 1.0f - t;
@@ -206,7 +207,7 @@ __device__ __forceinline__ void cubic_bases_x6( float t, float B[4], float D[4] 
     B[0] = a2 * a;
     B[3] = t2 * t;
     B[1] = fmaf( t2, fmaf( 3.0f, t, -6.0f ), 4.0f );            // 3t^3 - 6t^2 + 4
-    B[2] = ( ( 6.0f - B[0] ) - B[1] ) - B[3];
+    B[2] = fmaf( a2, fmaf( 3.0f, a, -6.0f ), 4.0f );            // B_1(1-t)
     D[0] = -a2;
     D[3] = t2;
     D[1] = fmaf( t, fmaf( 3.0f, t, -4.0f ), -1.0f ) + 1.0f;     // 3t^2 - 4t on the 2^-23 grid

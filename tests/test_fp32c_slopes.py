@@ -32,7 +32,7 @@ def bases_fp32(t):
     B0 = f32(a2 * a)
     B3 = f32(t2 * t)
     B1 = fmaf(t2, fmaf(f32(3), t, f32(-6)), f32(4))
-    B2 = f32(f32(f32(f32(6) - B0) - B1) - B3)
+    B2 = fmaf(a2, fmaf(f32(3), a, f32(-6)), f32(4))          # B_1(1 - t)
     D1 = f32(fmaf(t, fmaf(f32(3), t, f32(-4)), f32(-1)) + f32(1))
     D2 = f32(f32(a2 - t2) - D1)
     return (B0, B1, B2, B3), (f32(-a2), D1, D2, t2)
