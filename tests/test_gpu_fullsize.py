@@ -110,7 +110,7 @@ def test_landau3d_fp32c_run_meets_the_energy_target():
     FP64 run is the reference here; it matches the oracle to 1e-12 (the
     isolated-step tests above).  rho of the mean-free perturbation is reported
     and held to 5e-6: at the minima of the damped energy the perturbation is
-    ~1e-5 of its n = 0 size, and there the FP32C rho error reaches ~1.5e-6
+    ~1e-5 of its n = 0 size, and there the FP32C rho error reaches ~1.7e-6
     relative to it.  The kernels are deterministic, so these figures do not
     depend on the box."""
     from numericalflowiteration_amd import diagnostics as D
